@@ -1,0 +1,130 @@
+// recon_common.hpp -- host-side state, error reporting and the float32 voxel
+// math shared by the gfx950 kernels of the TSDF fusion path.
+//
+// Arithmetic contract (identical in every kernel and in the CPU oracle):
+// IEEE float32, one rounding per operation in the order written, no FMA
+// contraction (translation units carry `#pragma clang fp contract(off)` and
+// are built with -ffp-contract=off), correctly rounded division, denormals
+// kept.  The operation order is the reference's:
+//   voxel centre     ((v + 0.5f) * vs) + off                tsdf_volume.cu:117-119
+//   pose -> matrix   SE3(float4 quat, float3 t)              se3.cuh:42-57,124-140
+//   world -> camera  ((r0*x + r1*y) + r2*z) + t              se3.cuh:82-100
+//   camera -> pixel  (int)(((x * fx) / z) + cx)              camera_util.cu:7-19
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "recon/recon.h"
+
+namespace recon {
+
+// ---- error reporting -------------------------------------------------------
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+int hip_fail(hipError_t e, const char* what);
+
+#define RECON_HIP_TRY(expr, what)                         \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return ::recon::hip_fail(_e, what); \
+    } while (0)
+
+// Checks that the current device is a gfx950 part; caches the answer per
+// device.  Returns RECON_OK or RECON_ERR_DEVICE.
+int require_gfx950();
+
+// ---- pose ------------------------------------------------------------------
+// {qw,qx,qy,qz,tx,ty,tz} (double) -> float32 row-major R and t, narrowing the
+// way tsdf_volume.cu:271-272 does before building SE3<float>.
+void pose_to_matrix_f32(const double pose[7], float R[9], float t[3]);
+bool pose_is_finite(const double pose[7]);
+
+// ---- per-frame kernel parameters ------------------------------------------
+// Conservative tile culling: a voxel can only be updated when its camera
+// point p satisfies a_k . p + b_k > 0 for six half-spaces (z > 0, the four
+// image borders u > -1, u < cols, v > -1, v < rows, and z < max_depth +
+// truncation).  In world coordinates each is n_k . c + k_k > 0; `plane`
+// holds (n_k, k_k + margin_k), margin_k = 1e-3 of the half-space's magnitude
+// over the volume, three orders above float32 rounding of the per-voxel
+// test, so a tile whose voxel-centre box has max_c(n.c + k + margin) < 0
+// holds no voxel that passes.
+struct FrameParams {
+    float R[9];
+    float t[3];
+    float plane[6][4];
+    const float* depth;
+    int32_t pitch;   // floats per depth row
+    int32_t pad_;
+};
+
+struct VolumeParams {
+    int32_t gx, gy, gz;
+    int32_t tiles_x, tiles_y, tiles_z;
+    float vs[3];
+    float off[3];
+    float fx, fy, cx, cy;
+    float trunc, maxw, maxd;
+    int32_t cols, rows;
+    float fcols, frows;
+};
+
+constexpr int kMaxFusedFrames = 16;
+
+// Fills FrameParams for one frame of a volume.
+void make_frame_params(const VolumeParams& vp, const double pose[7], const float* depth,
+                       int32_t pitch_floats, FrameParams* fp);
+
+}  // namespace recon
+
+// ---- the handle -------------------------------------------------------------
+struct recon_tsdf {
+    recon_tsdf_config cfg;
+    recon_tsdf_info info;
+    recon::VolumeParams vp;  // cols/rows filled per call
+    float* D = nullptr;
+    float* W = nullptr;
+    int device = 0;
+    unsigned long long* update_counter = nullptr;
+    // marching-cubes scratch, grown on demand
+    void* mc_scratch = nullptr;
+    size_t mc_scratch_bytes = 0;
+    uint64_t version = 0;      // bumped by every write to D
+    uint64_t mc_version = ~0ull;  // version the cached row offsets belong to
+    int64_t mc_total = -1;
+};
+
+namespace recon {
+
+// Validates a configuration and derives voxel size / truncation / max depth
+// (tsdf_volume.cu:194-199).  Host only, no HIP calls.
+int describe(const recon_tsdf_config* cfg, recon_tsdf_info* info);
+
+// Marching cubes over a dense field; `scratch` / `scratch_bytes` may point
+// at a reusable buffer owned by the caller (grown by the callee).
+int mc_run(const float* field, const int32_t grid[3], const float vs[3], const float off[3],
+           float* verts, int32_t* tris, int64_t capacity, int64_t* total, bool count_only,
+           void** scratch, size_t* scratch_bytes, hipStream_t stream);
+
+}  // namespace recon
+
+// ---- device math --------------------------------------------------------------
+namespace recon {
+
+__device__ __forceinline__ float voxel_centre(int v, float vs, float off) {
+    return (((float)v + 0.5f) * vs) + off;
+}
+
+// Pixel index along one axis of camera point (pa, pz): s = ((pa*f)/pz) + c.
+// Valid iff -1 < s < size, which equals 0 <= (int)s < size for finite s.
+__device__ __forceinline__ bool pixel_index(float pa, float f, float pz, float c, float size,
+                                            int& out) {
+    const float s = ((pa * f) / pz) + c;
+    if (!(s > -1.0f && s < size)) return false;
+    out = (int)s;
+    return true;
+}
+
+}  // namespace recon

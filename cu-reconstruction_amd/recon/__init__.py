@@ -1,0 +1,228 @@
+"""Python front of the MI355X TSDF fusion path (librecon.so).
+
+`TSDFVolume` mirrors the reference's TSDFVolume / generate_triangular_surface
+API (cu-Reconstruction modules/include/cuda/tsdf_volume.cuh:31-105,
+modules/include/cuda/marching_cubes.cuh:13-15) over the C-ABI.  Device
+buffers are torch tensors on a ROCm device; torch is plumbing only, every
+computation runs in librecon.so's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import ffi
+from .ffi import ReconError, TsdfConfig, TsdfInfo, DepthView, check
+
+__all__ = ["TSDFVolume", "make_config", "depth_view", "depth_to_points", "mc_extract", "mc_count",
+           "mc_class_histogram", "ReconError", "TsdfConfig", "TsdfInfo", "ffi"]
+
+
+def _torch():
+    import torch  # noqa: F401  (imported before librecon.so so one HIP runtime is shared)
+    return torch
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+def make_config(grid, voxel_size=None, physical_size=None, offset=(0.0, 0.0, 0.0),
+                max_weight=128.0, intrinsics=(525.0, 525.0, 319.5, 239.5), image=(640, 480),
+                truncation=0.0, max_depth=0.0) -> TsdfConfig:
+    """TSDFVolumeConfig (tsdf_volume.cuh:17-29); give voxel_size or physical_size."""
+    cfg = TsdfConfig()
+    grid = [int(g) for g in grid]
+    if physical_size is None:
+        vs = voxel_size if np.ndim(voxel_size) else [voxel_size] * 3
+        physical_size = [float(np.float32(v) * np.float32(g)) for v, g in zip(vs, grid)]
+    cfg.voxel_grid_size = ffi.i32x3(grid)
+    cfg.physical_size = ffi.f32x3(physical_size)
+    cfg.global_offset = ffi.f32x3(offset)
+    cfg.max_weight = float(max_weight)
+    cfg.focal_x, cfg.focal_y, cfg.cx, cfg.cy = [float(v) for v in intrinsics]
+    cfg.camera_width, cfg.camera_height = int(image[0]), int(image[1])
+    cfg.max_cell_size = 0.0
+    cfg.truncation = float(truncation)
+    cfg.max_depth = float(max_depth)
+    return cfg
+
+
+def describe(cfg: TsdfConfig) -> TsdfInfo:
+    info = TsdfInfo()
+    check(ffi.load().recon_tsdf_describe(C.byref(cfg), C.byref(info)))
+    return info
+
+
+def depth_view(t) -> DepthView:
+    """View of a 2-D float32 device tensor (row stride may exceed cols)."""
+    torch = _torch()
+    if t.dtype != torch.float32 or t.dim() != 2 or t.stride(1) != 1:
+        raise ValueError("depth must be a 2-D float32 tensor with unit column stride")
+    return DepthView(t.data_ptr(), t.stride(0) * 4, t.shape[0], t.shape[1])
+
+
+class TSDFVolume:
+    """Dense float32 TSDF + weight volume on the current device."""
+
+    def __init__(self, cfg: TsdfConfig):
+        _torch()
+        self._lib = ffi.load()
+        h = C.c_void_p()
+        check(self._lib.recon_tsdf_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.cfg = cfg
+        self.info = TsdfInfo()
+        check(self._lib.recon_tsdf_get_info(self._h, C.byref(self.info)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.recon_tsdf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def grid(self):
+        return tuple(self.info.grid)
+
+    @property
+    def shape(self):  # numpy order
+        g = self.info.grid
+        return (g[2], g[1], g[0])
+
+    @property
+    def num_voxels(self) -> int:
+        return int(self.info.num_voxels)
+
+    @property
+    def truncation(self) -> float:
+        return float(self.info.truncation)
+
+    def distances_ptr(self) -> int:
+        return self._lib.recon_tsdf_distances(self._h)
+
+    def weights_ptr(self) -> int:
+        return self._lib.recon_tsdf_weights(self._h)
+
+    def reset(self, stream=None):
+        check(self._lib.recon_tsdf_reset(self._h, _stream(stream)))
+
+    def integrate(self, depth, pose, stream=None, rgb=None):
+        """TSDFVolume::integrate (tsdf_volume.cu:257-290); async on the stream."""
+        dv = depth_view(depth)
+        rgb_ptr = rgb.data_ptr() if rgb is not None else None
+        rgb_step = rgb.stride(0) if rgb is not None else 0
+        check(self._lib.recon_tsdf_integrate(self._h, C.byref(dv), rgb_ptr, rgb_step,
+                                             ffi.pose_array(pose), _stream(stream)))
+
+    def integrate_frames(self, depths, poses, stream=None):
+        """All frames of a (F, H, W) float32 tensor in one sweep per 16 frames."""
+        if depths.dim() != 3 or depths.stride(2) != 1:
+            raise ValueError("depths must be (F, H, W) with unit column stride")
+        poses = np.asarray(poses, dtype=np.float64).reshape(-1, 7)
+        if poses.shape[0] != depths.shape[0]:
+            raise ValueError("one pose per frame")
+        dv = DepthView(depths.data_ptr(), depths.stride(1) * 4, depths.shape[1], depths.shape[2])
+        check(self._lib.recon_tsdf_integrate_frames(
+            self._h, C.byref(dv), depths.stride(0) * 4, int(depths.shape[0]),
+            ffi.pose_array(poses.ravel()), _stream(stream)))
+
+    def set_update_counter(self, counter_tensor):
+        ptr = counter_tensor.data_ptr() if counter_tensor is not None else None
+        check(self._lib.recon_tsdf_set_update_counter(self._h, ptr))
+
+    def download(self, stream=None):
+        D = np.empty(self.shape, np.float32)
+        W = np.empty(self.shape, np.float32)
+        check(self._lib.recon_tsdf_download(self._h, D.ctypes.data, W.ctypes.data, _stream(stream)))
+        return D, W
+
+    def upload(self, D=None, W=None, stream=None):
+        Dp = np.ascontiguousarray(D, np.float32) if D is not None else None
+        Wp = np.ascontiguousarray(W, np.float32) if W is not None else None
+        for a in (Dp, Wp):
+            if a is not None and a.size != self.num_voxels:
+                raise ValueError("array size does not match the volume")
+        check(self._lib.recon_tsdf_upload(self._h, Dp.ctypes.data if Dp is not None else None,
+                                          Wp.ctypes.data if Wp is not None else None,
+                                          _stream(stream)))
+
+    def count_triangles(self, stream=None) -> int:
+        n = C.c_int64()
+        check(self._lib.recon_tsdf_count_triangles(self._h, C.byref(n), _stream(stream)))
+        return int(n.value)
+
+    def extract_mesh(self, verts=None, tris=None, stream=None):
+        """generate_triangular_surface (marching_cubes.cu:191-236) as a
+        triangle soup: verts (T, 3, 3) float32, tris (T, 3) int32."""
+        torch = _torch()
+        if verts is None:
+            n = self.count_triangles(stream)
+            verts = torch.empty((max(n, 1), 3, 3), dtype=torch.float32, device="cuda")
+            tris = torch.empty((max(n, 1), 3), dtype=torch.int32, device="cuda")
+        cap = verts.shape[0]
+        n = C.c_int64()
+        check(self._lib.recon_tsdf_extract_mesh(self._h, verts.data_ptr(),
+                                                tris.data_ptr() if tris is not None else None,
+                                                cap, C.byref(n), _stream(stream)))
+        t = int(n.value)
+        return verts[:t], (tris[:t] if tris is not None else None)
+
+
+def _grid_args(grid, voxel_size, offset):
+    return ffi.i32x3(grid), ffi.f32x3(voxel_size), ffi.f32x3(offset)
+
+
+def mc_count(field, grid, voxel_size, offset, stream=None) -> int:
+    g, vs, off = _grid_args(grid, voxel_size, offset)
+    n = C.c_int64()
+    check(ffi.load().recon_mc_count(field.data_ptr(), g, vs, off, C.byref(n), _stream(stream)))
+    return int(n.value)
+
+
+def mc_extract(field, grid, voxel_size, offset, capacity=None, stream=None, with_tris=True):
+    torch = _torch()
+    if capacity is None:
+        capacity = mc_count(field, grid, voxel_size, offset, stream)
+    verts = torch.empty((max(capacity, 1), 3, 3), dtype=torch.float32, device=field.device)
+    tris = torch.empty((max(capacity, 1), 3), dtype=torch.int32, device=field.device) if with_tris else None
+    g, vs, off = _grid_args(grid, voxel_size, offset)
+    n = C.c_int64()
+    st = ffi.load().recon_mc_extract(field.data_ptr(), g, vs, off, verts.data_ptr(),
+                                     tris.data_ptr() if tris is not None else None, capacity,
+                                     C.byref(n), _stream(stream))
+    check(st)
+    t = int(n.value)
+    return verts[:t], (tris[:t] if tris is not None else None)
+
+
+def mc_class_histogram(field, grid, stream=None):
+    torch = _torch()
+    hist = torch.zeros(256, dtype=torch.int64, device=field.device)
+    check(ffi.load().recon_mc_class_histogram(field.data_ptr(), ffi.i32x3(grid), hist.data_ptr(),
+                                              _stream(stream)))
+    return hist
+
+
+def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, stream=None):
+    """cudaPointCloud::extract_points (cuda_point_cloud.cu:179-226) for one
+    frame: returns xyz (H*W, 3) float32, valid (H*W,) uint8 and optionally
+    colours (H*W, 3) uint8."""
+    torch = _torch()
+    dv = depth_view(depth)
+    n = depth.shape[0] * depth.shape[1]
+    xyz = torch.empty((n, 3), dtype=torch.float32, device=depth.device)
+    valid = torch.empty((n,), dtype=torch.uint8, device=depth.device)
+    colors = torch.empty((n, 3), dtype=torch.uint8, device=depth.device) if with_colors else None
+    fx, fy, cx, cy = [float(v) for v in intrinsics]
+    check(ffi.load().recon_depth_to_points(
+        C.byref(dv), fx, fy, cx, cy, ffi.pose_array(transform),
+        rgb.data_ptr() if rgb is not None else None, rgb.stride(0) if rgb is not None else 0,
+        xyz.data_ptr(), valid.data_ptr(), colors.data_ptr() if colors is not None else None,
+        _stream(stream)))
+    return (xyz, valid, colors) if with_colors else (xyz, valid)

@@ -1,0 +1,149 @@
+"""ctypes binding of the C-ABI in include/recon/recon.h.
+
+This is the binding a Python caller of the TSDF fusion path uses; device
+memory is supplied by the caller (torch tensors in the tests and the
+benchmark) as raw pointers, and streams as hipStream_t values.  The product
+path is librecon.so: there is no fallback, a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parents[1]
+LIB_DIR = PKG / "lib"
+
+OK = 0
+ERR_NULL = -1
+ERR_SIZE = -2
+ERR_CONFIG = -3
+ERR_OOM = -4
+ERR_HIP = -5
+ERR_CAPACITY = -6
+ERR_RANGE = -7
+ERR_DEVICE = -8
+
+STATUS_NAMES = {OK: "OK", ERR_NULL: "ERR_NULL", ERR_SIZE: "ERR_SIZE", ERR_CONFIG: "ERR_CONFIG",
+                ERR_OOM: "ERR_OOM", ERR_HIP: "ERR_HIP", ERR_CAPACITY: "ERR_CAPACITY",
+                ERR_RANGE: "ERR_RANGE", ERR_DEVICE: "ERR_DEVICE"}
+
+
+class TsdfConfig(C.Structure):
+    _fields_ = [
+        ("voxel_grid_size", C.c_int32 * 3),
+        ("physical_size", C.c_float * 3),
+        ("global_offset", C.c_float * 3),
+        ("max_weight", C.c_float),
+        ("focal_x", C.c_float), ("focal_y", C.c_float),
+        ("cx", C.c_float), ("cy", C.c_float),
+        ("camera_width", C.c_uint32), ("camera_height", C.c_uint32),
+        ("max_cell_size", C.c_float),
+        ("truncation", C.c_float),
+        ("max_depth", C.c_float),
+    ]
+
+
+class TsdfInfo(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32 * 3),
+        ("voxel_size", C.c_float * 3),
+        ("global_offset", C.c_float * 3),
+        ("truncation", C.c_float),
+        ("max_weight", C.c_float),
+        ("max_depth", C.c_float),
+        ("intrinsics", C.c_float * 4),
+        ("num_voxels", C.c_int64),
+    ]
+
+
+class DepthView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("step", C.c_size_t), ("rows", C.c_int32),
+                ("cols", C.c_int32)]
+
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_pi64 = C.POINTER(C.c_int64)
+_d7 = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every function declared in include/recon/recon.h
+SIGNATURES = {
+    "recon_last_error": (C.c_char_p, []),
+    "recon_abi_version": (C.c_int, []),
+    "recon_tsdf_describe": (C.c_int, [C.POINTER(TsdfConfig), C.POINTER(TsdfInfo)]),
+    "recon_tsdf_create": (C.c_int, [C.POINTER(TsdfConfig), C.POINTER(_vp)]),
+    "recon_tsdf_destroy": (None, [_vp]),
+    "recon_tsdf_reset": (C.c_int, [_vp, _vp]),
+    "recon_tsdf_get_info": (C.c_int, [_vp, C.POINTER(TsdfInfo)]),
+    "recon_tsdf_distances": (_vp, [_vp]),
+    "recon_tsdf_weights": (_vp, [_vp]),
+    "recon_tsdf_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "recon_tsdf_upload": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "recon_tsdf_integrate": (C.c_int, [_vp, C.POINTER(DepthView), _vp, C.c_size_t, _d7, _vp]),
+    "recon_tsdf_integrate_frames": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _d7,
+                                              _vp]),
+    "recon_tsdf_set_update_counter": (C.c_int, [_vp, _vp]),
+    "recon_tsdf_count_triangles": (C.c_int, [_vp, _pi64, _vp]),
+    "recon_tsdf_extract_mesh": (C.c_int, [_vp, _vp, _vp, _i64, _pi64, _vp]),
+    "recon_mc_count": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), _pi64, _vp]),
+    "recon_mc_extract": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), _vp, _vp, _i64, _pi64, _vp]),
+    "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
+    "recon_depth_to_points": (C.c_int, [C.POINTER(DepthView), C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _d7, _vp, C.c_size_t, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return LIB_DIR / "librecon.so"
+
+
+def load() -> C.CDLL:
+    """Loads librecon.so (built by recon.build).  torch, when used for device
+    memory, must be imported first so that one HIP runtime serves both."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: run __graft_entry__.build() (or "
+                           "python cu-reconstruction_amd/recon/build.py) first")
+    lib = C.CDLL(str(p), mode=os.RTLD_NOW | os.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class ReconError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int) -> int:
+    if status != OK:
+        raise ReconError(status, load().recon_last_error().decode())
+    return status
+
+
+def pose_array(pose) -> C.Array:
+    vals = [float(v) for v in pose]
+    if len(vals) % 7:
+        raise ValueError("poses are 7-vectors {qw,qx,qy,qz,tx,ty,tz}")
+    return (C.c_double * len(vals))(*vals)
+
+
+def i32x3(v) -> C.Array:
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def f32x3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in v])

@@ -1,0 +1,314 @@
+"""Synthetic workloads of the TSDF fusion path.
+
+Each workload fixes a volume, a camera path, a scene of analytic primitives
+and a depth-noise model; depth frames are ray-cast by librecon_scene.so.
+All randomness comes from a project-owned splitmix64 stream, so every frame
+is bit-identical on every machine.
+
+  plain_small      128^3, 8 mm, trunc 4 voxels, cap 128; sphere r=0.3 m at the
+                   origin + plane z=-0.35; 10 cameras on a 1 m arc 10 deg apart
+                   looking at the origin; 640x480, f=525, c=(319.5, 239.5);
+                   noise 1 mm, 2 % dropout.
+  plain_large      512^3, 4 mm, offset (-1.024,-1.024,-0.1), trunc 5 voxels,
+                   cap 128; 40 seeded boxes (sides 0.05-0.6 m) on the floor
+                   z=0; 300 frames on a seeded smooth orbit (radius
+                   1.2-1.8 m); noise 0.5 mm + 0.0015 z^2, 3 % dropout; mesh
+                   after every 50 frames.
+  batched_uneven   64 scenes of 1-5 spheres/boxes, grids {64..256}^3,
+                   1-40 frames, 320x240 / 640x480 / 1280x720, f = 0.82 W.
+  differentiable   256^3, 6 mm, the plain_small scene, 20 frames.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import make_config
+
+PKG = Path(__file__).resolve().parents[1]
+
+PRIM_SPHERE, PRIM_BOX, PRIM_PLANE_Z = 0, 1, 2
+MASK64 = (1 << 64) - 1
+
+
+class Prim(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("p", C.c_double * 6)]
+
+
+class Noise(C.Structure):
+    _fields_ = [("sigma0", C.c_double), ("sigma_z2", C.c_double), ("dropout", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+def splitmix64(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+    return x ^ (x >> 31)
+
+
+class Rng:
+    """Sequential splitmix64 stream."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next(self) -> int:
+        self.state = (self.state + 1) & MASK64
+        return splitmix64(self.state)
+
+    def uniform(self, lo=0.0, hi=1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+    def integer(self, lo: int, hi: int) -> int:  # inclusive
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+
+def quat_from_matrix(R: np.ndarray) -> np.ndarray:
+    """Unit quaternion (w, x, y, z), w >= 0, of a rotation matrix."""
+    m = R
+    tr = m[0, 0] + m[1, 1] + m[2, 2]
+    if tr > 0:
+        s = math.sqrt(tr + 1.0) * 2
+        q = [0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s]
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s = math.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        q = [(m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s]
+    elif m[1, 1] > m[2, 2]:
+        s = math.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        q = [(m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s]
+    else:
+        s = math.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        q = [(m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s]
+    q = np.array(q, dtype=np.float64)
+    q /= np.linalg.norm(q)
+    return -q if q[0] < 0 else q
+
+
+def matrix_from_quat(q) -> np.ndarray:
+    w, x, y, z = np.asarray(q, np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def look_at_pose(centre, target, up=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """World->camera pose {qw,qx,qy,qz,tx,ty,tz} of a camera at `centre`
+    looking at `target`, OpenCV axes (x right, y down, z forward)."""
+    c = np.asarray(centre, np.float64)
+    z = np.asarray(target, np.float64) - c
+    z /= np.linalg.norm(z)
+    x = np.cross(z, np.asarray(up, np.float64))
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    R_cw = np.stack([x, y, z])  # rows: camera axes in world coordinates
+    t = -R_cw @ c
+    return np.concatenate([quat_from_matrix(R_cw), t])
+
+
+def pose_matrix(pose) -> tuple[np.ndarray, np.ndarray]:
+    pose = np.asarray(pose, np.float64)
+    return matrix_from_quat(pose[:4]), pose[4:7].copy()
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: tuple
+    voxel_size: float
+    offset: tuple
+    truncation: float
+    max_weight: float
+    intrinsics: tuple          # fx, fy, cx, cy
+    image: tuple               # cols, rows
+    prims: list                # (kind, params)
+    poses: np.ndarray          # (F, 7)
+    noise: tuple               # sigma0, sigma_z2, dropout, seed
+    mesh_every: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def num_frames(self) -> int:
+        return int(self.poses.shape[0])
+
+    @property
+    def num_voxels(self) -> int:
+        return int(np.prod(self.grid))
+
+    def config(self):
+        return make_config(self.grid, voxel_size=self.voxel_size, offset=self.offset,
+                           max_weight=self.max_weight, intrinsics=self.intrinsics,
+                           image=self.image, truncation=self.truncation)
+
+    def prim_array(self):
+        arr = (Prim * max(len(self.prims), 1))()
+        for i, (kind, p) in enumerate(self.prims):
+            arr[i].kind = kind
+            for k, v in enumerate(p):
+                arr[i].p[k] = float(v)
+        return arr
+
+    def render(self, frames=None, noise=True, nthreads=0) -> np.ndarray:
+        """Depth frames (F, rows, cols) float32 in metres."""
+        frames = np.arange(self.num_frames) if frames is None else np.asarray(frames)
+        cols, rows = self.image
+        out = np.empty((len(frames), rows, cols), np.float32)
+        poses = np.ascontiguousarray(self.poses[frames], np.float64)
+        ids = np.ascontiguousarray(frames, np.uint64)
+        nz = None
+        if noise:
+            nz = Noise(*self.noise[:3], self.noise[3])
+        fx, fy, cx, cy = self.intrinsics
+        rc = scene_lib().recon_scene_render(
+            self.prim_array(), len(self.prims), poses.ctypes.data, ids.ctypes.data, len(frames),
+            fx, fy, cx, cy, cols, rows, C.byref(nz) if nz is not None else None, out.ctypes.data,
+            nthreads)
+        if rc != 0:
+            raise RuntimeError(f"recon_scene_render failed ({rc})")
+        return out
+
+    def sdf(self, points: np.ndarray) -> np.ndarray:
+        lib = scene_lib()
+        prims = self.prim_array()
+        pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        return np.array([lib.recon_scene_sdf(prims, len(self.prims), p.ctypes.data) for p in pts])
+
+
+_scene = None
+
+
+def scene_lib() -> C.CDLL:
+    global _scene
+    if _scene is None:
+        p = PKG / "lib" / "librecon_scene.so"
+        if not p.exists():
+            raise RuntimeError(f"{p} is missing: build first")
+        _scene = C.CDLL(str(p))
+        _scene.recon_scene_render.restype = C.c_int
+        _scene.recon_scene_render.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                              C.c_int32, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, C.c_int32, C.c_int32, C.c_void_p,
+                                              C.c_void_p, C.c_int32]
+        _scene.recon_scene_sdf.restype = C.c_double
+        _scene.recon_scene_sdf.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    return _scene
+
+
+# ---- workloads ---------------------------------------------------------------
+
+def arc_poses(n: int, step_deg: float = 10.0, radius: float = 1.0, height: float = 0.0,
+              start_deg: float = 0.0, target=(0.0, 0.0, 0.0)) -> np.ndarray:
+    out = []
+    for k in range(n):
+        th = math.radians(start_deg + step_deg * k)
+        out.append(look_at_pose((radius * math.cos(th), radius * math.sin(th), height), target))
+    return np.array(out)
+
+
+def plain_small(num_frames: int = 10, grid: int = 128, voxel_size: float = 0.008) -> Workload:
+    half = grid * voxel_size / 2
+    return Workload(
+        name=f"plain_small_{grid}^3x{num_frames}",
+        grid=(grid, grid, grid), voxel_size=voxel_size, offset=(-half, -half, -half),
+        truncation=4 * voxel_size, max_weight=128.0,
+        intrinsics=(525.0, 525.0, 319.5, 239.5), image=(640, 480),
+        prims=[(PRIM_SPHERE, (0.0, 0.0, 0.0, 0.3)), (PRIM_PLANE_Z, (-0.35,))],
+        poses=arc_poses(num_frames), noise=(0.001, 0.0, 0.02, 0xC0FFEE))
+
+
+def differentiable(num_frames: int = 20) -> Workload:
+    w = plain_small(num_frames=num_frames, grid=256, voxel_size=0.006)
+    w.name = f"differentiable_256^3x{num_frames}"
+    return w
+
+
+def catmull_rom(p0, p1, p2, p3, t):
+    t2, t3 = t * t, t * t * t
+    return 0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t2 +
+                  (-p0 + 3 * p1 - 3 * p2 + p3) * t3)
+
+
+def plain_large(num_frames: int = 300, seed: int = 0x5EED0001) -> Workload:
+    rng = Rng(seed)
+    prims = [(PRIM_PLANE_Z, (0.0,))]
+    for _ in range(40):
+        sx, sy, sz = (rng.uniform(0.05, 0.6) for _ in range(3))
+        cx, cy = rng.uniform(-0.75, 0.75), rng.uniform(-0.75, 0.75)
+        prims.append((PRIM_BOX, (cx - sx / 2, cy - sy / 2, 0.0, cx + sx / 2, cy + sy / 2, sz)))
+    # periodic Catmull-Rom spline over 8 control points: radius 1.2-1.8 m,
+    # height 0.4-1.2 m, one revolution over the frames
+    nctl = 8
+    rad = [rng.uniform(1.2, 1.8) for _ in range(nctl)]
+    hgt = [rng.uniform(0.4, 1.2) for _ in range(nctl)]
+    phase = rng.uniform(0.0, 2 * math.pi)
+    poses = []
+    for f in range(num_frames):
+        s = nctl * f / num_frames
+        i = int(s)
+        t = s - i
+        idx = [(i - 1) % nctl, i % nctl, (i + 1) % nctl, (i + 2) % nctl]
+        r = catmull_rom(*[rad[j] for j in idx], t)
+        h = catmull_rom(*[hgt[j] for j in idx], t)
+        th = phase + 2 * math.pi * f / num_frames
+        poses.append(look_at_pose((r * math.cos(th), r * math.sin(th), h), (0.0, 0.0, 0.15)))
+    return Workload(
+        name=f"plain_large_512^3x{num_frames}",
+        grid=(512, 512, 512), voxel_size=0.004, offset=(-1.024, -1.024, -0.1),
+        truncation=5 * 0.004, max_weight=128.0,
+        intrinsics=(525.0, 525.0, 319.5, 239.5), image=(640, 480),
+        prims=prims, poses=np.array(poses), noise=(0.0005, 0.0015, 0.03, seed ^ 0xABCDEF),
+        mesh_every=50)
+
+
+RESOLUTIONS = [(320, 240), (640, 480), (1280, 720)]
+GRID_SIZES = [64, 96, 128, 192, 256]
+
+
+def batched_uneven(num_volumes: int = 64, seed: int = 0x5EED0002) -> list[Workload]:
+    rng = Rng(seed)
+    out = []
+    extent = 1.0
+    for v in range(num_volumes):
+        s = GRID_SIZES[rng.integer(0, 4)]
+        frames = rng.integer(1, 40)
+        cols, rows = RESOLUTIONS[rng.integer(0, 2)]
+        vs = extent / s
+        prims = []
+        for _ in range(rng.integer(1, 5)):
+            if rng.uniform() < 0.5:
+                prims.append((PRIM_SPHERE, (rng.uniform(-0.25, 0.25), rng.uniform(-0.25, 0.25),
+                                            rng.uniform(-0.25, 0.25), rng.uniform(0.05, 0.2))))
+            else:
+                c = [rng.uniform(-0.25, 0.25) for _ in range(3)]
+                h = [rng.uniform(0.03, 0.15) for _ in range(3)]
+                prims.append((PRIM_BOX, tuple(c[i] - h[i] for i in range(3)) +
+                              tuple(c[i] + h[i] for i in range(3))))
+        start = rng.uniform(0.0, 360.0)
+        elev = rng.uniform(-0.3, 0.5)
+        poses = arc_poses(frames, step_deg=9.0, radius=1.2, height=elev, start_deg=start)
+        fx = 0.82 * cols
+        out.append(Workload(
+            name=f"batched_{v}_{s}^3x{frames}@{cols}x{rows}",
+            grid=(s, s, s), voxel_size=vs, offset=(-extent / 2,) * 3, truncation=4 * vs,
+            max_weight=128.0, intrinsics=(fx, fx, cols / 2 - 0.5, rows / 2 - 0.5),
+            image=(cols, rows), prims=prims, poses=poses,
+            noise=(0.001, 0.0, 0.02, seed ^ (v * 0x9E37)), meta={"index": v}))
+    return out
+
+
+def tiny(grid=(16, 12, 10), voxel_size=0.05, image=(32, 24), num_frames=3, seed=7) -> Workload:
+    """A small scene for exhaustive checks: sphere + floor seen by a short arc."""
+    gx, gy, gz = grid
+    off = (-gx * voxel_size / 2, -gy * voxel_size / 2, -gz * voxel_size / 2)
+    cols, rows = image
+    f = 0.9 * cols
+    return Workload(
+        name=f"tiny_{gx}x{gy}x{gz}", grid=tuple(grid), voxel_size=voxel_size, offset=off,
+        truncation=3 * voxel_size, max_weight=2.0, intrinsics=(f, f, cols / 2 - 0.5, rows / 2 - 0.5),
+        image=image, prims=[(PRIM_SPHERE, (0.0, 0.0, 0.0, 0.18)), (PRIM_PLANE_Z, (-0.2,))],
+        poses=arc_poses(num_frames, step_deg=25.0, radius=1.0, height=0.2, start_deg=10.0),
+        noise=(0.002, 0.0, 0.05, seed))

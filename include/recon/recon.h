@@ -1,0 +1,192 @@
+/*
+ * recon.h -- C-ABI of the MI355X (gfx950) TSDF fusion path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every device pointer is a
+ * HIP device allocation; every `stream` argument is a hipStream_t passed as
+ * an opaque pointer (NULL = the default stream).  Calls marked "async" only
+ * enqueue work on the stream; calls marked "sync" wait for the stream before
+ * returning.  A handle is not thread-safe; distinct handles are independent.
+ *
+ * Reference interface replaced (cu-Reconstruction, paths relative to its root):
+ *   TSDFVolumeConfig              modules/include/cuda/tsdf_volume.cuh:17-29
+ *   TSDFVolume ctor / dtor        modules/src/cuda/tsdf_volume.cu:194-254
+ *   TSDFVolume::integrate         modules/include/cuda/tsdf_volume.cuh:49-51,
+ *                                 modules/src/cuda/tsdf_volume.cu:257-290
+ *   accessors                     modules/include/cuda/tsdf_volume.cuh:57-92
+ *   generate_triangular_surface   modules/include/cuda/marching_cubes.cuh:13-15,
+ *                                 modules/src/cuda/marching_cubes.cu:191-236
+ *   cudaPointCloud::extract_points modules/include/cuda/cuda_point_cloud.cuh:55-58,
+ *                                 modules/src/cuda/cuda_point_cloud.cu:179-226
+ * The reference reports errors as bool + a log line (tsdf_volume.cu:260-268);
+ * here each failure class has its own negative status and recon_last_error()
+ * returns the message.
+ */
+#ifndef RECON_RECON_H
+#define RECON_RECON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RECON_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    RECON_OK = 0,
+    RECON_ERR_NULL = -1,      /* a required pointer argument is NULL            */
+    RECON_ERR_SIZE = -2,      /* zero / negative / inconsistent image or buffer size */
+    RECON_ERR_CONFIG = -3,    /* invalid volume configuration                   */
+    RECON_ERR_OOM = -4,       /* device allocation failed                       */
+    RECON_ERR_HIP = -5,       /* a HIP runtime call or kernel launch failed     */
+    RECON_ERR_CAPACITY = -6,  /* output buffer too small (count still reported) */
+    RECON_ERR_RANGE = -7,     /* sizes overflow the kernels' 32-bit indexing    */
+    RECON_ERR_DEVICE = -8     /* no usable gfx950 device / wrong device         */
+};
+
+typedef void* recon_stream_t; /* hipStream_t */
+
+#if defined(__GNUC__)
+#define RECON_API __attribute__((visibility("default")))
+#else
+#define RECON_API
+#endif
+
+/* ---- volume configuration ---------------------------------------------- */
+/* Mirrors TSDFVolumeConfig (tsdf_volume.cuh:17-29).  The last three fields are
+ * additions: the reference hard-codes truncation = 1.1*|voxel_size|
+ * (tsdf_volume.cu:199), max depth = 24 (tsdf_volume.cu:169) and never applies
+ * max_weight (tsdf_volume.cu:179-185).  A value <= 0 in `truncation` or
+ * `max_depth` selects the reference's constant. */
+typedef struct recon_tsdf_config {
+    int32_t voxel_grid_size[3];   /* gx, gy, gz (voxels)                        */
+    float physical_size[3];       /* metres; voxel_size = physical / grid       */
+    float global_offset[3];       /* world position of the grid's min corner    */
+    float max_weight;             /* weight cap, >= 1                            */
+    float focal_x, focal_y;       /* pinhole intrinsics of the depth camera     */
+    float cx, cy;
+    uint32_t camera_width;        /* informational, as in the reference         */
+    uint32_t camera_height;
+    float max_cell_size;          /* unused by the reference; kept for layout   */
+    float truncation;             /* metres; <= 0 -> 1.1*|voxel_size|           */
+    float max_depth;              /* metres; <= 0 -> 24                         */
+} recon_tsdf_config;
+
+/* Read-only description of a created volume. */
+typedef struct recon_tsdf_info {
+    int32_t grid[3];
+    float voxel_size[3];
+    float global_offset[3];
+    float truncation;
+    float max_weight;
+    float max_depth;
+    float intrinsics[4];          /* fx, fy, cx, cy */
+    int64_t num_voxels;
+} recon_tsdf_info;
+
+/* Pitched single-channel float32 image in device memory, the shape of
+ * cv::cuda::PtrStepSzf (data, step in bytes, rows, cols). */
+typedef struct recon_depth_view {
+    const float* data;
+    size_t step;                  /* bytes between rows, >= 4*cols, multiple of 4 */
+    int32_t rows;
+    int32_t cols;
+} recon_depth_view;
+
+typedef struct recon_tsdf recon_tsdf; /* opaque handle */
+
+/* ---- library ------------------------------------------------------------ */
+/* Thread-local message of the last failing call on this thread ("" if none). */
+RECON_API const char* recon_last_error(void);
+RECON_API int recon_abi_version(void);
+/* sync.  Fills a volume description without allocating: validates `cfg`. */
+RECON_API int recon_tsdf_describe(const recon_tsdf_config* cfg, recon_tsdf_info* info);
+
+/* ---- lifecycle (TSDFVolume ctor/dtor, tsdf_volume.cu:194-254) ---------- */
+/* sync.  Allocates D and W (float32[gz][gy][gx], linear x + gx*(y + gy*z)) on
+ * the current device and sets D = truncation, W = 0 (tsdf_volume.cu:227-254).
+ * No voxel-centre table is allocated: centres are computed analytically. */
+RECON_API int recon_tsdf_create(const recon_tsdf_config* cfg, recon_tsdf** out);
+RECON_API void recon_tsdf_destroy(recon_tsdf* h);
+/* async.  D = truncation, W = 0. */
+RECON_API int recon_tsdf_reset(recon_tsdf* h, recon_stream_t stream);
+RECON_API int recon_tsdf_get_info(const recon_tsdf* h, recon_tsdf_info* info);
+
+/* ---- accessors (tsdf_volume.cuh:57-92) --------------------------------- */
+RECON_API float* recon_tsdf_distances(recon_tsdf* h);   /* device pointer, N floats */
+RECON_API float* recon_tsdf_weights(recon_tsdf* h);     /* device pointer, N floats */
+/* sync.  Copies D and/or W (either may be NULL) to host memory of N floats. */
+RECON_API int recon_tsdf_download(recon_tsdf* h, float* host_d, float* host_w, recon_stream_t stream);
+/* sync.  Copies D and/or W from host memory of N floats into the volume. */
+RECON_API int recon_tsdf_upload(recon_tsdf* h, const float* host_d, const float* host_w, recon_stream_t stream);
+
+/* ---- integrate (TSDFVolume::integrate, tsdf_volume.cu:144-190,257-290) -- */
+/* async.  One depth frame (metres, float32, device memory) seen from
+ * camera_pose = {qw,qx,qy,qz,tx,ty,tz}, world->camera, narrowed to float32
+ * as the reference does (tsdf_volume.cu:271-272).  `rgb` (pitched uchar3) is
+ * accepted and ignored, as in the reference; it may be NULL. */
+RECON_API int recon_tsdf_integrate(recon_tsdf* h, const recon_depth_view* depth,
+                         const uint8_t* rgb, size_t rgb_step,
+                         const double camera_pose[7], recon_stream_t stream);
+/* async.  Same, for `num_frames` frames applied in order in one pass over
+ * the volume: depth frame f starts at depth0 + f*frame_stride_bytes, all
+ * frames share rows/cols/step, poses are num_frames*7 doubles.  Results are
+ * identical to num_frames calls of recon_tsdf_integrate.  The depth maps must
+ * stay valid until the stream reaches this work. */
+RECON_API int recon_tsdf_integrate_frames(recon_tsdf* h, const recon_depth_view* depth0,
+                                size_t frame_stride_bytes, int32_t num_frames,
+                                const double* camera_poses, recon_stream_t stream);
+/* When `counter` (device uint64) is non-NULL, later integrate calls add the
+ * number of voxel updates they perform to it (diagnostic; slower path).
+ * Pass NULL to switch counting off. */
+RECON_API int recon_tsdf_set_update_counter(recon_tsdf* h, unsigned long long* counter);
+
+/* ---- marching cubes (generate_triangular_surface, marching_cubes.cu) --- */
+/* The mesh is the reference's triangle soup (marching_cubes.cu:159-189):
+ * triangle t owns vertices 3t, 3t+1, 3t+2 and triangles are ordered by cube
+ * z (outer), y, x, then table slot.  verts: 9 floats per triangle (x,y,z of
+ * its three vertices); tris: 3 int32 per triangle (the vertex indices).
+ * sync.  Total triangle count of the current volume. */
+RECON_API int recon_tsdf_count_triangles(recon_tsdf* h, int64_t* num_triangles, recon_stream_t stream);
+/* sync.  Writes min(count, capacity) triangles; *num_triangles = count.
+ * Returns RECON_ERR_CAPACITY when count > capacity.  `tris` may be NULL. */
+RECON_API int recon_tsdf_extract_mesh(recon_tsdf* h, float* verts, int32_t* tris,
+                            int64_t capacity_triangles, int64_t* num_triangles,
+                            recon_stream_t stream);
+/* Same two calls over any dense float32[gz][gy][gx] field in device memory,
+ * with corner positions (v + 0.5) * voxel_size + offset. */
+RECON_API int recon_mc_count(const float* field, const int32_t grid[3], const float voxel_size[3],
+                   const float offset[3], int64_t* num_triangles, recon_stream_t stream);
+RECON_API int recon_mc_extract(const float* field, const int32_t grid[3], const float voxel_size[3],
+                     const float offset[3], float* verts, int32_t* tris,
+                     int64_t capacity_triangles, int64_t* num_triangles,
+                     recon_stream_t stream);
+/* async.  hist_device (256 uint64, device memory) <- number of cubes of each
+ * class, bit i of a class set iff corner i is < 0 (marching_cubes.cu:76-84).
+ * The reference has no such call; it exposes the classification that the
+ * extraction is built on. */
+RECON_API int recon_mc_class_histogram(const float* field, const int32_t grid[3],
+                             unsigned long long* hist_device, recon_stream_t stream);
+
+/* ---- depth -> point cloud (point_cloud_extract, cuda_point_cloud.cu:22-70) */
+/* async.  Pixel (x,y) with depth d>0: p = ((x-cx)*d/fx, (y-cy)*d/fy, d)
+ * computed in float64 and rounded to float32, then p_world = T*p in float32
+ * with transform = {qw,qx,qy,qz,tx,ty,tz} (camera->world, as the reference
+ * uses it, cuda_point_cloud.cu:60).  Output index y*cols + x:
+ * xyz[3*i..3*i+2], valid[i] = 1; pixels with d <= 0 write (0,0,0) and
+ * valid = 0 (the reference leaves them unwritten).  `colors_out` (3 bytes
+ * per point, may be NULL) receives the rgb pixel, or 255,255,255 when `rgb`
+ * is NULL, or 0,0,0 for invalid pixels. */
+RECON_API int recon_depth_to_points(const recon_depth_view* depth, double fx, double fy,
+                          double cx, double cy, const double transform[7],
+                          const uint8_t* rgb, size_t rgb_step,
+                          float* xyz, uint8_t* valid, uint8_t* colors_out,
+                          recon_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RECON_RECON_H */

@@ -1,0 +1,117 @@
+/*
+ * oracle.h -- CPU restatement of the reference's TSDF fusion path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product library never links it.
+ *
+ * Everything here follows cu-Reconstruction's source (paths relative to its
+ * root) under the build contract listed per function: the reference's own
+ * operation order in IEEE float32, no FMA contraction (built with
+ * -ffp-contract=off), correctly rounded division.  Three reference quirks are
+ * replaced, identically in the oracle and the GPU path:
+ *   - sdf uses the voxel's camera-space z, not its world z
+ *     (tsdf_volume.cu:172-173 compute voxel_cam and then use voxel_ct.z);
+ *     identical to the reference at the identity pose;
+ *   - a voxel must have camera z > 0 (camera_util.cu:9-10,46-47 map z == 0
+ *     to pixel (0,0) and project points behind the camera mirrored);
+ *   - the weight is capped, W' = min(W+1, max_weight), with
+ *     D' = (D*W + tsdf)/(W+1) (tsdf_volume.cu:179-185 ignore max_weight; the
+ *     cap follows octree.cu:125-127);
+ *   - marching cubes uses the layer stride gx*gy (marching_cubes.cu:211
+ *     uses gx+gy, correct only for 2x2xN grids).
+ */
+#ifndef RECON_ORACLE_H
+#define RECON_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_volume {
+    int32_t grid[3];        /* gx, gy, gz */
+    float voxel_size[3];
+    float offset[3];
+    float truncation;
+    float max_weight;
+    float max_depth;
+    float fx, fy, cx, cy;
+} oracle_volume;
+
+/* Pose {qw,qx,qy,qz,tx,ty,tz} narrowed to float and expanded to a row-major
+ * 3x3 rotation + translation in float32 (tsdf_volume.cu:271-272,
+ * se3.cuh:42-57,124-140). */
+void oracle_pose_matrix_f32(const double pose[7], float R[9], float t[3]);
+
+/* Voxel centre in float32, ((v + 0.5f) * vs) + off, each step rounded
+ * (tsdf_volume.cu:117-119). */
+void oracle_voxel_centre_f32(const oracle_volume* vol, int32_t x, int32_t y, int32_t z, float c[3]);
+
+/* D = truncation, W = 0 (tsdf_volume.cu:227-254). */
+void oracle_tsdf_init(const oracle_volume* vol, float* D, float* W);
+
+/* One frame of cu_tsdf_integrate (tsdf_volume.cu:144-190) under the contract
+ * above.  depth: rows x cols floats with `pitch` floats per row.  When
+ * `update_mask` is non-NULL it receives 1 for every voxel updated by this
+ * frame, 0 otherwise.  Returns the number of updated voxels.  OpenMP over
+ * (y,z) rows with `nthreads` threads (<= 0: library default), which is the
+ * reference kernel's own decomposition (tsdf_volume.cu:151-163). */
+int64_t oracle_tsdf_integrate_f32(const oracle_volume* vol, float* D, float* W,
+                                  const float* depth, int32_t rows, int32_t cols,
+                                  int64_t pitch, const double pose[7],
+                                  uint8_t* update_mask, int nthreads);
+
+/* The same frame with every branch decision (pixel index, depth validity,
+ * the sdf >= -truncation test) taken from the float32 path, and the values
+ * (camera z, sdf, clamp, running average) computed in float64 from the
+ * float-narrowed pose.  *decision_flips (may be NULL) receives the number of
+ * voxels whose decision an independent float64 evaluation would change.
+ * Returns the number of updated voxels. */
+int64_t oracle_tsdf_integrate_f64(const oracle_volume* vol, double* D, double* W,
+                                  const float* depth, int32_t rows, int32_t cols,
+                                  int64_t pitch, const double pose[7],
+                                  int64_t* decision_flips, int nthreads);
+
+/* Marching cubes, class pass (marching_cubes.cu:76-84,111-130).  Fills
+ * class_hist[256] (may be NULL) over all (gx-1)(gy-1)(gz-1) cubes and returns
+ * the triangle count. */
+int64_t oracle_mc_count(const float* field, const int32_t grid[3],
+                        int64_t* class_hist, int nthreads);
+
+/* Marching cubes emission (marching_cubes.cu:23-74,86-189,191-236): triangle
+ * soup in cube order z, y, x, slot.  Writes at most `capacity` triangles:
+ * verts 9 floats, tris 3 int32 (3t,3t+1,3t+2), edge_ids 3 int32 per triangle
+ * (the table edge of each vertex; may be NULL), cube_ids (linear cube index
+ * x + (gx-1)*(y + (gy-1)*z); may be NULL).  Returns the total count. */
+int64_t oracle_mc_extract(const float* field, const int32_t grid[3],
+                          const float voxel_size[3], const float offset[3],
+                          float* verts, int32_t* tris, int32_t* edge_ids,
+                          int64_t* cube_ids, int64_t capacity, int nthreads);
+
+/* point_cloud_extract (cuda_point_cloud.cu:22-70): float64 back-projection,
+ * float32 transform.  Writes every pixel: invalid (d <= 0) pixels get
+ * (0,0,0) and valid = 0.  colors_out may be NULL; rgb may be NULL (white). */
+void oracle_depth_to_points(const float* depth, int32_t rows, int32_t cols, int64_t pitch,
+                            double fx, double fy, double cx, double cy,
+                            const double transform[7],
+                            const uint8_t* rgb, int64_t rgb_pitch,
+                            float* xyz, uint8_t* valid, uint8_t* colors_out);
+
+/* point_cloud_from_depth (point_cloud_utility.cpp:76-115): all-float64 path,
+ * keeps min < d < max, transform applied in float64 (the caller's Sophus SE3
+ * as a rotation from the normalised quaternion).  Writes up to `capacity`
+ * points (3 doubles each) and their pixel indices; returns the count. */
+int64_t oracle_point_cloud_from_depth_f64(const float* depth, int32_t rows, int32_t cols,
+                                          int64_t pitch, double fx, double fy, double cx,
+                                          double cy, const double transform[7],
+                                          double depth_min, double depth_max,
+                                          double* xyz, int64_t* pixel_index,
+                                          int64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RECON_ORACLE_H */

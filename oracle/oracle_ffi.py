@@ -1,0 +1,181 @@
+"""ctypes binding of oracle/lib/liboracle.so (the CPU restatement).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "lib" / "liboracle.so"
+
+
+class OracleVolume(C.Structure):
+    _fields_ = [("grid", C.c_int32 * 3), ("voxel_size", C.c_float * 3), ("offset", C.c_float * 3),
+                ("truncation", C.c_float), ("max_weight", C.c_float), ("max_depth", C.c_float),
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run make -C oracle")
+        L = C.CDLL(str(LIB))
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        sig = {
+            "oracle_pose_matrix_f32": (None, [vp, vp, vp]),
+            "oracle_voxel_centre_f32": (None, [vp, i32, i32, i32, vp]),
+            "oracle_tsdf_init": (None, [vp, vp, vp]),
+            "oracle_tsdf_integrate_f32": (i64, [vp, vp, vp, vp, i32, i32, i64, vp, vp, C.c_int]),
+            "oracle_tsdf_integrate_f64": (i64, [vp, vp, vp, vp, i32, i32, i64, vp, vp, C.c_int]),
+            "oracle_mc_count": (i64, [vp, vp, vp, C.c_int]),
+            "oracle_mc_extract": (i64, [vp, vp, vp, vp, vp, vp, vp, vp, i64, C.c_int]),
+            "oracle_depth_to_points": (None, [vp, i32, i32, i64, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, vp, vp, i64, vp, vp, vp]),
+            "oracle_point_cloud_from_depth_f64": (i64, [vp, i32, i32, i64, C.c_double, C.c_double,
+                                                        C.c_double, C.c_double, vp, C.c_double,
+                                                        C.c_double, vp, vp, i64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def volume(grid, voxel_size, offset, truncation, max_weight, intrinsics, max_depth=24.0):
+    """Oracle volume; voxel size in float32 as physical/grid is computed
+    (tsdf_volume.cu:197) when `voxel_size` is given as the physical size
+    divided out already."""
+    v = OracleVolume()
+    v.grid = (C.c_int32 * 3)(*[int(g) for g in grid])
+    v.voxel_size = (C.c_float * 3)(*[float(np.float32(s)) for s in voxel_size])
+    v.offset = (C.c_float * 3)(*[float(np.float32(o)) for o in offset])
+    v.truncation = float(np.float32(truncation))
+    v.max_weight = float(np.float32(max_weight))
+    v.max_depth = float(np.float32(max_depth))
+    v.fx, v.fy, v.cx, v.cy = [float(np.float32(k)) for k in intrinsics]
+    return v
+
+
+def volume_from_config(cfg) -> OracleVolume:
+    """From a recon_tsdf_config: voxel_size = physical / grid in float32,
+    truncation default 1.1 * |voxel_size| (tsdf_volume.cu:197-199)."""
+    grid = [int(g) for g in cfg.voxel_grid_size]
+    phys = np.array(list(cfg.physical_size), np.float32)
+    vs = phys / np.array(grid, np.float32)
+    if cfg.truncation > 0:
+        trunc = np.float32(cfg.truncation)
+    else:
+        dot = (vs[0] * vs[0] + vs[1] * vs[1]) + vs[2] * vs[2]
+        trunc = np.float32(1.1) * np.sqrt(np.float32(dot))
+    maxd = np.float32(cfg.max_depth) if cfg.max_depth > 0 else np.float32(24.0)
+    return volume(grid, vs, list(cfg.global_offset), trunc, cfg.max_weight,
+                  (cfg.focal_x, cfg.focal_y, cfg.cx, cfg.cy), maxd)
+
+
+def shape(vol: OracleVolume):
+    g = vol.grid
+    return (g[2], g[1], g[0])
+
+
+def init(vol: OracleVolume):
+    D = np.empty(shape(vol), np.float32)
+    W = np.empty(shape(vol), np.float32)
+    lib().oracle_tsdf_init(C.byref(vol), _p(D), _p(W))
+    return D, W
+
+
+def pose_arr(pose):
+    return np.ascontiguousarray(np.asarray(pose, np.float64).reshape(7))
+
+
+def integrate_f32(vol, D, W, depth, pose, mask=False, nthreads=0):
+    depth = np.ascontiguousarray(depth, np.float32)
+    rows, cols = depth.shape
+    m = np.empty(shape(vol), np.uint8) if mask else None
+    n = lib().oracle_tsdf_integrate_f32(C.byref(vol), _p(D), _p(W), _p(depth), rows, cols, cols,
+                                        _p(pose_arr(pose)), _p(m), nthreads)
+    return (n, m) if mask else n
+
+
+def integrate_f64(vol, D64, W64, depth, pose, nthreads=0):
+    depth = np.ascontiguousarray(depth, np.float32)
+    rows, cols = depth.shape
+    flips = C.c_int64()
+    n = lib().oracle_tsdf_integrate_f64(C.byref(vol), _p(D64), _p(W64), _p(depth), rows, cols, cols,
+                                        _p(pose_arr(pose)), C.addressof(flips), nthreads)
+    return n, flips.value
+
+
+def pose_matrix_f32(pose):
+    R = np.empty(9, np.float32)
+    t = np.empty(3, np.float32)
+    lib().oracle_pose_matrix_f32(_p(pose_arr(pose)), _p(R), _p(t))
+    return R.reshape(3, 3), t
+
+
+def mc_count(field, grid, hist=False, nthreads=0):
+    field = np.ascontiguousarray(field, np.float32)
+    g = np.array(grid, np.int32)
+    h = np.zeros(256, np.int64) if hist else None
+    n = lib().oracle_mc_count(_p(field), _p(g), _p(h), nthreads)
+    return (n, h) if hist else n
+
+
+def mc_extract(field, grid, voxel_size, offset, nthreads=0):
+    """Returns verts (T,3,3) float32, tris (T,3) int32, edge ids (T,3) int32,
+    cube ids (T,) int64."""
+    field = np.ascontiguousarray(field, np.float32)
+    g = np.array(grid, np.int32)
+    vs = np.array(voxel_size, np.float32)
+    off = np.array(offset, np.float32)
+    n = mc_count(field, grid, nthreads=nthreads)
+    verts = np.empty((max(n, 1), 3, 3), np.float32)
+    tris = np.empty((max(n, 1), 3), np.int32)
+    edges = np.empty((max(n, 1), 3), np.int32)
+    cubes = np.empty((max(n, 1),), np.int64)
+    t = lib().oracle_mc_extract(_p(field), _p(g), _p(vs), _p(off), _p(verts), _p(tris), _p(edges),
+                                _p(cubes), n, nthreads)
+    assert t == n
+    return verts[:n], tris[:n], edges[:n], cubes[:n]
+
+
+def depth_to_points(depth, intrinsics, transform, rgb=None):
+    depth = np.ascontiguousarray(depth, np.float32)
+    rows, cols = depth.shape
+    xyz = np.empty((rows * cols, 3), np.float32)
+    valid = np.empty(rows * cols, np.uint8)
+    colors = np.empty((rows * cols, 3), np.uint8)
+    fx, fy, cx, cy = [float(v) for v in intrinsics]
+    rgbp = np.ascontiguousarray(rgb, np.uint8) if rgb is not None else None
+    lib().oracle_depth_to_points(_p(depth), rows, cols, cols, fx, fy, cx, cy,
+                                 _p(pose_arr(transform)), _p(rgbp), cols * 3, _p(xyz), _p(valid),
+                                 _p(colors))
+    return xyz, valid, colors
+
+
+def point_cloud_from_depth_f64(depth, intrinsics, transform, dmin=0.0, dmax=1e30):
+    depth = np.ascontiguousarray(depth, np.float32)
+    rows, cols = depth.shape
+    fx, fy, cx, cy = [float(v) for v in intrinsics]
+    xyz = np.empty((rows * cols, 3), np.float64)
+    idx = np.empty(rows * cols, np.int64)
+    n = lib().oracle_point_cloud_from_depth_f64(_p(depth), rows, cols, cols, fx, fy, cx, cy,
+                                                _p(pose_arr(transform)), dmin, dmax, _p(xyz),
+                                                _p(idx), rows * cols)
+    return xyz[:n], idx[:n]
