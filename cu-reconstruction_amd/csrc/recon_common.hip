@@ -131,8 +131,9 @@ void make_frame_params(const VolumeParams& vp, const double pose[7], const float
                        int32_t pitch_floats, FrameParams* fp) {
     pose_to_matrix_f32(pose, fp->R, fp->t);
     fp->depth = depth;
+    fp->bound0 = nullptr;
+    fp->bound1 = nullptr;
     fp->pitch = pitch_floats;
-    fp->pad_ = 0;
     const double fx = vp.fx, fy = vp.fy, cx = vp.cx, cy = vp.cy;
     const double a[6][3] = {{0, 0, 1},
                             {fx, 0, cx + 1.0},
@@ -149,6 +150,7 @@ void make_frame_params(const VolumeParams& vp, const double pose[7], const float
     }
     const double tabs = std::fabs((double)fp->t[0]) + std::fabs((double)fp->t[1]) +
                         std::fabs((double)fp->t[2]);
+    fp->zmargin = (float)(1e-6 * (tabs + cmax[0] + cmax[1] + cmax[2]) + 1e-6);
     for (int k = 0; k < 6; ++k) {
         double n[3];
         for (int i = 0; i < 3; ++i)  // n = R^T a

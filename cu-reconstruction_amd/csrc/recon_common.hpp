@@ -51,14 +51,25 @@ bool pose_is_finite(const double pose[7]);
 // over the volume, three orders above float32 rounding of the per-voxel
 // test, so a tile whose voxel-centre box has max_c(n.c + k + margin) < 0
 // holds no voxel that passes.
+//
+// Depth-bound culling: `bound0` / `bound1` hold, per 8x8 / 32x32 pixel block
+// of the frame, the largest valid depth (0 < d <= max_depth) in the block, or
+// 0 when the block has none.  A voxel can only be updated through a pixel of
+// its tile's image footprint whose depth is >= (voxel camera z) - truncation,
+// so a tile whose footprint maximum is below (tile's least camera z) -
+// truncation - margin holds no voxel that passes.
 struct FrameParams {
     float R[9];
     float t[3];
     float plane[6][4];
     const float* depth;
+    const float* bound0;
+    const float* bound1;
     int32_t pitch;   // floats per depth row
-    int32_t pad_;
+    float zmargin;   // bound on float32 error of a camera z: 1e-6 * (|t| + max|c|)
 };
+
+constexpr int kBound0 = 8, kBound1 = 32;  // pixel block edges of the two bound levels
 
 struct VolumeParams {
     int32_t gx, gy, gz;
@@ -69,6 +80,7 @@ struct VolumeParams {
     float trunc, maxw, maxd;
     int32_t cols, rows;
     float fcols, frows;
+    int32_t b0x, b0y, b1x, b1y;  // bound-level block counts
 };
 
 constexpr int kMaxFusedFrames = 16;
@@ -87,13 +99,17 @@ struct recon_tsdf {
     float* D = nullptr;
     float* W = nullptr;
     int device = 0;
+    int num_cus = 256;
+    int integrate_blocks_per_cu = 4;
     unsigned long long* update_counter = nullptr;
+    // per-frame depth bounds, grown on demand
+    float* bounds = nullptr;
+    size_t bounds_floats = 0;
+    void* list = nullptr;            // cull list entries
+    unsigned* list_count = nullptr;  // inside `list`
     // marching-cubes scratch, grown on demand
     void* mc_scratch = nullptr;
     size_t mc_scratch_bytes = 0;
-    uint64_t version = 0;      // bumped by every write to D
-    uint64_t mc_version = ~0ull;  // version the cached row offsets belong to
-    int64_t mc_total = -1;
 };
 
 namespace recon {
@@ -126,5 +142,6 @@ __device__ __forceinline__ bool pixel_index(float pa, float f, float pz, float c
     out = (int)s;
     return true;
 }
+
 
 }  // namespace recon

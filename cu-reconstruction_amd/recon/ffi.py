@@ -100,7 +100,9 @@ _lib = None
 
 
 def lib_path() -> Path:
-    return LIB_DIR / "librecon.so"
+    """librecon.so of this tree, or the file named by $RECON_LIB."""
+    env = os.environ.get("RECON_LIB")
+    return Path(env) if env else LIB_DIR / "librecon.so"
 
 
 def load() -> C.CDLL:

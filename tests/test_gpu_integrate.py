@@ -13,10 +13,10 @@ from recon import scenes
 
 pytestmark = pytest.mark.gpu
 
-F64_TOL = 1e-6  # absolute, metres
+F64_TOL = 1e-6  # absolute, metres, for volumes within a few metres of the origin
 
 
-def _check(w, depth, fused=False, pitch_pad=0, f64=True):
+def _check(w, depth, fused=False, pitch_pad=0, f64=True, f64_tol=F64_TOL):
     cfg = w.config()
     o = oracle_run(cfg, depth, w.poses, f64=f64)
     g = gpu_run(cfg, depth, w.poses, fused=fused, pitch_pad=pitch_pad, count=True)
@@ -28,7 +28,7 @@ def _check(w, depth, fused=False, pitch_pad=0, f64=True):
         assert g["counts"].tolist() == [int(c) for c in o["counts"]]
     if f64:
         err = np.abs(g["D"].astype(np.float64) - o["D64"]).max()
-        assert err <= F64_TOL, err
+        assert err <= f64_tol, err
         assert np.array_equal(g["W"].astype(np.float64), o["W64"])
     g["vol"].close()
     return o
@@ -92,3 +92,56 @@ def test_empty_depth_leaves_volume(cuda):
     o = _check(w, depth)
     assert sum(o["counts"]) == 0
     assert (o["W"] == 0).all()
+
+
+def _random_pose_workload(seed, grid=(96, 80, 72), vs=0.01, offset=None, inside=False):
+    """Cameras at random positions (inside the volume when `inside`) looking
+    at random points of the volume, random-valued depth maps."""
+    rng = np.random.default_rng(seed)
+    w = scenes.tiny(grid=grid, voxel_size=vs, image=(160, 120), num_frames=6)
+    if offset is not None:
+        w.offset = offset
+    lo = np.array(w.offset)
+    hi = lo + np.array(grid) * vs
+    poses = []
+    for _ in range(w.num_frames):
+        if inside:
+            c = lo + rng.uniform(0.2, 0.8, 3) * (hi - lo)
+        else:
+            c = (lo + hi) / 2 + rng.normal(0, 1, 3) * (hi - lo)
+        target = lo + rng.uniform(0, 1, 3) * (hi - lo)
+        poses.append(scenes.look_at_pose(c, target, up=(0.3, 0.2, 1.0)))
+    w.poses = np.array(poses)
+    depth = rng.uniform(0.0, 2.0 * np.linalg.norm(hi - lo), (w.num_frames, 120, 160)).astype(np.float32)
+    depth[rng.random(depth.shape) < 0.3] = 0.0
+    # patches of constant depth exercise the block bounds
+    depth[:, 20:60, 40:100] = rng.uniform(0.05, 1.0)
+    return w, depth
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_cull_random_poses_outside(cuda, seed):
+    w, depth = _random_pose_workload(seed)
+    _check(w, depth)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_cull_camera_inside_volume(cuda, seed):
+    w, depth = _random_pose_workload(seed, inside=True)
+    _check(w, depth)
+
+
+def test_cull_far_from_origin(cuda):
+    # 70 m from the origin a float32 ulp is 7.6e-6 m, so the float64 bound
+    # scales with the coordinates: 4 ulp of the largest coordinate
+    w, depth = _random_pose_workload(6, vs=0.02, offset=(50.0, -70.0, 30.0), inside=True)
+    _check(w, depth, f64_tol=4 * 2.0 ** -23 * 72.0)
+    w, depth = _random_pose_workload(7, vs=0.02, offset=(50.0, -70.0, 30.0))
+    _check(w, depth, fused=True, f64=False)
+
+
+def test_cull_fused_random(cuda):
+    w, depth = _random_pose_workload(8, inside=True)
+    w.poses = np.concatenate([w.poses] * 4)
+    depth = np.concatenate([depth] * 4)
+    _check(w, depth, fused=True, f64=False)
