@@ -251,8 +251,8 @@ def plain_large(num_frames: int = 300, seed: int = 0x5EED0001) -> Workload:
         i = int(s)
         t = s - i
         idx = [(i - 1) % nctl, i % nctl, (i + 1) % nctl, (i + 2) % nctl]
-        r = catmull_rom(*[rad[j] for j in idx], t)
-        h = catmull_rom(*[hgt[j] for j in idx], t)
+        r = min(max(catmull_rom(*[rad[j] for j in idx], t), 1.2), 1.8)
+        h = min(max(catmull_rom(*[hgt[j] for j in idx], t), 0.4), 1.2)
         th = phase + 2 * math.pi * f / num_frames
         poses.append(look_at_pose((r * math.cos(th), r * math.sin(th), h), (0.0, 0.0, 0.15)))
     return Workload(
