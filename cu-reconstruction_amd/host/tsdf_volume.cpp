@@ -1,0 +1,237 @@
+// tsdf_volume.cpp -- the C++ drop-in of include/recon/tsdf_volume.hpp over the
+// C-ABI of recon.h.  Every call waits for its stream, as the reference's
+// calls wait for the device (tsdf_volume.cu:253,287; marching_cubes.cu:223).
+#include "recon/tsdf_volume.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <fstream>
+
+namespace cuphoto {
+
+namespace {
+
+void log_error(const char* where) {
+    std::fprintf(stderr, "[cuphoto] %s -- %s\n", where, recon_last_error());
+}
+
+recon_tsdf_config to_c(const TSDFVolumeConfig& c) {
+    recon_tsdf_config r{};
+    r.voxel_grid_size[0] = (int32_t)c.voxel_grid_size.x;
+    r.voxel_grid_size[1] = (int32_t)c.voxel_grid_size.y;
+    r.voxel_grid_size[2] = (int32_t)c.voxel_grid_size.z;
+    r.physical_size[0] = c.physical_size.x;
+    r.physical_size[1] = c.physical_size.y;
+    r.physical_size[2] = c.physical_size.z;
+    r.global_offset[0] = c.global_offset.x;
+    r.global_offset[1] = c.global_offset.y;
+    r.global_offset[2] = c.global_offset.z;
+    r.max_weight = c.max_weight;
+    r.focal_x = c.focal_x;
+    r.focal_y = c.focal_y;
+    r.cx = c.cx;
+    r.cy = c.cy;
+    r.camera_width = c.camera_width;
+    r.camera_height = c.camera_height;
+    r.max_cell_size = c.max_cell_size;
+    r.truncation = c.truncation;
+    r.max_depth = c.max_depth;
+    return r;
+}
+
+bool sync(hipStream_t s, const char* where) {
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "[cuphoto] %s -- %s\n", where, hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+TSDFVolume::TSDFVolume(const TSDFVolumeConfig& cfg, hipStream_t stream) : cfg_(cfg), stream_(stream) {
+    const recon_tsdf_config c = to_c(cfg);
+    if (recon_tsdf_create(&c, &h_) != RECON_OK) {
+        log_error("TSDFVolume::TSDFVolume()");
+        h_ = nullptr;
+    }
+}
+
+TSDFVolume::~TSDFVolume() { recon_tsdf_destroy(h_); }
+
+bool TSDFVolume::integrate_raw(const uint8_t* rgb, size_t rgb_step, const float* depth, size_t step,
+                               int rows, int cols, const std::array<r64, 7>& pose) {
+    if (!h_) {
+        std::fprintf(stderr, "[cuphoto] TSDFVolume::integrate() -- volume was not created\n");
+        return false;
+    }
+    const recon_depth_view dv{depth, step, rows, cols};
+    if (recon_tsdf_integrate(h_, &dv, rgb, rgb_step, pose.data(), stream_) != RECON_OK) {
+        log_error("TSDFVolume::integrate()");
+        return false;
+    }
+    return sync(stream_, "TSDFVolume::integrate()");
+}
+
+bool TSDFVolume::integrate_frames(const float* depth0, size_t step, int rows, int cols,
+                                  size_t frame_stride_bytes,
+                                  const std::vector<std::array<r64, 7>>& poses) {
+    if (!h_) return false;
+    const recon_depth_view dv{depth0, step, rows, cols};
+    std::vector<double> flat;
+    flat.reserve(7 * poses.size());
+    for (const auto& p : poses) flat.insert(flat.end(), p.begin(), p.end());
+    if (recon_tsdf_integrate_frames(h_, &dv, frame_stride_bytes, (int32_t)poses.size(), flat.data(),
+                                    stream_) != RECON_OK) {
+        log_error("TSDFVolume::integrate_frames()");
+        return false;
+    }
+    return sync(stream_, "TSDFVolume::integrate_frames()");
+}
+
+r32* TSDFVolume::voxel_distances_data() const { return recon_tsdf_distances(h_); }
+r32* TSDFVolume::voxel_weights_data() const { return recon_tsdf_weights(h_); }
+
+float3 TSDFVolume::scale_voxel_size() const {
+    recon_tsdf_info info{};
+    if (!h_ || recon_tsdf_get_info(h_, &info) != RECON_OK) return make_float3(0.f, 0.f, 0.f);
+    return make_float3(info.voxel_size[0], info.voxel_size[1], info.voxel_size[2]);
+}
+
+r32 TSDFVolume::truncation() const {
+    recon_tsdf_info info{};
+    if (!h_ || recon_tsdf_get_info(h_, &info) != RECON_OK) return 0.f;
+    return info.truncation;
+}
+
+void TSDFVolume::reset() {
+    if (h_ && recon_tsdf_reset(h_, stream_) == RECON_OK) sync(stream_, "TSDFVolume::reset()");
+}
+
+void generate_triangular_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
+                                 std::vector<int3>& triangles) {
+    recon_tsdf* h = tsdf_volume.handle();
+    hipStream_t s = tsdf_volume.stream();
+    int64_t n = 0;
+    if (!h || recon_tsdf_count_triangles(h, &n, s) != RECON_OK) {
+        log_error("generate_triangular_surface()");
+        return;
+    }
+    if (n == 0) return;
+    float* dverts = nullptr;
+    if (hipMalloc((void**)&dverts, sizeof(float) * 9 * (size_t)n) != hipSuccess) {
+        std::fprintf(stderr, "[cuphoto] generate_triangular_surface() -- out of device memory\n");
+        return;
+    }
+    int64_t got = 0;
+    if (recon_tsdf_extract_mesh(h, dverts, nullptr, n, &got, s) != RECON_OK) {
+        log_error("generate_triangular_surface()");
+        (void)hipFree(dverts);
+        return;
+    }
+    // append, as the reference's host loop does (marching_cubes.cu:176-183):
+    // triangle t refers to the three vertices it appended
+    const size_t v0 = verts.size();
+    verts.resize(v0 + 3 * (size_t)got);
+    const hipError_t e = hipMemcpy(verts.data() + v0, dverts, sizeof(float) * 9 * (size_t)got,
+                                   hipMemcpyDeviceToHost);
+    (void)hipFree(dverts);
+    if (e != hipSuccess) {
+        verts.resize(v0);
+        std::fprintf(stderr, "[cuphoto] generate_triangular_surface() -- %s\n", hipGetErrorString(e));
+        return;
+    }
+    triangles.reserve(triangles.size() + (size_t)got);
+    for (int64_t t = 0; t < got; ++t) {
+        const int b = (int)(v0 + 3 * t);
+        triangles.push_back(make_int3(b, b + 1, b + 2));
+    }
+}
+
+PointCloud::PointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
+                       hipStream_t stream)
+    : K_(camera_mat), total_(total_number_pts), stream_(stream) {
+    if (total_ == 0) return;
+    if (hipMalloc((void**)&xyz_, 12 * total_) != hipSuccess ||
+        hipMalloc((void**)&valid_, total_) != hipSuccess ||
+        hipMalloc((void**)&rgb_, 3 * total_) != hipSuccess) {
+        std::fprintf(stderr, "[cuphoto] PointCloud -- can't allocate %zu points\n", total_);
+        total_ = 0;
+    }
+}
+
+PointCloud::~PointCloud() {
+    if (xyz_) (void)hipFree(xyz_);
+    if (valid_) (void)hipFree(valid_);
+    if (rgb_) (void)hipFree(rgb_);
+}
+
+bool PointCloud::extract_raw(const float* depth, size_t step, int rows, int cols,
+                             const uint8_t* rgb, size_t rgb_step, const std::array<r64, 7>& T,
+                             int frame_idx) {
+    if (!depth) {
+        std::fprintf(stderr, "[cuphoto] PointCloud::extract_points() -- depth map is null pointer\n");
+        return false;
+    }
+    const size_t n = (size_t)rows * (size_t)cols;
+    const size_t start = (size_t)frame_idx * n;
+    if (frame_idx < 0 || start + n > total_) {
+        std::fprintf(stderr, "[cuphoto] PointCloud::extract_points() -- frame %d does not fit\n",
+                     frame_idx);
+        return false;
+    }
+    const recon_depth_view dv{depth, step, rows, cols};
+    // column-major K: cx, cy at K[2][0], K[2][1] (cuda_point_cloud.cu:44-47)
+    if (recon_depth_to_points(&dv, K_[0], K_[4], K_[6], K_[7], T.data(), rgb, rgb_step,
+                              xyz_ + 3 * start, valid_ + start, rgb_ + 3 * start, stream_) != RECON_OK) {
+        log_error("PointCloud::extract_points()");
+        return false;
+    }
+    num_ += n;  // cuda_point_cloud.cu:221 counts every pixel
+    return sync(stream_, "PointCloud::extract_points()");
+}
+
+bool PointCloud::save_ply(const std::string& path) const {
+    if (path.empty() || num_ == 0) return false;
+    std::vector<float> xyz(3 * num_);
+    std::vector<uint8_t> valid(num_), rgb(3 * num_);
+    if (hipMemcpy(xyz.data(), xyz_, 12 * num_, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(valid.data(), valid_, num_, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(rgb.data(), rgb_, 3 * num_, hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    size_t nv = 0;
+    for (uint8_t v : valid) nv += v;
+    std::ofstream of(path);
+    if (!of) return false;
+    of << "ply\nformat ascii 1.0\nelement vertex " << nv
+       << "\nproperty float x\nproperty float y\nproperty float z\nproperty uchar red\n"
+          "property uchar green\nproperty uchar blue\nend_header\n";
+    for (size_t i = 0; i < num_; ++i) {
+        if (!valid[i]) continue;
+        of << xyz[3 * i] << " " << xyz[3 * i + 1] * -1.f << " " << xyz[3 * i + 2] * -1.f << " "
+           << (int)rgb[3 * i] << " " << (int)rgb[3 * i + 1] << " " << (int)rgb[3 * i + 2] << "\n";
+    }
+    return (bool)of;
+}
+
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<int3>& triangles) {
+    std::ofstream of(file_name);
+    if (!of.is_open()) return false;
+    of << "ply"
+       << '\n' << "format ascii 1.0"
+       << '\n' << "element vertex " << vertices.size()
+       << '\n' << "property float x"
+       << '\n' << "property float y"
+       << '\n' << "property float z"
+       << '\n' << "element face " << triangles.size()
+       << '\n' << "property list uchar int vertex_indices"
+       << '\n' << "end_header\n";
+    for (const float3& v : vertices) of << v.x << " " << v.y << " " << v.z << "\n";
+    for (const int3& t : triangles) of << "3 " << t.x << " " << t.y << " " << t.z << "\n";
+    return (bool)of;
+}
+
+}  // namespace cuphoto

@@ -90,9 +90,10 @@ def build(verbose: bool = False, jobs: int = 8) -> dict[str, Path]:
         test_src = ROOT / "tests" / "cpp" / "tsdf_wrapper_test.cpp"
         if test_src.exists():
             exe = LIB / "tsdf_wrapper_test"
-            _run([HIPCC, *HIP_FLAGS, test_src, f"-I{PKG / 'scene'}", f"-L{LIB}",
-                  "-lrecon_host", "-lrecon", "-lrecon_scene", "-Wl,-rpath,$ORIGIN", "-o", exe],
-                 verbose)
+            oracle_lib = ROOT / "oracle" / "lib"
+            _run([HIPCC, *HIP_FLAGS, test_src, f"-I{PKG / 'scene'}", f"-I{ROOT / 'oracle'}",
+                  f"-L{LIB}", f"-L{oracle_lib}", "-lrecon_host", "-lrecon", "-lrecon_scene",
+                  "-loracle", "-Wl,-rpath,$ORIGIN:$ORIGIN/../../oracle/lib", "-o", exe], verbose)
             out["wrapper_test"] = exe
     return out
 

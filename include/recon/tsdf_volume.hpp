@@ -1,0 +1,158 @@
+// tsdf_volume.hpp -- C++ drop-in for the TSDF API of cu-Reconstruction,
+// layered on the C-ABI of recon.h (librecon_host.so over librecon.so).
+//
+// Same names, argument meanings and error behaviour as the reference:
+//   TSDFVolumeConfig             modules/include/cuda/tsdf_volume.cuh:17-29
+//   TSDFVolume                   modules/include/cuda/tsdf_volume.cuh:31-105
+//   generate_triangular_surface  modules/include/cuda/marching_cubes.cuh:13-15
+//   cudaPointCloud::extract_points  modules/include/cuda/cuda_point_cloud.cuh:55-58
+//   write_mesh_to_ply            modules/src/utils.cpp:96-118
+// Calls block until the device work is done (the reference synchronises the
+// device after every kernel) and report failure as `false` plus a log line on
+// stderr.  Images are anything shaped like cv::cuda::PtrStepSz<T> (members
+// data, step in bytes, rows, cols), so OpenCV's types pass through unchanged.
+// Differences that follow the build contract: colour is optional, the weight
+// cap is applied, the sdf uses camera-space z, marching cubes uses the gx*gy
+// layer stride, and voxel centres are computed, not stored (there is no
+// transformation_voxels_data()).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "recon/recon.h"
+
+namespace cuphoto {
+
+using r32 = float;
+using r64 = double;
+using ui32 = uint32_t;
+
+struct TSDFVolumeConfig {
+    dim3 voxel_grid_size;
+    float3 physical_size;   // in physical units
+    float3 global_offset;   // grid offset
+    r32 max_weight;
+    r32 focal_x;
+    r32 focal_y;
+    r32 cx;
+    r32 cy;
+    ui32 camera_width;
+    ui32 camera_height;
+    r32 max_cell_size = 0.0f;
+    // additions (0 keeps the reference's constants 1.1*|voxel_size| and 24 m)
+    r32 truncation = 0.0f;
+    r32 max_depth = 0.0f;
+};
+
+// Device image view with the members of cv::cuda::PtrStepSz<T>.
+template <typename T>
+struct PtrStepSz {
+    T* data = nullptr;
+    size_t step = 0;
+    int rows = 0;
+    int cols = 0;
+    explicit operator bool() const { return data != nullptr; }
+};
+using PtrStepSzf = PtrStepSz<float>;
+using PtrStepSzb = PtrStepSz<unsigned char>;
+
+class TSDFVolume {
+public:
+    using Ptr = std::shared_ptr<TSDFVolume>;
+
+    explicit TSDFVolume(const TSDFVolumeConfig& cfg, hipStream_t stream = nullptr);
+    ~TSDFVolume();
+    TSDFVolume(const TSDFVolume&) = delete;
+    TSDFVolume& operator=(const TSDFVolume&) = delete;
+
+    // tsdf_volume.cu:257-290; `color` is accepted and ignored.
+    template <typename Color, typename Depth>
+    bool integrate(const Color& color, const Depth& depth, const std::array<r64, 7>& camera_pose) {
+        return integrate_raw(reinterpret_cast<const uint8_t*>(color.data), color.step,
+                             reinterpret_cast<const float*>(depth.data), depth.step, depth.rows,
+                             depth.cols, camera_pose);
+    }
+    // Frames f = 0..n-1 of a (n, rows, cols) stack, applied in order.
+    bool integrate_frames(const float* depth0, size_t step, int rows, int cols,
+                          size_t frame_stride_bytes, const std::vector<std::array<r64, 7>>& poses);
+
+    r32* voxel_distances_data() const;
+    r32* voxel_weights_data() const;
+    dim3 voxel_grid_size() const { return cfg_.voxel_grid_size; }
+    float3 global_offset() const { return cfg_.global_offset; }
+    float3 scale_voxel_size() const;
+    float3 voxel_phys_size() const { return cfg_.physical_size; }
+    r32 truncation() const;
+
+    bool ok() const { return h_ != nullptr; }
+    void reset();
+    recon_tsdf* handle() const { return h_; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    bool integrate_raw(const uint8_t* rgb, size_t rgb_step, const float* depth, size_t step,
+                       int rows, int cols, const std::array<r64, 7>& pose);
+
+    TSDFVolumeConfig cfg_;
+    recon_tsdf* h_ = nullptr;
+    hipStream_t stream_ = nullptr;
+};
+
+// marching_cubes.cu:191-236: appends the triangle soup of the volume's zero
+// level set (three fresh vertices per triangle, cube order z, y, x, slot).
+void generate_triangular_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
+                                 std::vector<int3>& triangles);
+
+// Device point buffer filled by extract_points (cudaPointCloud's role on the
+// TSDF path): xyz float3, a validity byte and an rgb triple per point.
+class PointCloud {
+public:
+    // camera_mat: column-major 3x3 intrinsics, as the reference passes them
+    // (K[0][0], K[1][1], K[2][0], K[2][1] are fx, fy, cx, cy;
+    // cuda_point_cloud.cu:44-47).
+    PointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
+               hipStream_t stream = nullptr);
+    ~PointCloud();
+    PointCloud(const PointCloud&) = delete;
+    PointCloud& operator=(const PointCloud&) = delete;
+
+    // cuda_point_cloud.cu:179-226: back-projects frame `frame_idx` into
+    // points [frame_idx*W*H, (frame_idx+1)*W*H).
+    template <typename Depth, typename Color>
+    bool extract_points(const Depth& depth, const Color& colors, const std::array<r64, 7>& transform,
+                        int frame_idx) {
+        return extract_raw(reinterpret_cast<const float*>(depth.data), depth.step, depth.rows,
+                           depth.cols, reinterpret_cast<const uint8_t*>(colors.data), colors.step,
+                           transform, frame_idx);
+    }
+    float* xyz() const { return xyz_; }
+    uint8_t* valid() const { return valid_; }
+    uint8_t* colors() const { return rgb_; }
+    size_t get_total_num_points() const { return total_; }
+    size_t get_num_points() const { return num_; }
+    // copies the valid points to the host and writes an ASCII PLY
+    // (cuda_point_cloud.cu:459-495, including its y/z sign flip)
+    bool save_ply(const std::string& path) const;
+
+private:
+    bool extract_raw(const float* depth, size_t step, int rows, int cols, const uint8_t* rgb,
+                     size_t rgb_step, const std::array<r64, 7>& T, int frame_idx);
+    std::array<r64, 9> K_;
+    size_t total_ = 0, num_ = 0;
+    float* xyz_ = nullptr;
+    uint8_t* valid_ = nullptr;
+    uint8_t* rgb_ = nullptr;
+    hipStream_t stream_ = nullptr;
+};
+
+// utils.cpp:96-118: ASCII PLY of vertices and triangles.
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<int3>& triangles);
+
+}  // namespace cuphoto
