@@ -9,10 +9,13 @@ TSDFVolume::integrate path), with a marching-cubes extraction after every
 
 Printed value: swept voxel-updates per second over the whole step
 (512^3 * 300 / step time), in GVoxel-updates/s.  The roofline object
-covers the integrate kernel: algorithmic bytes per launch
+covers one integrate call (its three launches: depth_bounds_kernel,
+cull_kernel, integrate_list_kernel): algorithmic bytes per call
 (16 B per updated voxel + 4 B per depth pixel, the update count measured
 exactly by the kernel's counting mode outside the timed region) divided by
-its average launch duration from HIP events on the launch stream.
+the call's average duration from HIP events on the launch stream; traffic
+is the HBM bytes per call from the committed rocprofv3 summary
+(profiles/integrate_plain_large_512.md).
 """
 from __future__ import annotations
 
@@ -28,6 +31,18 @@ sys.path.insert(0, str(ROOT / "cu-reconstruction_amd"))
 
 METRIC = "GVoxel-updates/s + % HBM roofline; batched vs looped; fwd+bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
+PROFILE = ROOT / "profiles" / "integrate_plain_large_512.md"
+
+
+def profiled_traffic():
+    """HBM bytes per integrate call measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-kernel scaled) and committed with the commands that
+    produced them; None when no such summary is present."""
+    import re
+    if not PROFILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_call: (\d+)", PROFILE.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def log(msg: str) -> None:
@@ -66,6 +81,10 @@ def main() -> int:
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only", choices=["integrate", "copy"],
+                    help="profiling aids: 'integrate' runs --steps passes of the per-frame "
+                         "integrate only; 'copy' copies a 256 MiB buffer 20 times (the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the TSDF path here is single-GPU "
@@ -82,6 +101,14 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.only == "copy":
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 20 x 256 MiB")
+        return 0
     w = scenes.plain_large(num_frames=args.frames)
     t0 = time.perf_counter()
     depth = w.render(nthreads=min(os.cpu_count() or 1, 16))
@@ -91,6 +118,14 @@ def main() -> int:
     vol = recon.TSDFVolume(w.config())
     stream = torch.cuda.current_stream()
     F = w.num_frames
+    if args.only == "integrate":
+        for s in range(args.steps):
+            vol.reset(stream)
+            for f in range(F):
+                vol.integrate(dev[f], w.poses[f], stream)
+        torch.cuda.synchronize()
+        log(f"integrated {args.steps} x {F} frames")
+        return 0
     mesh_at = [f for f in range(F) if w.mesh_every and (f + 1) % w.mesh_every == 0]
 
     # exact update counts U_f (counting mode, outside the timed region)
@@ -192,7 +227,10 @@ def main() -> int:
                    "frames": F, "image": list(w.image), "mesh_every": w.mesh_every,
                    "integrate": "one launch per frame (TSDFVolume::integrate)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(),
+                     "traffic_source": str(PROFILE.relative_to(ROOT)) if PROFILE.exists() else None,
+                     "algorithmic_bytes": int(alg_bytes.mean())},
         "integrate_ms_per_launch": round(avg_launch_ms, 4),
         "integrate_gvu_s_swept": round(w.num_voxels / (avg_launch_ms / 1000.0) / 1e9, 2),
         "integrate_gvu_s_touched": round(float(U.mean()) / (avg_launch_ms / 1000.0) / 1e9, 2),
