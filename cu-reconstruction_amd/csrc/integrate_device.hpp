@@ -1,0 +1,354 @@
+// integrate_device.hpp -- device code of the projective integrate shared by
+// the single-volume path (tsdf_volume.hip) and the batched path
+// (tsdf_batch.hip): depth bounds, frustum/depth culling of wave-tiles and the
+// per-voxel update of cu_tsdf_integrate (cu-Reconstruction
+// modules/src/cuda/tsdf_volume.cu:144-190, camera_util.cu:7-51,
+// se3.cuh:82-100) under the arithmetic contract of recon_common.hpp.
+#pragma once
+
+#include "recon_common.hpp"
+
+namespace recon {
+
+constexpr int kThreads = 256;
+
+// Wave-tile of 4*LX x (64/LX) x NR voxels.  Lane l owns voxels x..x+3
+// (x = x0 + 4*(l % LX)) of row y = y0 + l / LX for z = z0 .. z0+NR-1.  With
+// LX = 8 every row segment of a wave is one aligned 128-byte line of D and W
+// when gx is a multiple of 32.
+template <int LX, int NR>
+struct TileShape {
+    static constexpr int kLanesX = LX;
+    static constexpr int kRows = NR;
+    static constexpr int X = 4 * LX;
+    static constexpr int Y = 64 / LX;
+    static constexpr int Z = NR;
+};
+using DefaultShape = TileShape<8, 2>;
+
+// Frame parameters by value (kernel arguments) or from a device table; both
+// are read as `frames.f[i]`.
+template <int MAXF>
+struct FrameBlock {
+    FrameParams f[MAXF];
+};
+struct FrameTable {
+    const FrameParams* f;
+};
+
+struct ListEntry {
+    uint32_t tile;     // wave-tile index x + tiles_x * (y + tiles_y * z)
+    uint32_t volume;   // volume index (batched path), 0 otherwise
+    uint64_t frames;   // bit f: frame f of the volume may update the tile
+};
+
+// Max over a voxel-centre box of n.c + k.
+__device__ __forceinline__ float plane_max(const float* p, const float lo[3], const float hi[3]) {
+    float g = p[3];
+    g = g + fmaxf(p[0] * lo[0], p[0] * hi[0]);
+    g = g + fmaxf(p[1] * lo[1], p[1] * hi[1]);
+    g = g + fmaxf(p[2] * lo[2], p[2] * hi[2]);
+    return g;
+}
+
+template <class S>
+__device__ __forceinline__ void tile_origin(const VolumeParams& vp, uint32_t id, int& x0, int& y0,
+                                            int& z0) {
+    const uint32_t tx = id % (uint32_t)vp.tiles_x;
+    const uint32_t tyz = id / (uint32_t)vp.tiles_x;
+    x0 = (int)tx * S::X;
+    y0 = (int)(tyz % (uint32_t)vp.tiles_y) * S::Y;
+    z0 = (int)(tyz / (uint32_t)vp.tiles_y) * S::Z;
+}
+
+// ---- depth bounds -------------------------------------------------------------
+// One 32x32 pixel block (workgroup of 256 threads): the largest valid depth
+// (0 < d <= maxd) of each 8x8 sub-block into bound0 and of the block into
+// bound1, 0 when none is valid.  Valid depths are positive floats, so their
+// bit patterns order like ints.
+struct BoundsJob {
+    const float* depth;
+    float* bound0;   // b0x * b0y floats
+    float* bound1;   // b1x * b1y floats (bound0 + b0x*b0y)
+    int32_t pitch, cols, rows;
+    int32_t b0x, b0y, b1x, b1y;
+    float maxd;
+};
+
+__device__ __forceinline__ void bounds_block(const BoundsJob& j, int bx, int by, int* s0, int* s1) {
+    const int t = threadIdx.x;
+    if (t < 16) s0[t] = 0;
+    if (t == 0) *s1 = 0;
+    __syncthreads();
+    const int r = t >> 3, c4 = (t & 7) * 4;
+    const int y = by * kBound1 + r;
+    const int xb = bx * kBound1 + c4;
+    int m = 0;
+    if (y < j.rows) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = xb + k;
+            if (x < j.cols) {
+                const float d = j.depth[y * j.pitch + x];
+                if (d > 0.0f && d <= j.maxd) m = max(m, __float_as_int(d));
+            }
+        }
+    }
+    if (m) {
+        atomicMax(&s0[(r >> 3) * 4 + (c4 >> 3)], m);
+        atomicMax(s1, m);
+    }
+    __syncthreads();
+    if (t < 16) {
+        const int ox = bx * 4 + (t & 3), oy = by * 4 + (t >> 2);
+        if (ox < j.b0x && oy < j.b0y) j.bound0[oy * j.b0x + ox] = __int_as_float(s0[t]);
+    }
+    if (t == 0) j.bound1[by * j.b1x + bx] = __int_as_float(*s1);
+}
+
+// ---- culling ------------------------------------------------------------------
+// True if some block of [bx0,bx1] x [by0,by1] of a bound level holds a depth
+// >= need.  Four independent loads per step keep the reads in flight together.
+__device__ __forceinline__ bool bound_rect_reaches(const float* __restrict__ lvl, int stride,
+                                                   int bx0, int bx1, int by0, int by1,
+                                                   float need) {
+    for (int by = by0; by <= by1; ++by) {
+        const float* row = lvl + by * stride;
+        for (int bx = bx0; bx <= bx1; bx += 4) {
+            // indices past bx1 repeat bx1, which leaves the maximum unchanged
+            const float a = row[bx];
+            const float b = row[min(bx + 1, bx1)];
+            const float c = row[min(bx + 2, bx1)];
+            const float d = row[min(bx + 3, bx1)];
+            if (fmaxf(fmaxf(a, b), fmaxf(c, d)) >= need) return true;
+        }
+    }
+    return false;
+}
+
+// Depth-bound test of a voxel-centre box for one frame (one thread).
+// Returns false only when no voxel of the box can be updated by the frame:
+// the box's eight corners bound the camera z of its voxels and, when all lie
+// in front of the camera, the hull of their projections bounds the voxels'
+// pixel coordinates.  Two pixels of slack cover float32 rounding, the
+// hardware reciprocal used here (<= 1 ulp) and the truncation to an index.
+__device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const FrameParams& F,
+                                              const float lo[3], const float hi[3]) {
+    float zmin = 3.0e38f, umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float cx = (i & 1) ? hi[0] : lo[0];
+        const float cy = (i & 2) ? hi[1] : lo[1];
+        const float cz = (i & 4) ? hi[2] : lo[2];
+        const float px = ((F.R[0] * cx + F.R[1] * cy) + F.R[2] * cz) + F.t[0];
+        const float py = ((F.R[3] * cx + F.R[4] * cy) + F.R[5] * cz) + F.t[1];
+        const float pz = ((F.R[6] * cx + F.R[7] * cy) + F.R[8] * cz) + F.t[2];
+        const float rz = __builtin_amdgcn_rcpf(pz);
+        const float u = (px * vp.fx) * rz + vp.cx, v = (py * vp.fy) * rz + vp.cy;
+        zmin = fminf(zmin, pz);
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+    }
+    // corners at or behind the camera: the footprint is unbounded, keep
+    if (!(zmin > 1e-3f)) return true;
+    umin = fmaxf(umin - 2.0f, 0.0f); umax = fminf(umax + 2.0f, vp.fcols - 1.0f);
+    vmin = fmaxf(vmin - 2.0f, 0.0f); vmax = fminf(vmax + 2.0f, vp.frows - 1.0f);
+    if (!(umin <= umax && vmin <= vmax)) return false;  // footprint leaves the image
+    const int iu0 = (int)umin, iu1 = (int)umax, iv0 = (int)vmin, iv1 = (int)vmax;
+    const float margin = 1e-5f * (fabsf(zmin) + vp.trunc) + F.zmargin;
+    const float need = zmin - vp.trunc - margin;
+    // 32x32 blocks first (a superset of the footprint's 8x8 blocks): a coarse
+    // maximum below `need` settles the test with a few loads
+    if (!bound_rect_reaches(F.bound1, vp.b1x, iu0 / kBound1, iu1 / kBound1, iv0 / kBound1,
+                            iv1 / kBound1, need))
+        return false;
+    const int bx0 = iu0 / kBound0, bx1 = iu1 / kBound0, by0 = iv0 / kBound0, by1 = iv1 / kBound0;
+    if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 96) return true;
+    return bound_rect_reaches(F.bound0, vp.b0x, bx0, bx1, by0, by1, need);
+}
+
+// Frames (bit f of the result) whose frustum and depth bound reach wave-tile
+// `id` of a volume.
+template <class S, class Frames>
+__device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const int nframes,
+                                                const Frames& frames, uint32_t id) {
+    int x0, y0, z0;
+    tile_origin<S>(vp, id, x0, y0, z0);
+    float lo[3], hi[3];
+    lo[0] = voxel_centre(x0, vp.vs[0], vp.off[0]);
+    lo[1] = voxel_centre(y0, vp.vs[1], vp.off[1]);
+    lo[2] = voxel_centre(z0, vp.vs[2], vp.off[2]);
+    hi[0] = voxel_centre(min(x0 + S::X - 1, vp.gx - 1), vp.vs[0], vp.off[0]);
+    hi[1] = voxel_centre(min(y0 + S::Y - 1, vp.gy - 1), vp.vs[1], vp.off[1]);
+    hi[2] = voxel_centre(min(z0 + S::Z - 1, vp.gz - 1), vp.vs[2], vp.off[2]);
+    uint64_t live = 0;
+    for (int f = 0; f < nframes; ++f) {
+        const FrameParams& F = frames.f[f];
+        bool in = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) in = in && (plane_max(F.plane[k], lo, hi) >= 0.0f);
+        if (in && depth_reaches(vp, F, lo, hi)) live |= 1ull << f;
+    }
+    return live;
+}
+
+// Appends the calling workgroup's surviving tiles (`live` != 0) to `list`
+// with one atomic per workgroup.  Every thread of the workgroup must call.
+__device__ __forceinline__ void append_tile(uint64_t live, uint32_t tile, uint32_t volume,
+                                            ListEntry* __restrict__ list,
+                                            unsigned* __restrict__ list_count, unsigned* s_wave,
+                                            unsigned* s_base) {
+    const unsigned n = live != 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    unsigned wave_base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i) {
+        wave_base += i < wave ? s_wave[i] : 0u;
+        total += s_wave[i];
+    }
+    if (threadIdx.x == 0) *s_base = total ? atomicAdd(list_count, total) : 0u;
+    __syncthreads();
+    if (n) list[*s_base + wave_base + incl - n] = ListEntry{tile, volume, live};
+}
+
+// ---- the per-voxel update -------------------------------------------------------
+// Projection is branch-free so the 4*NR depth gathers of a frame issue back
+// to back; D and W of a row are read only if one of its voxels passes, and
+// written once after all frames (frames are applied in order, so the result
+// equals one sweep per frame).
+template <class S, class Frames, bool VEC, bool COUNT>
+__device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int nframes,
+                                               const Frames& frames, const uint64_t live,
+                                               const int x0, const int y0, const int z0,
+                                               unsigned& cnt) {
+    constexpr int NR = S::kRows, NV = 4 * S::kRows;
+    float* __restrict__ D = vp.D;
+    float* __restrict__ W = vp.W;
+    const int lane = threadIdx.x & 63;
+    const int x = x0 + 4 * (lane % S::kLanesX);
+    const int y = y0 + lane / S::kLanesX;
+    if (x >= vp.gx || y >= vp.gy) return;
+    const int nvox = VEC ? 4 : min(4, vp.gx - x);
+    int base[NR];
+    bool row_ok[NR];
+    float czw[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int z = z0 + r;
+        row_ok[r] = z < vp.gz;
+        base[r] = x + vp.gx * (y + vp.gy * min(z, vp.gz - 1));
+        czw[r] = voxel_centre(z, vp.vs[2], vp.off[2]);
+    }
+    const float cyw = voxel_centre(y, vp.vs[1], vp.off[1]);
+    float cxw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cxw[j] = voxel_centre(x + j, vp.vs[0], vp.off[0]);
+
+    float d[NV], w[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) d[j] = w[j] = 0.0f;
+    bool loaded[NR], dirty[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) loaded[r] = dirty[r] = false;
+
+    for (int f = 0; f < nframes; ++f) {
+        if (!((live >> f) & 1ull)) continue;
+        const FrameParams& F = frames.f[f];
+        const float rxy = F.R[1] * cyw, ryy = F.R[4] * cyw, rzy = F.R[7] * cyw;
+        bool ok[NV];
+        float pz[NV], dep[NV];
+        int addr[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int r = j >> 2, i = j & 3;
+            const float px = ((F.R[0] * cxw[i] + rxy) + F.R[2] * czw[r]) + F.t[0];
+            const float py = ((F.R[3] * cxw[i] + ryy) + F.R[5] * czw[r]) + F.t[1];
+            pz[j] = ((F.R[6] * cxw[i] + rzy) + F.R[8] * czw[r]) + F.t[2];
+            int u, v;
+            const bool in_z = pz[j] > 0.0f;
+            const bool in_u = pixel_index(px, vp.fx, pz[j], vp.cx, vp.fcols, u);
+            const bool in_v = pixel_index(py, vp.fy, pz[j], vp.cy, vp.frows, v);
+            ok[j] = in_z && in_u && in_v && row_ok[r] && (VEC || i < nvox);
+            addr[j] = ok[j] ? v * F.pitch + u : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NV; ++j) dep[j] = F.depth[addr[j]];
+        bool pass[NV], any[NR];
+        float tsdf[NV];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) any[r] = false;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const float sdf = dep[j] - pz[j];
+            pass[j] = ok[j] && !(dep[j] <= 0.0f || dep[j] > vp.maxd) && (sdf >= -vp.trunc);
+            tsdf[j] = sdf > 0.0f ? fminf(sdf, vp.trunc) : sdf;
+            any[j >> 2] = any[j >> 2] || pass[j];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (any[r] && !loaded[r]) {
+                if (VEC) {
+                    const float4 d4 = *reinterpret_cast<const float4*>(D + base[r]);
+                    const float4 w4 = *reinterpret_cast<const float4*>(W + base[r]);
+                    d[4 * r + 0] = d4.x; d[4 * r + 1] = d4.y; d[4 * r + 2] = d4.z; d[4 * r + 3] = d4.w;
+                    w[4 * r + 0] = w4.x; w[4 * r + 1] = w4.y; w[4 * r + 2] = w4.z; w[4 * r + 3] = w4.w;
+                } else {
+                    for (int i = 0; i < nvox; ++i) {
+                        d[4 * r + i] = D[base[r] + i];
+                        w[4 * r + i] = W[base[r] + i];
+                    }
+                }
+                loaded[r] = true;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            if (!pass[j]) continue;
+            // tsdf_volume.cu:178-186 with the weight cap
+            const float nw = w[j] + 1.0f;
+            d[j] = ((d[j] * w[j]) + tsdf[j]) / nw;
+            w[j] = nw < vp.maxw ? nw : vp.maxw;
+            dirty[j >> 2] = true;
+            if (COUNT) ++cnt;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (!dirty[r]) continue;
+        if (VEC) {
+            *reinterpret_cast<float4*>(D + base[r]) =
+                make_float4(d[4 * r + 0], d[4 * r + 1], d[4 * r + 2], d[4 * r + 3]);
+            *reinterpret_cast<float4*>(W + base[r]) =
+                make_float4(w[4 * r + 0], w[4 * r + 1], w[4 * r + 2], w[4 * r + 3]);
+        } else {
+            for (int i = 0; i < nvox; ++i) {
+                D[base[r] + i] = d[4 * r + i];
+                W[base[r] + i] = w[4 * r + i];
+            }
+        }
+    }
+}
+
+// D <- value_d, W <- 0 (tsdf_volume.cu:21-26,230-238), four voxels per thread.
+__device__ __forceinline__ void fill4(float* __restrict__ D, float* __restrict__ W, long long i,
+                                      long long n, float value_d) {
+    if (i + 3 < n && (i & 3) == 0) {
+        *reinterpret_cast<float4*>(D + i) = make_float4(value_d, value_d, value_d, value_d);
+        *reinterpret_cast<float4*>(W + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        for (long long k = i; k < n && k < i + 4; ++k) {
+            D[k] = value_d;
+            W[k] = 0.0f;
+        }
+    }
+}
+
+}  // namespace recon

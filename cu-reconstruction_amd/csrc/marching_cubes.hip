@@ -17,91 +17,20 @@
 // soup: triangle t owns vertices 3t, 3t+1, 3t+2 (marching_cubes.cu:171).
 #pragma clang fp contract(off)
 
-#include "recon_common.hpp"
-
-#include "recon/mc_tables.h"
+#include "mc_device.hpp"
 
 #include <algorithm>
 
 namespace recon {
 
-__constant__ uint64_t c_tri[256] = RECON_MC_TRI_PACKED_INIT;
-__constant__ int c_edge[12][2] = RECON_MC_EDGE_CORNERS_INIT;
-
-struct McParams {
-    const float* f;
-    int32_t gx, gy, gz;
-    int32_t cx;        // cubes per row = gx - 1
-    int32_t cyr;       // cube rows per layer = gy - 1
-    int32_t rows;      // (gy - 1) * (gz - 1)
-    float vs[3];
-    float off[3];
-};
-
 constexpr int kRowsPerBlock = 4;  // one wavefront per row, 256 threads
-
-// Corner i of cube (x,y,z) at voxel (x,y,z) + offset_i with offsets
-// 0:(0,0,0) 1:(1,0,0) 2:(1,0,1) 3:(0,0,1) 4:(0,1,0) 5:(1,1,0) 6:(1,1,1)
-// 7:(0,1,1) (marching_cubes.cu:118-127).
-__device__ __forceinline__ void load_corners(const McParams& p, int x, int y, int z, float v[8]) {
-    const int sy = p.gx, sz = p.gx * p.gy;
-    const int i0 = x + sy * y + sz * z;
-    v[0] = p.f[i0];
-    v[1] = p.f[i0 + 1];
-    v[2] = p.f[i0 + sz + 1];
-    v[3] = p.f[i0 + sz];
-    v[4] = p.f[i0 + sy];
-    v[5] = p.f[i0 + sy + 1];
-    v[6] = p.f[i0 + sy + sz + 1];
-    v[7] = p.f[i0 + sy + sz];
-}
-
-// define_voxel_type (marching_cubes.cu:76-84)
-__device__ __forceinline__ int cube_class(const float v[8]) {
-    int c = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) c |= (v[i] < 0.0f ? 1 : 0) << i;
-    return c;
-}
-
-__device__ __forceinline__ int class_ntri(int c) {
-    return (c == 0 || c == 255) ? 0 : (int)(c_tri[c] & 0xFull);
-}
-
-__device__ __forceinline__ int row_of_block(int lane_row) {
-    return blockIdx.x * kRowsPerBlock + lane_row;
-}
-
-// Sum over the 64 lanes of a wavefront.
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-// Inclusive prefix sum over the 64 lanes of a wavefront.
-__device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
 
 __global__ __launch_bounds__(256) void mc_row_count_kernel(const McParams p,
                                                            int32_t* __restrict__ row_count) {
     const int lane = threadIdx.x & 63;
-    const int r = row_of_block(threadIdx.x >> 6);
+    const int r = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
     if (r >= p.rows) return;
-    const int y = r % p.cyr, z = r / p.cyr;
-    int n = 0;
-    for (int x = lane; x < p.cx; x += 64) {
-        float v[8];
-        load_corners(p, x, y, z, v);
-        n += class_ntri(cube_class(v));
-    }
-    n = wave_sum(n);
+    const int n = mc_row_triangles(p, r, lane);
     if (lane == 0) row_count[r] = n;
 }
 
@@ -111,7 +40,7 @@ __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
     h[threadIdx.x] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int r = row_of_block(threadIdx.x >> 6);
+    const int r = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
     if (r < p.rows) {
         const int y = r % p.cyr, z = r / p.cyr;
         for (int x = lane; x < p.cx; x += 64) {
@@ -124,26 +53,17 @@ __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
     if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
-// ---- exclusive scan of int32 row counts into int64 offsets -----------------
-constexpr int kScanItems = 4;
-constexpr int kScanBlock = 256 * kScanItems;
-
-__device__ __forceinline__ long long block_exclusive_scan(long long v, long long* sh,
-                                                          long long* total) {
-    // sh: 256 entries of shared scratch
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const long long o = t >= d ? sh[t - d] : 0;
-        __syncthreads();
-        sh[t] += o;
-        __syncthreads();
-    }
-    const long long incl = sh[t];
-    *total = sh[255];
-    __syncthreads();
-    return incl - v;
+__global__ __launch_bounds__(256) void mc_emit_kernel(const McParams p,
+                                                      const int32_t* __restrict__ row_count,
+                                                      const long long* __restrict__ row_offset,
+                                                      float* __restrict__ verts,
+                                                      int32_t* __restrict__ tris,
+                                                      long long capacity) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
+    if (r >= p.rows) return;
+    if (row_count[r] == 0) return;
+    mc_emit_row(p, r, lane, row_offset[r], verts, tris, capacity, 0);
 }
 
 __global__ __launch_bounds__(256) void scan_block_sums_kernel(const int32_t* __restrict__ in, int n,
@@ -195,78 +115,18 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t* __restri
     }
 }
 
-// compute_intersection (marching_cubes.cu:23-51): orient the edge so that it
-// runs from the negative corner when start > 0 and end < 0, then
-// p = s + ((-ws) / (we - ws)) * (e - s), component by component.
-__device__ __forceinline__ void edge_vertex(const McParams& p, int x, int y, int z,
-                                            const float v[8], int e, float out[3]) {
-    int a = c_edge[e][0], b = c_edge[e][1];
-    float wa = v[a], wb = v[b];
-    if ((wa > 0.0f) && (wb < 0.0f)) {
-        const int ti = a; a = b; b = ti;
-        const float tw = wa; wa = wb; wb = tw;
+int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
+                          hipStream_t s) {
+    const int nb = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb), dim3(256), 0, s, counts, n, sums);
+    RECON_HIP_TRY(hipGetLastError(), "scan_block_sums_kernel launch");
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, s, sums, nb);
+    RECON_HIP_TRY(hipGetLastError(), "scan_top_kernel launch");
+    if (offsets) {
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, n, sums, offsets);
+        RECON_HIP_TRY(hipGetLastError(), "scan_apply_kernel launch");
     }
-    // corner offsets packed as bits: dx = bit0, dy = bit1, dz = bit2 of k_off
-    // (0:000 1:001 2:101 3:100 4:010 5:011 6:111 7:110)
-    constexpr unsigned k_off = (0u << 0) | (1u << 3) | (5u << 6) | (4u << 9) | (2u << 12) |
-                               (3u << 15) | (7u << 18) | (6u << 21);
-    const unsigned oa = (k_off >> (3 * a)) & 7u, ob = (k_off >> (3 * b)) & 7u;
-    const float ratio = (-wa) / (wb - wa);
-    const int ia[3] = {x + (int)(oa & 1u), y + (int)((oa >> 1) & 1u), z + (int)(oa >> 2)};
-    const int ib[3] = {x + (int)(ob & 1u), y + (int)((ob >> 1) & 1u), z + (int)(ob >> 2)};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float s = voxel_centre(ia[k], p.vs[k], p.off[k]);
-        const float en = voxel_centre(ib[k], p.vs[k], p.off[k]);
-        const float edge = en - s;
-        const float o = ratio * edge;
-        out[k] = s + o;
-    }
-}
-
-__global__ __launch_bounds__(256) void mc_emit_kernel(const McParams p,
-                                                      const int32_t* __restrict__ row_count,
-                                                      const long long* __restrict__ row_offset,
-                                                      float* __restrict__ verts,
-                                                      int32_t* __restrict__ tris,
-                                                      long long capacity) {
-    const int lane = threadIdx.x & 63;
-    const int r = row_of_block(threadIdx.x >> 6);
-    if (r >= p.rows) return;
-    if (row_count[r] == 0) return;
-    const int y = r % p.cyr, z = r / p.cyr;
-    long long base = row_offset[r];
-    for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        const int x = x0 + lane;
-        float v[8];
-        int c = 0, nt = 0;
-        if (x < p.cx) {
-            load_corners(p, x, y, z, v);
-            c = cube_class(v);
-            nt = class_ntri(c);
-        }
-        const int incl = wave_inclusive_scan(nt, lane);
-        const int chunk_total = __shfl(incl, 63, 64);
-        long long t = base + (incl - nt);
-        if (nt) {
-            const uint64_t word = c_tri[c];
-            for (int s = 0; s < nt; ++s, ++t) {
-                if (t >= capacity) break;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int e = (int)((word >> (4 + 4 * (3 * s + k))) & 0xFull);
-                    float q[3];
-                    edge_vertex(p, x, y, z, v, e, q);
-                    float* dst = verts + 9 * t + 3 * k;
-                    dst[0] = q[0];
-                    dst[1] = q[1];
-                    dst[2] = q[2];
-                    if (tris) tris[3 * t + k] = (int32_t)(3 * t + k);
-                }
-            }
-        }
-        base += chunk_total;
-    }
+    return RECON_OK;
 }
 
 static McParams make_params(const float* field, const int32_t grid[3], const float vs[3],
@@ -325,17 +185,12 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     const int blocks = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
     hipLaunchKernelGGL(mc_row_count_kernel, dim3(blocks), dim3(256), 0, s, p, row_count);
     RECON_HIP_TRY(hipGetLastError(), "mc_row_count_kernel launch");
-    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb), dim3(256), 0, s, row_count, rows, sums);
-    RECON_HIP_TRY(hipGetLastError(), "scan_block_sums_kernel launch");
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, s, sums, nb);
-    RECON_HIP_TRY(hipGetLastError(), "scan_top_kernel launch");
+    int rc = launch_exclusive_scan(row_count, rows, count_only ? nullptr : row_offset, sums, s);
+    if (rc) return rc;
     long long host_total = 0;
     RECON_HIP_TRY(hipMemcpyAsync(&host_total, sums + nb, sizeof(long long), hipMemcpyDeviceToHost, s),
                   "marching cubes total");
     if (!count_only) {
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, row_count, rows, sums,
-                           row_offset);
-        RECON_HIP_TRY(hipGetLastError(), "scan_apply_kernel launch");
         if (capacity > 0 && verts) {
             hipLaunchKernelGGL(mc_emit_kernel, dim3(blocks), dim3(256), 0, s, p, row_count,
                                row_offset, verts, tris, (long long)capacity);

@@ -72,6 +72,8 @@ struct FrameParams {
 constexpr int kBound0 = 8, kBound1 = 32;  // pixel block edges of the two bound levels
 
 struct VolumeParams {
+    float* D;
+    float* W;
     int32_t gx, gy, gz;
     int32_t tiles_x, tiles_y, tiles_z;
     float vs[3];
