@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT / "cu-reconstruction_amd"))
 METRIC = "GVoxel-updates/s + % HBM roofline; batched vs looped; fwd+bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
 PROFILE = ROOT / "profiles" / "integrate_plain_large_512.md"
+TUNE_CACHE = ROOT / "cu-reconstruction_amd" / "tuning" / "batch_mi355x.json"
 
 
 def profiled_traffic():
@@ -73,6 +74,74 @@ def cpu_baseline(w, depth, seconds: float) -> dict:
                       f"{w.name} workload ({dt:.1f} s, OpenMP over voxel rows)"}
 
 
+def bench_batched(steps: int, autotune_to: str | None = None) -> dict:
+    """64 uneven volumes: one graph launch (recon_batch_run) against the loop
+    of the single-volume path over volumes and frames (reset, one integrate
+    call per frame, mesh extraction), with identical results."""
+    import numpy as np
+    import torch
+
+    import recon
+    from recon import scenes
+
+    ws = scenes.batched_uneven()
+    t0 = time.perf_counter()
+    depths = [torch.from_numpy(w.render(nthreads=min(os.cpu_count() or 1, 16))).cuda() for w in ws]
+    torch.cuda.synchronize()
+    log(f"batched: rendered {sum(w.num_frames for w in ws)} frames for {len(ws)} volumes in "
+        f"{time.perf_counter() - t0:.1f} s")
+    spec = [(w.config(), d, w.poses) for w, d in zip(ws, depths)]
+    if autotune_to:
+        Path(autotune_to).parent.mkdir(parents=True, exist_ok=True)
+        recon.Batch.autotune(spec, autotune_to, iterations=3)
+        log(f"batched: autotuned into {autotune_to}")
+    vols = [recon.TSDFVolume(w.config()) for w in ws]
+    stream = torch.cuda.current_stream()
+
+    def looped():
+        counts = []
+        for v, w, d in zip(vols, ws, depths):
+            v.reset(stream)
+            for f in range(w.num_frames):
+                v.integrate(d[f], w.poses[f], stream)
+            verts, _ = v.extract_mesh(stream=stream)
+            counts.append(verts.shape[0])
+        return counts
+
+    counts = looped()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        looped()
+    torch.cuda.synchronize()
+    looped_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    cache = str(TUNE_CACHE) if TUNE_CACHE.exists() else None
+    batch = recon.Batch(spec, mesh_capacity=int(sum(counts) * 1.05) + 1024, tune_cache=cache)
+    batch.run(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(stream)
+    torch.cuda.synchronize()
+    batched_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    # identical results: every mesh size, and D / W of a sample of volumes
+    ranges = batch.mesh_ranges()
+    same = [int(r) for r in ranges[:, 1]] == counts
+    for i in range(0, len(ws), 16):
+        D, W = batch.download(i)
+        D0, W0 = vols[i].download()
+        same = same and np.array_equal(D.view(np.uint32), D0.view(np.uint32)) and \
+            np.array_equal(W.view(np.uint32), W0.view(np.uint32))
+    upd = sum(w.num_voxels * w.num_frames for w in ws)
+    batch.close()
+    for v in vols:
+        v.close()
+    return {"volumes": len(ws), "voxel_frames": upd, "batched_ms": round(batched_ms, 3),
+            "looped_ms": round(looped_ms, 3), "batched_vs_looped": round(looped_ms / batched_ms, 2),
+            "batched_gvu_s_swept": round(upd / (batched_ms / 1000.0) / 1e9, 2),
+            "identical_results": bool(same), "tuned": cache is not None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +150,9 @@ def main() -> int:
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-volume measurement")
+    ap.add_argument("--autotune", metavar="JSON",
+                    help="run the batch autotuner on the batched workload and write its table")
     ap.add_argument("--only", choices=["integrate", "copy"],
                     help="profiling aids: 'integrate' runs --steps passes of the per-frame "
                          "integrate only; 'copy' copies a 256 MiB buffer 20 times (the "
@@ -239,10 +311,15 @@ def main() -> int:
         "fused_frames_ms_per_step": round(fused_ms, 3),
         "fused_frames_gvu_s": round(swept / (fused_ms / 1000.0) / 1e9, 3),
     }
+    vol.close()
+    del dev
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("timing the batched volumes")
+        result["batched"] = bench_batched(max(args.steps, 2), args.autotune)
     if not args.no_cpu:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(w, depth, args.cpu_seconds)
-    vol.close()
     print(json.dumps(result), flush=True)
     return 0
 

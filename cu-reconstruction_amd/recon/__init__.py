@@ -15,7 +15,7 @@ import numpy as np
 from . import ffi
 from .ffi import ReconError, TsdfConfig, TsdfInfo, DepthView, check
 
-__all__ = ["TSDFVolume", "make_config", "depth_view", "depth_to_points", "mc_extract", "mc_count",
+__all__ = ["TSDFVolume", "Batch", "make_config", "depth_view", "depth_to_points", "mc_extract", "mc_count",
            "mc_class_histogram", "ReconError", "TsdfConfig", "TsdfInfo", "ffi"]
 
 
@@ -226,3 +226,93 @@ def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, s
         xyz.data_ptr(), valid.data_ptr(), colors.data_ptr() if colors is not None else None,
         _stream(stream)))
     return (xyz, valid, colors) if with_colors else (xyz, valid)
+
+
+class Batch:
+    """Many volumes integrated and meshed by one graph launch
+    (recon_batch_*).  `volumes` is a list of (config, depth tensor (F, H, W)
+    on the device, poses (F, 7)); the depth tensors are kept alive here
+    because the graph reads them on every run."""
+
+    def __init__(self, volumes, mesh_capacity=0, tune_cache=None):
+        _torch()
+        self._lib = ffi.load()
+        self._keep = []
+        arr = (ffi.BatchVolume * len(volumes))()
+        for i, (cfg, depth, poses) in enumerate(volumes):
+            poses = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, 7))
+            self._keep.append((depth, poses))
+            v = arr[i]
+            v.config = cfg
+            v.num_frames = int(poses.shape[0])
+            if v.num_frames:
+                if depth.dim() != 3 or depth.shape[0] != v.num_frames or depth.stride(2) != 1:
+                    raise ValueError("depth must be (F, H, W) with one frame per pose")
+                v.rows, v.cols = int(depth.shape[1]), int(depth.shape[2])
+                v.depth_step = depth.stride(1) * 4
+                v.frame_stride = depth.stride(0) * 4
+                v.depth = depth.data_ptr()
+                v.poses = poses.ctypes.data
+        self._arr = arr
+        self.n = len(volumes)
+        h = C.c_void_p()
+        check(self._lib.recon_batch_create(arr, self.n, int(mesh_capacity),
+                                           tune_cache.encode() if tune_cache else None, C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def autotune(volumes, cache_path, iterations=3, stream=None):
+        b = (ffi.BatchVolume * len(volumes))()
+        keep = []
+        for i, (cfg, depth, poses) in enumerate(volumes):
+            poses = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, 7))
+            keep.append(poses)
+            b[i].config = cfg
+            b[i].num_frames = int(poses.shape[0])
+            b[i].rows, b[i].cols = int(depth.shape[1]), int(depth.shape[2])
+            b[i].depth_step = depth.stride(1) * 4
+            b[i].frame_stride = depth.stride(0) * 4
+            b[i].depth = depth.data_ptr()
+            b[i].poses = poses.ctypes.data
+        check(ffi.load().recon_batch_autotune(b, len(volumes), str(cache_path).encode(), iterations,
+                                              _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.recon_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, stream=None):
+        check(self._lib.recon_batch_run(self._h, _stream(stream)))
+
+    def info(self, i) -> TsdfInfo:
+        info = TsdfInfo()
+        check(self._lib.recon_batch_volume_data(self._h, i, None, None, C.byref(info)))
+        return info
+
+    def download(self, i, stream=None):
+        info = self.info(i)
+        shape = (info.grid[2], info.grid[1], info.grid[0])
+        D = np.empty(shape, np.float32)
+        W = np.empty(shape, np.float32)
+        check(self._lib.recon_batch_download(self._h, i, D.ctypes.data, W.ctypes.data,
+                                             _stream(stream)))
+        return D, W
+
+    def mesh_ranges(self, stream=None):
+        r = np.zeros(2 * self.n, np.int64)
+        p = C.c_void_p()
+        check(self._lib.recon_batch_meshes(self._h, r.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           C.byref(p), _stream(stream)))
+        return r.reshape(-1, 2)
+
+    def download_mesh(self, i, stream=None):
+        n = int(self.mesh_ranges(stream)[i, 1])
+        out = np.empty((max(n, 1), 3, 3), np.float32)
+        got = C.c_int64()
+        check(self._lib.recon_batch_download_mesh(self._h, i, out.ctypes.data, n, C.byref(got),
+                                                  _stream(stream)))
+        return out[:got.value]

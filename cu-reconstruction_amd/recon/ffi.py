@@ -57,6 +57,12 @@ class TsdfInfo(C.Structure):
     ]
 
 
+class BatchVolume(C.Structure):
+    _fields_ = [("config", TsdfConfig), ("num_frames", C.c_int32), ("rows", C.c_int32),
+                ("cols", C.c_int32), ("depth_step", C.c_size_t), ("frame_stride", C.c_size_t),
+                ("depth", C.c_void_p), ("poses", C.c_void_p)]
+
+
 class DepthView(C.Structure):
     _fields_ = [("data", C.c_void_p), ("step", C.c_size_t), ("rows", C.c_int32),
                 ("cols", C.c_int32)]
@@ -94,6 +100,16 @@ SIGNATURES = {
     "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
     "recon_depth_to_points": (C.c_int, [C.POINTER(DepthView), C.c_double, C.c_double, C.c_double,
                                         C.c_double, _d7, _vp, C.c_size_t, _vp, _vp, _vp, _vp]),
+    "recon_batch_create": (C.c_int, [C.POINTER(BatchVolume), _i32, _i64, C.c_char_p,
+                                     C.POINTER(_vp)]),
+    "recon_batch_destroy": (None, [_vp]),
+    "recon_batch_run": (C.c_int, [_vp, _vp]),
+    "recon_batch_volume_data": (C.c_int, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp),
+                                          C.POINTER(TsdfInfo)]),
+    "recon_batch_meshes": (C.c_int, [_vp, _pi64, C.POINTER(_vp), _vp]),
+    "recon_batch_autotune": (C.c_int, [C.POINTER(BatchVolume), _i32, C.c_char_p, _i32, _vp]),
+    "recon_batch_download": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "recon_batch_download_mesh": (C.c_int, [_vp, _i32, _vp, _i64, _pi64, _vp]),
 }
 
 _lib = None

@@ -185,6 +185,65 @@ RECON_API int recon_depth_to_points(const recon_depth_view* depth, double fx, do
                           float* xyz, uint8_t* valid, uint8_t* colors_out,
                           recon_stream_t stream);
 
+/* ---- batched volumes ----------------------------------------------------- */
+/* No counterpart in the reference, which builds one TSDFVolume per scene and
+ * blocks on every frame (tsdf_surface_reconstruction.cpp:125-138).  A batch
+ * holds many independent volumes (sizes, frame counts and image resolutions
+ * may all differ); one recon_batch_run replays a hipGraph that resets every
+ * volume, integrates all of each volume's frames in order and meshes every
+ * volume.  Results equal recon_tsdf_reset + recon_tsdf_integrate per frame +
+ * recon_tsdf_extract_mesh on each volume. */
+typedef struct recon_batch recon_batch;
+
+typedef struct recon_batch_volume {
+    recon_tsdf_config config;     /* volume and the camera intrinsics of its frames */
+    int32_t num_frames;           /* 0..64 */
+    int32_t rows, cols;           /* depth image size of every frame */
+    size_t depth_step;            /* bytes per depth row */
+    size_t frame_stride;          /* bytes between consecutive frames */
+    const float* depth;           /* device; frame f at depth + f*frame_stride; must stay
+                                     valid while the batch exists */
+    const double* poses;          /* host, num_frames * 7 {qw,qx,qy,qz,tx,ty,tz}; copied */
+} recon_batch_volume;
+
+/* sync.  Plans the batch (device tables, one arena for all D and W, work
+ * lists) and captures its graph.  mesh_capacity_triangles = 0 builds it
+ * without meshing.  tune_cache_path (may be NULL) is a JSON file written by
+ * recon_batch_autotune; volumes whose (grid, image size) class it lists use
+ * the tile shape found there. */
+RECON_API int recon_batch_create(const recon_batch_volume* volumes, int32_t num_volumes,
+                                 int64_t mesh_capacity_triangles, const char* tune_cache_path,
+                                 recon_batch** out);
+RECON_API void recon_batch_destroy(recon_batch* b);
+/* async.  One graph launch: reset, integrate and mesh every volume. */
+RECON_API int recon_batch_run(recon_batch* b, recon_stream_t stream);
+/* Device pointers of volume i's D and W (float32[gz][gy][gx]) and its description. */
+RECON_API int recon_batch_volume_data(recon_batch* b, int32_t i, float** distances, float** weights,
+                                      recon_tsdf_info* info);
+/* sync.  ranges[2i], ranges[2i+1] <- first triangle and triangle count of
+ * volume i in the shared vertex buffer *verts (9 floats per triangle, the
+ * soup order of recon_tsdf_extract_mesh).  RECON_ERR_CAPACITY when the
+ * meshes exceeded the capacity given at creation (the counts are still set). */
+RECON_API int recon_batch_meshes(recon_batch* b, int64_t* ranges, float** verts,
+                                 recon_stream_t stream);
+/* sync.  Copies volume i's D and/or W (either may be NULL) to host memory. */
+RECON_API int recon_batch_download(recon_batch* b, int32_t i, float* host_d, float* host_w,
+                                   recon_stream_t stream);
+/* sync.  Copies volume i's mesh (9 floats per triangle) to host memory;
+ * *num_triangles <- its triangle count; RECON_ERR_CAPACITY if it exceeds
+ * capacity_triangles (nothing is copied then). */
+RECON_API int recon_batch_download_mesh(recon_batch* b, int32_t i, float* host_verts,
+                                        int64_t capacity_triangles, int64_t* num_triangles,
+                                        recon_stream_t stream);
+/* sync.  Tunes the tile shape of each (grid, image size) class of `volumes`:
+ * starting from the variants in cache_path (or the default), each class in
+ * turn tries every variant, timing the whole batch's integrate graph
+ * (`iterations` runs), and keeps a variant that makes it >= 2 % faster.  The
+ * choices are merged into the JSON file cache_path. */
+RECON_API int recon_batch_autotune(const recon_batch_volume* volumes, int32_t num_volumes,
+                                   const char* cache_path, int32_t iterations,
+                                   recon_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
