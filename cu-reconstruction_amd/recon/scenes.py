@@ -312,3 +312,66 @@ def tiny(grid=(16, 12, 10), voxel_size=0.05, image=(32, 24), num_frames=3, seed=
         image=image, prims=[(PRIM_SPHERE, (0.0, 0.0, 0.0, 0.18)), (PRIM_PLANE_Z, (-0.2,))],
         poses=arc_poses(num_frames, step_deg=25.0, radius=1.0, height=0.2, start_deg=10.0),
         noise=(0.002, 0.0, 0.05, seed))
+
+
+def _union_sdf(prims, p: np.ndarray) -> np.ndarray:
+    """Signed distance to the union of primitives, vectorised over (n, 3)
+    points: the same conventions as recon_scene_sdf (solid below a z plane)."""
+    best = np.full(p.shape[0], np.inf)
+    for kind, q in prims:
+        if kind == PRIM_SPHERE:
+            d = np.linalg.norm(p - np.asarray(q[:3]), axis=1) - q[3]
+        elif kind == PRIM_BOX:
+            lo, hi = np.asarray(q[:3]), np.asarray(q[3:6])
+            r = np.abs(p - 0.5 * (lo + hi)) - 0.5 * (hi - lo)
+            d = np.linalg.norm(np.maximum(r, 0.0), axis=1) + np.minimum(r.max(axis=1), 0.0)
+        else:
+            d = p[:, 2] - q[0]
+        best = np.minimum(best, d)
+    return best
+
+
+def _sample_prim_surface(kind, q, m: int, lo: np.ndarray, hi: np.ndarray,
+                         rng: np.random.Generator) -> tuple[np.ndarray, float]:
+    """m points uniform on one primitive's surface (a z plane: its part over
+    the box [lo, hi]) and that surface's area."""
+    if kind == PRIM_SPHERE:
+        v = rng.normal(size=(m, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        return np.asarray(q[:3]) + q[3] * v, 4.0 * math.pi * q[3] ** 2
+    if kind == PRIM_BOX:
+        b0, b1 = np.asarray(q[:3], np.float64), np.asarray(q[3:6], np.float64)
+        s = b1 - b0
+        areas = np.array([s[1] * s[2], s[1] * s[2], s[0] * s[2], s[0] * s[2], s[0] * s[1], s[0] * s[1]])
+        face = rng.choice(6, size=m, p=areas / areas.sum())
+        pts = b0 + rng.uniform(size=(m, 3)) * s
+        axis = face // 2
+        pts[np.arange(m), axis] = np.where(face % 2 == 0, b0[axis], b1[axis])
+        return pts, float(areas.sum())
+    pts = np.c_[rng.uniform(lo[0], hi[0], m), rng.uniform(lo[1], hi[1], m), np.full(m, q[0])]
+    return pts, float((hi[0] - lo[0]) * (hi[1] - lo[1]))
+
+
+def surface_points(w: Workload, n: int, seed: int = 0x5EED0003, margin_voxels: float = 1.5) -> np.ndarray:
+    """n points (float64, (n, 3)) uniform over the visible surface of the
+    scene inside the volume shrunk by `margin_voxels`: each primitive's
+    surface, less the parts inside another primitive, drawn in proportion to
+    its clipped area."""
+    rng = np.random.default_rng(seed)
+    vs = w.voxel_size
+    lo = np.asarray(w.offset, np.float64) + margin_voxels * vs
+    hi = np.asarray(w.offset, np.float64) + np.asarray(w.grid, np.float64) * vs - margin_voxels * vs
+    pools, areas = [], []
+    m = max(4 * n, 4096)
+    for kind, q in w.prims:
+        pts, area = _sample_prim_surface(kind, q, m, lo, hi, rng)
+        keep = np.all((pts >= lo) & (pts <= hi), axis=1) & (_union_sdf(w.prims, pts) >= -1e-9)
+        pools.append(pts[keep])
+        areas.append(area * keep.mean())
+    areas = np.asarray(areas)
+    if areas.sum() <= 0:
+        return np.zeros((0, 3))
+    counts = rng.multinomial(n, areas / areas.sum())
+    out = [pool[rng.choice(len(pool), size=c, replace=c > len(pool))] for pool, c in zip(pools, counts)
+           if c > 0]
+    return np.concatenate(out, axis=0)

@@ -19,7 +19,9 @@
  *     D' = (D*W + tsdf)/(W+1) (tsdf_volume.cu:179-185 ignore max_weight; the
  *     cap follows octree.cu:125-127);
  *   - marching cubes uses the layer stride gx*gy (marching_cubes.cu:211
- *     uses gx+gy, correct only for 2x2xN grids).
+ *     uses gx+gy, correct only for 2x2xN grids);
+ *   - the trilinear sampler weights neighbour (1,1,1) by xyz and (0,1,1) by
+ *     (1-x)yz (tsdf_volume.cu:100-101 swap the two).
  */
 #ifndef RECON_ORACLE_H
 #define RECON_ORACLE_H
@@ -109,6 +111,35 @@ int64_t oracle_point_cloud_from_depth_f64(const float* depth, int32_t rows, int3
                                           double depth_min, double depth_max,
                                           double* xyz, int64_t* pixel_index,
                                           int64_t capacity);
+
+/* ---- differentiable integrate (adjoint_oracle.c), float64 ---------------- */
+/* Rt: per frame a world->camera row-major rotation (9 doubles) then the
+ * translation (3 doubles).  Depth is sampled bilinearly: depth[j][i] sits at
+ * pixel coordinates (i, j); a voxel needs z > 0, its 2x2 taps inside the
+ * image and all four valid.  Otherwise as oracle_tsdf_integrate_f32. */
+void oracle_diff_forward_f64(const oracle_volume* vol, double* D, double* W, const float* depth,
+                             int32_t rows, int32_t cols, const double* Rt, int32_t nframes,
+                             int nthreads);
+/* Trilinear support of a world point (tsdf_volume.cu:46-104). */
+int oracle_trilinear_support(const oracle_volume* vol, const double pw[3], int64_t ids[8],
+                             double coef[8]);
+/* L = (1/K) sum_k w_k r_k^2, r_k = trilinear TSDF at point k; lambda (may be
+ * NULL) <- dL/dD. */
+double oracle_surface_loss_f64(const oracle_volume* vol, const double* D, const double* points,
+                               const double* weights, int64_t npoints, double* lambda);
+/* dL/dxi (6 per frame, xi = (rho, omega), left perturbation) and dL/ddepth
+ * (may be NULL) from lambda = dL/dD of the final volume, by per-voxel
+ * recomputation of the frame decisions. */
+void oracle_diff_adjoint_f64(const oracle_volume* vol, const float* depth, int32_t rows,
+                             int32_t cols, const double* Rt, int32_t nframes, const double* lambda,
+                             double* grad_pose, double* grad_depth);
+/* Per-point weights: 1 when no decision of the point's support voxels (frame
+ * validity, footprint, the -trunc skip, the +trunc clamp) can change under a
+ * perturbation of size <= h of one pose coordinate or one depth tap, and the
+ * samples cross no pixel line with a bilinear slope jump above `kink`. */
+void oracle_stable_points(const oracle_volume* vol, const float* depth, int32_t rows,
+                          int32_t cols, const double* Rt, int32_t nframes, const double* points,
+                          int64_t npoints, double h, double kink, double* weights);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,14 @@ def lib() -> C.CDLL:
                                                         C.c_double, C.c_double, vp, C.c_double,
                                                         C.c_double, vp, vp, i64]),
         }
+        sig.update({
+            "oracle_diff_forward_f64": (None, [vp, vp, vp, vp, i32, i32, vp, i32, C.c_int]),
+            "oracle_trilinear_support": (C.c_int, [vp, vp, vp, vp]),
+            "oracle_surface_loss_f64": (C.c_double, [vp, vp, vp, vp, i64, vp]),
+            "oracle_diff_adjoint_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp]),
+            "oracle_stable_points": (None, [vp, vp, i32, i32, vp, i32, vp, i64, C.c_double,
+                                            C.c_double, vp]),
+        })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
@@ -179,3 +187,90 @@ def point_cloud_from_depth_f64(depth, intrinsics, transform, dmin=0.0, dmax=1e30
                                                 _p(pose_arr(transform)), dmin, dmax, _p(xyz),
                                                 _p(idx), rows * cols)
     return xyz[:n], idx[:n]
+
+
+# ---- differentiable integrate (float64) ----------------------------------------
+
+def rt_from_poses(poses) -> np.ndarray:
+    """(F, 12) float64 world->camera [R row-major | t] from (F, 7) poses."""
+    poses = np.asarray(poses, np.float64).reshape(-1, 7)
+    out = np.empty((poses.shape[0], 12))
+    for f, p in enumerate(poses):
+        w, x, y, z = p[:4] / np.linalg.norm(p[:4])
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        out[f, :9] = R.ravel()
+        out[f, 9:] = p[4:]
+    return out
+
+
+def diff_forward(vol, depth, Rt, nthreads=0):
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    D = np.full(shape(vol), float(np.float32(vol.truncation)), np.float64)
+    W = np.zeros(shape(vol), np.float64)
+    Rt = np.ascontiguousarray(Rt, np.float64)
+    lib().oracle_diff_forward_f64(C.byref(vol), _p(D), _p(W), _p(depth), rows, cols, _p(Rt), F,
+                                  nthreads)
+    return D, W
+
+
+def trilinear_support(vol, point):
+    """(ids (8,) int64, coef (8,) float64) of a world point, or None when the
+    support leaves the grid."""
+    pw = np.ascontiguousarray(point, np.float64)
+    ids = np.empty(8, np.int64)
+    coef = np.empty(8, np.float64)
+    if not lib().oracle_trilinear_support(C.byref(vol), _p(pw), _p(ids), _p(coef)):
+        return None
+    return ids, coef
+
+
+def surface_loss(vol, D, points, weights=None, want_lambda=True):
+    pts = np.ascontiguousarray(points, np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    lam = np.empty(shape(vol), np.float64) if want_lambda else None
+    L = lib().oracle_surface_loss_f64(C.byref(vol), _p(np.ascontiguousarray(D, np.float64)),
+                                      _p(pts), _p(w), pts.shape[0], _p(lam))
+    return (L, lam) if want_lambda else L
+
+
+def diff_adjoint(vol, depth, Rt, lam, want_depth=True):
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    gp = np.empty((F, 6), np.float64)
+    gd = np.empty((F, rows, cols), np.float64) if want_depth else None
+    lib().oracle_diff_adjoint_f64(C.byref(vol), _p(depth), rows, cols,
+                                  _p(np.ascontiguousarray(Rt, np.float64)), F,
+                                  _p(np.ascontiguousarray(lam, np.float64)), _p(gp), _p(gd))
+    return gp, gd
+
+
+def stable_points(vol, depth, Rt, points, h=1e-4, kink=1e-5):
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    pts = np.ascontiguousarray(points, np.float64)
+    w = np.empty(pts.shape[0], np.float64)
+    lib().oracle_stable_points(C.byref(vol), _p(depth), rows, cols,
+                               _p(np.ascontiguousarray(Rt, np.float64)), F, _p(pts), pts.shape[0],
+                               float(h), float(kink), _p(w))
+    return w
+
+
+def se3_exp_left(Rt, xi):
+    """Exp(xi) T for xi = (rho, omega) (float64, Rodrigues)."""
+    R = Rt[:9].reshape(3, 3)
+    t = Rt[9:]
+    rho, om = np.asarray(xi[:3], np.float64), np.asarray(xi[3:], np.float64)
+    th = np.linalg.norm(om)
+    K = np.array([[0, -om[2], om[1]], [om[2], 0, -om[0]], [-om[1], om[0], 0]])
+    if th < 1e-12:
+        Re, V = np.eye(3) + K, np.eye(3) + 0.5 * K
+    else:
+        Re = np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
+        V = np.eye(3) + (1 - np.cos(th)) / th ** 2 * K + (th - np.sin(th)) / th ** 3 * K @ K
+    out = np.empty(12)
+    out[:9] = (Re @ R).ravel()
+    out[9:] = Re @ t + V @ rho
+    return out

@@ -1,0 +1,265 @@
+"""Pins the float64 oracle of the differentiable integrate (bilinear depth
+sampling, an extension of cu_tsdf_integrate, tsdf_volume.cu:144-190), of the
+surface loss on the reference's trilinear sampler (interpolate_trilinearly,
+tsdf_volume.cu:46-104) and of the adjoint of both:
+
+  - the trilinear support against affine fields (which trilinear weights
+    reproduce exactly) and against scipy.ndimage.map_coordinates;
+  - dL/dD against central differences of the loss (quadratic in D, so the
+    difference is exact up to rounding);
+  - the bilinear forward against a closed form: a depth map linear in pixel
+    coordinates is sampled exactly by bilinear interpolation;
+  - the weight cap, the +trunc clamp and the -trunc skip of the adjoint
+    against a hand-derived single-voxel case;
+  - pose and depth gradients against central finite differences of the loss
+    (step 1e-4 on smooth depth; 1e-6 on the noisy depth of the small plain
+    workload), on the points whose decisions the step cannot change.
+
+Relative error of a gradient is max|fd - g| / max|g| over all its entries."""
+import numpy as np
+import pytest
+from scipy.ndimage import map_coordinates
+from scipy.spatial.transform import Rotation
+
+import oracle_ffi as O
+from recon import scenes
+
+INTR = (525.0, 525.0, 319.5, 239.5)
+
+
+def _centres(vol):
+    g = list(vol.grid)
+    vs = [np.float64(np.float32(vol.voxel_size[i])) for i in range(3)]
+    off = [np.float64(np.float32(o)) for o in vol.offset]
+    z, y, x = np.meshgrid(*[(np.arange(g[i]) + 0.5) * vs[i] + off[i] for i in (2, 1, 0)],
+                          indexing="ij")
+    return np.stack([x, y, z], -1)
+
+
+def _unravel(vol, ids):
+    gx, gy = vol.grid[0], vol.grid[1]
+    return np.stack([ids % gx, (ids // gx) % gy, ids // (gx * gy)], -1)
+
+
+# neighbour order of interpolate_trilinearly (tsdf_volume.cu:85-92)
+REF_NEIGHBOURS = np.array([(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1),
+                           (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)])
+
+
+def test_trilinear_support_reproduces_affine_fields_and_order():
+    vol = O.volume((12, 10, 9), [0.05, 0.04, 0.06], (-0.3, -0.2, -0.27), 0.1, 128.0, INTR)
+    rng = np.random.default_rng(11)
+    c = _centres(vol)
+    lo_c, hi_c = c[0, 0, 0], c[-1, -1, -1]
+    for _ in range(200):
+        a, b = rng.normal(size=3), rng.normal()
+        field = (c @ a + b).ravel()
+        # interior points, and points in the half voxel below the first
+        # centre, where the lower corner clamps to 0 and the weights
+        # extrapolate (as the reference's cubic_coord does)
+        p = rng.uniform(lo_c - 0.45 * np.array([0.05, 0.04, 0.06]), hi_c - 1e-9)
+        ids, coef = O.trilinear_support(vol, p)
+        assert abs(coef.sum() - 1.0) < 1e-12
+        assert abs(coef @ field[ids] - (p @ a + b)) < 1e-11
+        assert (_unravel(vol, ids) - _unravel(vol, ids[:1]) == REF_NEIGHBOURS).all()
+        assert (_unravel(vol, ids[:1]) >= 0).all()
+    # beyond the last centre the 2x2x2 support would leave the grid
+    assert O.trilinear_support(vol, hi_c + 0.01) is None
+
+
+def test_trilinear_matches_scipy_linear_interpolation():
+    vol = O.volume((16, 14, 11), [0.02] * 3, (-0.16, -0.14, -0.11), 0.06, 128.0, INTR)
+    rng = np.random.default_rng(12)
+    field = rng.normal(size=(11, 14, 16))
+    c = _centres(vol)
+    pts = rng.uniform(c[0, 0, 0], c[-1, -1, -1] - 1e-9, size=(500, 3))
+    vs = np.float64(np.float32(0.02))
+    off = np.array([np.float64(np.float32(o)) for o in vol.offset])
+    idx = ((pts - off) / vs - 0.5)[:, ::-1].T  # (z, y, x) index coordinates
+    want = map_coordinates(field, idx, order=1)
+    got = np.empty(len(pts))
+    for k, p in enumerate(pts):
+        ids, coef = O.trilinear_support(vol, p)
+        got[k] = coef @ field.ravel()[ids]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_lambda_is_the_gradient_of_the_loss():
+    w = scenes.tiny(grid=(20, 18, 16), voxel_size=0.03)
+    vol = O.volume_from_config(w.config())
+    rng = np.random.default_rng(13)
+    D = rng.normal(0, 0.05, size=O.shape(vol))
+    pts = scenes.surface_points(w, 3000, seed=13)
+    wts = (rng.uniform(size=len(pts)) < 0.7).astype(np.float64)
+    L, lam = O.surface_loss(vol, D, pts, wts)
+    # L itself from the supports
+    r = np.array([coef @ D.ravel()[ids] for ids, coef in (O.trilinear_support(vol, p) for p in pts)])
+    assert abs(L - np.mean(wts * r * r)) < 1e-15 + 1e-12 * L
+    active = np.flatnonzero(lam)
+    assert len(active) > 100
+    for v in rng.choice(active, 40, replace=False):
+        h = 1e-3
+        Dp, Dm = D.copy(), D.copy()
+        Dp.ravel()[v] += h
+        Dm.ravel()[v] -= h
+        fd = (O.surface_loss(vol, Dp, pts, wts, want_lambda=False) -
+              O.surface_loss(vol, Dm, pts, wts, want_lambda=False)) / (2 * h)
+        assert abs(fd - lam.ravel()[v]) <= 1e-9 * np.abs(lam).max()
+    # voxels outside every weighted support get no gradient
+    untouched = np.ones(D.size, bool)
+    for p, wk in zip(pts, wts):
+        if wk:
+            untouched[O.trilinear_support(vol, p)[0]] = False
+    assert (lam.ravel()[untouched] == 0).all()
+
+
+def _pose_from_rotation(rot, t):
+    q = rot.as_quat()  # x, y, z, w
+    return np.array([q[3], q[0], q[1], q[2], *t])
+
+
+def test_bilinear_forward_closed_form_on_a_linear_depth_map():
+    """depth[j][i] = A + B i + C j: bilinear sampling at (u, v) is exactly
+    A + B u + C v, so every updated voxel has D = min(A + B u + C v - z, trunc)
+    and W = 1; u, v, z from scipy's rotation."""
+    rows, cols = 120, 160
+    intr = (150.0, 150.0, 79.5, 59.5)
+    vol = O.volume((24, 22, 20), [0.02] * 3, (-0.24, -0.22, -0.2), 0.05, 128.0, intr)
+    rng = np.random.default_rng(14)
+    jj, ii = np.mgrid[0:rows, 0:cols]
+    c = _centres(vol)
+    for trial in range(4):
+        A, B, Cc = 0.85 + 0.05 * trial, 1e-3 * rng.normal(), 1e-3 * rng.normal()
+        depth = (A + B * ii + Cc * jj).astype(np.float32)
+        rot = Rotation.from_rotvec(rng.normal(0, 0.45, 3))
+        pose = _pose_from_rotation(rot, [*rng.normal(0, 0.03, 2), 0.85])
+        D, W = O.diff_forward(vol, depth[None], O.rt_from_poses(pose[None]))
+        p = c @ rot.as_matrix().T + pose[4:]
+        z = p[..., 2]
+        u = p[..., 0] * intr[0] / z + intr[2]
+        v = p[..., 1] * intr[1] / z + intr[3]
+        inside = (z > 0) & (np.floor(u) >= 0) & (np.floor(u) <= cols - 2) & \
+                 (np.floor(v) >= 0) & (np.floor(v) <= rows - 2)
+        # the rounded float32 map departs from the plane by <= 1 ulp
+        sdf = (A + B * u + Cc * v) - z
+        upd = inside & (sdf >= -vol.truncation)
+        amb = (np.abs(sdf + vol.truncation) < 1e-6) | \
+              (np.minimum(np.abs(u - np.round(u)), np.abs(v - np.round(v))) < 1e-9)
+        assert upd.sum() > 1000 and (~upd & ~amb).sum() > 100
+        assert ((W == 1) == upd)[~amb].all()
+        want = np.minimum(sdf, vol.truncation)
+        err = np.abs(D - want)[upd & ~amb]
+        assert err.max() < 2e-7, err.max()
+        assert (D[~upd & ~amb] == np.float32(vol.truncation)).all()
+
+
+def test_adjoint_single_voxel_cap_clamp_and_skip():
+    """One voxel, identity rotations, constant depth per frame, cap 2.
+    Frames 0,1,2,4 update it (frame 2 clamped at +trunc, frame 3 skipped as
+    sdf < -trunc), so D = (2/9) t0 + (2/9) t1 + (2/9) trunc + (1/3) t4 and
+    dD/drho_z(f) = -k_f with k = (2/9, 2/9, 0, 0, 1/3); dD/domega = p x g."""
+    trunc = 0.05
+    vol = O.volume((1, 1, 1), [0.02] * 3, (-0.01, -0.01, -0.01), trunc, 2.0, INTR)
+    F = 5
+    ts = np.array([[0.01, -0.02, 1.0], [-0.03, 0.01, 1.1], [0.02, 0.02, 0.9],
+                   [0.0, 0.01, 1.2], [0.015, -0.01, 1.05]])
+    sdfs = np.array([0.013, -0.021, 0.2, -0.3, 0.034])
+    c = 0.5 * np.float64(np.float32(0.02)) + np.float64(np.float32(-0.01))  # ~0, per axis
+    pc = ts + c  # camera points of the voxel centre
+    zc = pc[:, 2]
+    depth = np.stack([np.full((480, 640), z + s, np.float32) for z, s in zip(zc, sdfs)])
+    sdf32 = depth[:, 0, 0].astype(np.float64) - zc
+    Rt = np.zeros((F, 12))
+    Rt[:, [0, 4, 8]] = 1.0
+    Rt[:, 9:] = ts
+    D, W = O.diff_forward(vol, depth, Rt)
+    k = np.array([2 / 9, 2 / 9, 2 / 9, 0.0, 1 / 3])
+    tsdf = np.minimum(sdf32, np.float64(np.float32(trunc)))
+    assert W.ravel()[0] == 2.0
+    assert abs(D.ravel()[0] - k @ np.where(np.arange(F) == 3, 0.0, tsdf)) < 1e-15
+    lam = np.full((1, 1, 1), 0.7)
+    gp, gd = O.diff_adjoint(vol, depth, Rt, lam)
+    kg = 0.7 * np.array([2 / 9, 2 / 9, 0.0, 0.0, 1 / 3])  # clamped frame: zero
+    g = np.zeros((F, 3))
+    g[:, 2] = -kg
+    np.testing.assert_allclose(gp[:, :3], g, atol=1e-15)
+    np.testing.assert_allclose(gp[:, 3:], np.cross(pc, g), atol=1e-15)
+    # the depth taps carry the same weights, split bilinearly
+    np.testing.assert_allclose(gd.reshape(F, -1).sum(1), kg, atol=1e-15)
+    assert (np.count_nonzero(gd.reshape(F, -1), axis=1) <= 4).all()
+
+
+def _fd_setup(noise, F=4, grid=64, vs=0.016, npts=8000, h=1e-4, kink=1e-4):
+    w = scenes.plain_small(num_frames=F, grid=grid, voxel_size=vs)
+    depth = w.render(noise=noise)
+    vol = O.volume_from_config(w.config())
+    Rt = O.rt_from_poses(w.poses)
+    pts = scenes.surface_points(w, npts, seed=21)
+    wts = O.stable_points(vol, depth, Rt, pts, h=h, kink=kink)
+    D, _ = O.diff_forward(vol, depth, Rt)
+    L, lam = O.surface_loss(vol, D, pts, wts)
+    gp, gd = O.diff_adjoint(vol, depth, Rt, lam)
+    return w, depth, vol, Rt, pts, wts, gp, gd
+
+
+def _pose_fd(vol, depth, Rt, pts, wts, h):
+    F = Rt.shape[0]
+    fd = np.zeros((F, 6))
+    for f in range(F):
+        for i in range(6):
+            e = np.zeros(6)
+            e[i] = h
+            Rp, Rm = Rt.copy(), Rt.copy()
+            Rp[f] = O.se3_exp_left(Rt[f], e)
+            Rm[f] = O.se3_exp_left(Rt[f], -e)
+            Lp = O.surface_loss(vol, O.diff_forward(vol, depth, Rp)[0], pts, wts, want_lambda=False)
+            Lm = O.surface_loss(vol, O.diff_forward(vol, depth, Rm)[0], pts, wts, want_lambda=False)
+            fd[f, i] = (Lp - Lm) / (2 * h)
+    return fd
+
+
+@pytest.mark.parametrize("noise,h,kink,tol", [(False, 1e-4, 2e-5, 1e-4), (True, 1e-6, 1e-4, 1e-5)],
+                         ids=["smooth-h1e-4", "noisy-h1e-6"])
+def test_pose_gradient_matches_central_differences(noise, h, kink, tol):
+    _, depth, vol, Rt, pts, wts, gp, _ = _fd_setup(noise, h=h, kink=kink)
+    assert wts.mean() > 0.1
+    assert (np.abs(gp).max(axis=1) > 0).all()
+    fd = _pose_fd(vol, depth, Rt, pts, wts, h)
+    err = np.abs(fd - gp).max() / np.abs(gp).max()
+    assert err < tol, (err, gp, fd)
+
+
+def test_depth_gradient_matches_central_differences():
+    _, depth, vol, Rt, pts, wts, gp, gd = _fd_setup(True, h=1e-4)
+    rng = np.random.default_rng(22)
+    flat = np.abs(gd).ravel()
+    top = np.argsort(-flat)[:12]
+    valid = np.flatnonzero((depth.ravel() > 1e-3) & (flat > 0))
+    picks = np.r_[top, rng.choice(valid, 12, replace=False)]
+    for idx in picks:
+        f, j, i = np.unravel_index(idx, depth.shape)
+        d0 = depth[f, j, i]
+        dp, dm = depth.copy(), depth.copy()
+        dp[f, j, i] = np.float32(d0 + 1e-4)
+        dm[f, j, i] = np.float32(d0 - 1e-4)
+        step = np.float64(dp[f, j, i]) - np.float64(dm[f, j, i])
+        Lp = O.surface_loss(vol, O.diff_forward(vol, dp, Rt)[0], pts, wts, want_lambda=False)
+        Lm = O.surface_loss(vol, O.diff_forward(vol, dm, Rt)[0], pts, wts, want_lambda=False)
+        fd = (Lp - Lm) / step
+        assert abs(fd - gd[f, j, i]) <= 1e-6 * flat.max(), (f, j, i, fd, gd[f, j, i])
+
+
+def test_adjoint_is_linear_in_lambda():
+    """Gradients of a sum of lambdas equal the sum of the gradients."""
+    w = scenes.tiny(num_frames=4)
+    depth = w.render()
+    vol = O.volume_from_config(w.config())
+    Rt = O.rt_from_poses(w.poses)
+    rng = np.random.default_rng(23)
+    l1 = rng.normal(size=O.shape(vol)) * (rng.uniform(size=O.shape(vol)) < 0.2)
+    l2 = rng.normal(size=O.shape(vol)) * (rng.uniform(size=O.shape(vol)) < 0.2)
+    g1, d1 = O.diff_adjoint(vol, depth, Rt, l1)
+    g2, d2 = O.diff_adjoint(vol, depth, Rt, l2)
+    g3, d3 = O.diff_adjoint(vol, depth, Rt, 0.5 * l1 - 2.0 * l2)
+    np.testing.assert_allclose(g3, 0.5 * g1 - 2.0 * g2, rtol=1e-10, atol=1e-12 * np.abs(g3).max())
+    np.testing.assert_allclose(d3, 0.5 * d1 - 2.0 * d2, rtol=1e-10, atol=1e-12 * np.abs(d3).max())
