@@ -134,36 +134,7 @@ void make_frame_params(const VolumeParams& vp, const double pose[7], const float
     fp->bound0 = nullptr;
     fp->bound1 = nullptr;
     fp->pitch = pitch_floats;
-    const double fx = vp.fx, fy = vp.fy, cx = vp.cx, cy = vp.cy;
-    const double a[6][3] = {{0, 0, 1},
-                            {fx, 0, cx + 1.0},
-                            {-fx, 0, (double)vp.cols - cx},
-                            {0, fy, cy + 1.0},
-                            {0, -fy, (double)vp.rows - cy},
-                            {0, 0, -1}};
-    const double b[6] = {0, 0, 0, 0, 0, (double)vp.maxd + (double)vp.trunc};
-    const int g[3] = {vp.gx, vp.gy, vp.gz};
-    double cmax[3];
-    for (int i = 0; i < 3; ++i) {
-        const double lo = vp.off[i], hi = (double)vp.off[i] + (double)g[i] * vp.vs[i];
-        cmax[i] = std::fmax(std::fabs(lo), std::fabs(hi)) + std::fabs((double)vp.vs[i]);
-    }
-    const double tabs = std::fabs((double)fp->t[0]) + std::fabs((double)fp->t[1]) +
-                        std::fabs((double)fp->t[2]);
-    fp->zmargin = (float)(1e-6 * (tabs + cmax[0] + cmax[1] + cmax[2]) + 1e-6);
-    for (int k = 0; k < 6; ++k) {
-        double n[3];
-        for (int i = 0; i < 3; ++i)  // n = R^T a
-            n[i] = (double)fp->R[0 * 3 + i] * a[k][0] + (double)fp->R[1 * 3 + i] * a[k][1] +
-                   (double)fp->R[2 * 3 + i] * a[k][2];
-        const double kk = a[k][0] * fp->t[0] + a[k][1] * fp->t[1] + a[k][2] * fp->t[2] + b[k];
-        const double a1 = std::fabs(a[k][0]) + std::fabs(a[k][1]) + std::fabs(a[k][2]);
-        double scale = a1 * tabs + std::fabs(b[k]);
-        for (int i = 0; i < 3; ++i) scale += std::fabs(n[i]) * cmax[i];
-        const double margin = 1e-3 * scale + 1e-12;
-        for (int i = 0; i < 3; ++i) fp->plane[k][i] = (float)n[i];
-        fp->plane[k][3] = (float)(kk + margin);
-    }
+    frame_planes(vp, fp);
 }
 
 }  // namespace recon

@@ -91,6 +91,42 @@ constexpr int kMaxFusedFrames = 16;
 void make_frame_params(const VolumeParams& vp, const double pose[7], const float* depth,
                        int32_t pitch_floats, FrameParams* fp);
 
+// The culling half-spaces and z margin of a frame from its R and t (float64
+// arithmetic, the same on host and device).  Half-space k in camera space is
+// a_k . p + b_k > 0; in world space n = R^T a_k, k = a_k . t + b_k, widened
+// by 1e-3 of the largest magnitude the terms reach over the volume.
+__host__ __device__ inline void frame_planes(const VolumeParams& vp, FrameParams* fp) {
+    const double fx = vp.fx, fy = vp.fy, cx = vp.cx, cy = vp.cy;
+    const double a[6][3] = {{0, 0, 1},
+                            {fx, 0, cx + 1.0},
+                            {-fx, 0, (double)vp.cols - cx},
+                            {0, fy, cy + 1.0},
+                            {0, -fy, (double)vp.rows - cy},
+                            {0, 0, -1}};
+    const double b[6] = {0, 0, 0, 0, 0, (double)vp.maxd + (double)vp.trunc};
+    const int g[3] = {vp.gx, vp.gy, vp.gz};
+    double cmax[3];
+    for (int i = 0; i < 3; ++i) {
+        const double lo = vp.off[i], hi = (double)vp.off[i] + (double)g[i] * vp.vs[i];
+        cmax[i] = ::fmax(::fabs(lo), ::fabs(hi)) + ::fabs((double)vp.vs[i]);
+    }
+    const double tabs = ::fabs((double)fp->t[0]) + ::fabs((double)fp->t[1]) + ::fabs((double)fp->t[2]);
+    fp->zmargin = (float)(1e-6 * (tabs + cmax[0] + cmax[1] + cmax[2]) + 1e-6);
+    for (int k = 0; k < 6; ++k) {
+        double n[3];
+        for (int i = 0; i < 3; ++i)  // n = R^T a
+            n[i] = (double)fp->R[0 * 3 + i] * a[k][0] + (double)fp->R[1 * 3 + i] * a[k][1] +
+                   (double)fp->R[2 * 3 + i] * a[k][2];
+        const double kk = a[k][0] * fp->t[0] + a[k][1] * fp->t[1] + a[k][2] * fp->t[2] + b[k];
+        const double a1 = ::fabs(a[k][0]) + ::fabs(a[k][1]) + ::fabs(a[k][2]);
+        double scale = a1 * tabs + ::fabs(b[k]);
+        for (int i = 0; i < 3; ++i) scale += ::fabs(n[i]) * cmax[i];
+        const double margin = 1e-3 * scale + 1e-12;
+        for (int i = 0; i < 3; ++i) fp->plane[k][i] = (float)n[i];
+        fp->plane[k][3] = (float)(kk + margin);
+    }
+}
+
 }  // namespace recon
 
 // ---- the handle -------------------------------------------------------------
@@ -112,6 +148,9 @@ struct recon_tsdf {
     // marching-cubes scratch, grown on demand
     void* mc_scratch = nullptr;
     size_t mc_scratch_bytes = 0;
+    // differentiable path scratch (frame table, depth bounds, partial sums)
+    void* diff_scratch = nullptr;
+    size_t diff_scratch_bytes = 0;
 };
 
 namespace recon {

@@ -294,6 +294,7 @@ extern "C" void recon_tsdf_destroy(recon_tsdf* h) {
     if (h->mc_scratch) (void)hipFree(h->mc_scratch);
     if (h->bounds) (void)hipFree(h->bounds);
     if (h->list) (void)hipFree(h->list);
+    if (h->diff_scratch) (void)hipFree(h->diff_scratch);
     delete h;
 }
 

@@ -132,6 +132,73 @@ class TSDFVolume:
             self._h, C.byref(dv), depths.stride(0) * 4, int(depths.shape[0]),
             ffi.pose_array(poses.ravel()), _stream(stream)))
 
+    # ---- differentiable integrate -------------------------------------------
+    @staticmethod
+    def _frames_view(depths):
+        if depths.dim() != 3 or depths.stride(2) != 1:
+            raise ValueError("depths must be (F, H, W) with unit column stride")
+        return DepthView(depths.data_ptr(), depths.stride(1) * 4, depths.shape[1], depths.shape[2])
+
+    @staticmethod
+    def _check_rt(rt, nframes):
+        torch = _torch()
+        if rt.dtype != torch.float32 or not rt.is_cuda or not rt.is_contiguous() or \
+                tuple(rt.shape) != (nframes, 12):
+            raise ValueError("poses_rt must be a contiguous (F, 12) float32 device tensor")
+
+    def diff_integrate(self, depths, poses_rt, stream=None):
+        """Bilinear-sampling integrate of (F, H, W) depths seen from device
+        world->camera matrices poses_rt (F, 12): row-major R, then t."""
+        self._check_rt(poses_rt, int(depths.shape[0]))
+        dv = self._frames_view(depths)
+        check(self._lib.recon_tsdf_diff_integrate(self._h, C.byref(dv), depths.stride(0) * 4,
+                                                  int(depths.shape[0]), poses_rt.data_ptr(),
+                                                  _stream(stream)))
+
+    def surface_loss(self, points, weights=None, want_lambda=True, stream=None):
+        """(loss (1,) float64, residuals (K,), lambda ids (8K,) int64, lambda
+        (8K,) float32) for (K, 3) float32 device points."""
+        torch = _torch()
+        if points.dtype != torch.float32 or not points.is_contiguous() or points.dim() != 2:
+            raise ValueError("points must be a contiguous (K, 3) float32 device tensor")
+        K = int(points.shape[0])
+        dev = points.device
+        loss = torch.empty(1, dtype=torch.float64, device=dev)
+        res = torch.empty(K, dtype=torch.float32, device=dev)
+        ids = torch.empty(8 * K, dtype=torch.int64, device=dev) if want_lambda else None
+        lam = torch.empty(8 * K, dtype=torch.float32, device=dev) if want_lambda else None
+        wptr = None
+        if weights is not None:
+            if weights.dtype != torch.float32 or not weights.is_contiguous() or weights.numel() != K:
+                raise ValueError("weights must be a contiguous (K,) float32 device tensor")
+            wptr = weights.data_ptr()
+        check(self._lib.recon_tsdf_surface_loss(
+            self._h, points.data_ptr(), wptr, K, res.data_ptr(), loss.data_ptr(),
+            ids.data_ptr() if ids is not None else None, lam.data_ptr() if lam is not None else None,
+            _stream(stream)))
+        return loss, res, ids, lam
+
+    def diff_adjoint(self, depths, poses_rt, lambda_ids, lam, want_depth=True, stream=None):
+        """(grad_pose (F, 6) float64, grad_depth like depths or None): the
+        adjoint of diff_integrate for dL/dD given as (voxel index, value)
+        pairs; xi = (rho, omega), left perturbation."""
+        torch = _torch()
+        F = int(depths.shape[0])
+        self._check_rt(poses_rt, F)
+        if lambda_ids.dtype != torch.int64 or lam.dtype != torch.float32 or \
+                lambda_ids.numel() != lam.numel() or not lambda_ids.is_contiguous() or \
+                not lam.is_contiguous():
+            raise ValueError("lambda_ids (int64) and lam (float32) must be contiguous and equal-sized")
+        dv = self._frames_view(depths)
+        gp = torch.empty((F, 6), dtype=torch.float64, device=depths.device)
+        gd = torch.empty_strided(depths.size(), depths.stride(), dtype=torch.float32,
+                                 device=depths.device) if want_depth else None
+        check(self._lib.recon_tsdf_diff_adjoint(
+            self._h, C.byref(dv), depths.stride(0) * 4, F, poses_rt.data_ptr(),
+            lambda_ids.data_ptr(), lam.data_ptr(), int(lam.numel()), gp.data_ptr(),
+            gd.data_ptr() if gd is not None else None, _stream(stream)))
+        return gp, gd
+
     def set_update_counter(self, counter_tensor):
         ptr = counter_tensor.data_ptr() if counter_tensor is not None else None
         check(self._lib.recon_tsdf_set_update_counter(self._h, ptr))
@@ -207,6 +274,15 @@ def mc_class_histogram(field, grid, stream=None):
     check(ffi.load().recon_mc_class_histogram(field.data_ptr(), ffi.i32x3(grid), hist.data_ptr(),
                                               _stream(stream)))
     return hist
+
+
+def pose_to_matrix(poses) -> np.ndarray:
+    """(F, 7) {qw,qx,qy,qz,tx,ty,tz} poses -> (F, 12) float32 [R row-major | t],
+    narrowed as the plain integrate narrows them."""
+    poses = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, 7))
+    out = np.empty((poses.shape[0], 12), np.float32)
+    check(ffi.load().recon_pose_to_matrix(poses.ctypes.data, poses.shape[0], out.ctypes.data))
+    return out
 
 
 def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, stream=None):

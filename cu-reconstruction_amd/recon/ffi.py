@@ -110,6 +110,11 @@ SIGNATURES = {
     "recon_batch_autotune": (C.c_int, [C.POINTER(BatchVolume), _i32, C.c_char_p, _i32, _vp]),
     "recon_batch_download": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
     "recon_batch_download_mesh": (C.c_int, [_vp, _i32, _vp, _i64, _pi64, _vp]),
+    "recon_pose_to_matrix": (C.c_int, [_vp, _i32, _vp]),
+    "recon_tsdf_diff_integrate": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp]),
+    "recon_tsdf_surface_loss": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "recon_tsdf_diff_adjoint": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp,
+                                          _vp, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None

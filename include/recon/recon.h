@@ -185,6 +185,54 @@ RECON_API int recon_depth_to_points(const recon_depth_view* depth, double fx, do
                           float* xyz, uint8_t* valid, uint8_t* colors_out,
                           recon_stream_t stream);
 
+/* ---- differentiable integrate -------------------------------------------- */
+/* No counterpart in the reference.  An extension of cu_tsdf_integrate
+ * (tsdf_volume.cu:144-190) whose depth lookup is bilinear, so that the
+ * volume is differentiable in the camera poses and the depth maps:
+ * depth[j][i] sits at pixel coordinates (i, j); a voxel samples a frame when
+ * its camera z > 0, its continuous pixel coordinates s = ((x*f)/z) + c lie in
+ * [0, size-1) on both axes and the four taps are valid (0 < d <= max_depth).
+ * The update is otherwise the plain one (skip below -trunc, clamp at +trunc,
+ * weight cap).  Poses are DEVICE float32 world->camera matrices, 12 per frame
+ * (row-major R, then t), so a caller can refine them on the device.  Calls
+ * grow per-handle scratch on first use (hipMalloc); a graph capture should
+ * follow one uncaptured call of the same sizes. */
+
+/* host.  poses (num_poses*7 doubles {qw,qx,qy,qz,tx,ty,tz}) -> rt (12 floats
+ * each) narrowed as recon_tsdf_integrate narrows them (tsdf_volume.cu:271-272,
+ * se3.cuh:42-57). */
+RECON_API int recon_pose_to_matrix(const double* poses, int32_t num_poses, float* rt);
+/* async.  Integrates num_frames frames (depth frame f at depth0 + f *
+ * frame_stride_bytes, at least 2x2) in order with bilinear sampling. */
+RECON_API int recon_tsdf_diff_integrate(recon_tsdf* h, const recon_depth_view* depth0,
+                                        size_t frame_stride_bytes, int32_t num_frames,
+                                        const float* poses_rt, recon_stream_t stream);
+/* async.  L = (1/K) sum_k w_k r_k^2, r_k the trilinear TSDF at world point k
+ * (interpolate_trilinearly, tsdf_volume.cu:46-104, with neighbour (1,1,1)
+ * weighted x*y*z and (0,1,1) (1-x)*y*z, which the reference swaps; a point
+ * whose 2x2x2 support leaves the grid has r = 0 and no gradient).  Device
+ * arrays: points 3K floats, weights K floats (NULL: all 1), residuals K
+ * floats (may be NULL), *loss one double.  lambda_ids / lambda (8K each, both
+ * or neither) receive dL/dD as (voxel index, value) pairs, 8 per point in
+ * the sampler's neighbour order, index -1 for a point without support. */
+RECON_API int recon_tsdf_surface_loss(recon_tsdf* h, const float* points, const float* weights,
+                                      int64_t num_points, float* residuals, double* loss,
+                                      int64_t* lambda_ids, float* lambda, recon_stream_t stream);
+/* async.  Given dL/dD of the volume after recon_tsdf_diff_integrate of the
+ * same frames and poses (the pairs of recon_tsdf_surface_loss, or any list;
+ * repeated indices add; indices outside the volume are ignored):
+ * grad_pose (device, num_frames*6 doubles) <- dL/dxi_f, xi = (rho, omega)
+ * the left perturbation T_f <- Exp(xi) T_f (a camera point moves by
+ * rho + omega x p); grad_depth (device, may be NULL; the depth frames'
+ * layout) <- dL/ddepth.  The +trunc clamp and the skip below -trunc are
+ * taken as constant (their subgradient 0).  Pose sums are reduced in a fixed
+ * order (run-to-run identical); depth sums use float atomics. */
+RECON_API int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* depth0,
+                                      size_t frame_stride_bytes, int32_t num_frames,
+                                      const float* poses_rt, const int64_t* lambda_ids,
+                                      const float* lambda, int64_t num_lambda, double* grad_pose,
+                                      float* grad_depth, recon_stream_t stream);
+
 /* ---- batched volumes ----------------------------------------------------- */
 /* No counterpart in the reference, which builds one TSDFVolume per scene and
  * blocks on every frame (tsdf_surface_reconstruction.cpp:125-138).  A batch

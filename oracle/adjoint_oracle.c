@@ -94,7 +94,57 @@ void oracle_diff_forward_f64(const oracle_volume* vol, double* D, double* W, con
     }
 }
 
-/* Trilinear support of a world point (tsdf_volume.cu:46-104): lower corner
+/* The same forward in float32, one rounding per operation in the order
+ * written (the GPU kernel's order): camera point ((r0 x + r1 y) + r2 z) + t,
+ * s = ((p * f) / z) + c, footprint 0 <= s < size - 1, taps at (int)s,
+ * d = (((ia ib) d00 + (a ib) d10) + (ia b) d01) + (a b) d11 with ia = 1 - a. */
+void oracle_diff_forward_f32(const oracle_volume* vol, float* D, float* W, const float* depth,
+                             int32_t rows, int32_t cols, const float* rt, int32_t nframes,
+                             int nthreads) {
+    const int64_t gx = vol->grid[0], gy = vol->grid[1], gz = vol->grid[2];
+    const float trunc = vol->truncation, cap = vol->max_weight, maxd = vol->max_depth;
+    const float fcols = (float)cols, frows = (float)rows;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t z = 0; z < gz; ++z)
+        for (int64_t y = 0; y < gy; ++y) {
+            float c[3];
+            for (int64_t x = 0; x < gx; ++x) {
+                const int64_t i = x + gx * (y + gy * z);
+                oracle_voxel_centre_f32(vol, (int32_t)x, (int32_t)y, (int32_t)z, c);
+                for (int f = 0; f < nframes; ++f) {
+                    const float* R = rt + 12 * f;
+                    const float px = ((R[0] * c[0] + R[1] * c[1]) + R[2] * c[2]) + R[9];
+                    const float py = ((R[3] * c[0] + R[4] * c[1]) + R[5] * c[2]) + R[10];
+                    const float pz = ((R[6] * c[0] + R[7] * c[1]) + R[8] * c[2]) + R[11];
+                    if (!(pz > 0.0f)) continue;
+                    const float su = ((px * vol->fx) / pz) + vol->cx;
+                    const float sv = ((py * vol->fy) / pz) + vol->cy;
+                    if (!(su >= 0.0f && su < fcols - 1.0f && sv >= 0.0f && sv < frows - 1.0f)) continue;
+                    const int i0 = (int)su, j0 = (int)sv;
+                    const float a = su - (float)i0, b = sv - (float)j0;
+                    const float* q = depth + (int64_t)f * rows * cols + (int64_t)j0 * cols + i0;
+                    const float d00 = q[0], d10 = q[1], d01 = q[cols], d11 = q[cols + 1];
+                    if (!(d00 > 0.0f && d00 <= maxd && d10 > 0.0f && d10 <= maxd && d01 > 0.0f &&
+                          d01 <= maxd && d11 > 0.0f && d11 <= maxd))
+                        continue;
+                    const float ia = 1.0f - a, ib = 1.0f - b;
+                    const float d = (((ia * ib) * d00 + (a * ib) * d10) + (ia * b) * d01) + (a * b) * d11;
+                    const float sdf = d - pz;
+                    if (!(sdf >= -trunc)) continue;
+                    const float tsdf = sdf > 0.0f ? fminf(sdf, trunc) : sdf;
+                    const float w = W[i], nw = w + 1.0f;
+                    D[i] = ((D[i] * w) + tsdf) / nw;
+                    W[i] = nw < cap ? nw : cap;
+                }
+            }
+        }
+}
+
+/* Trilinear support of a world point (tsdf_volume.cu:46-104): grid-local q
+ * (points more than 1e-4 below the grid move onto its face), lower corner
  * floor(q/vs - 0.5) clamped at 0, fractions (q - centre(lower)) / vs,
  * neighbour order 0:(0,0,0) 1:(1,0,0) 2:(1,0,1) 3:(0,0,1) 4:(0,1,0) 5:(1,1,0)
  * 6:(1,1,1) 7:(0,1,1).  The reference gives neighbour 6 the weight
@@ -106,7 +156,13 @@ int oracle_trilinear_support(const oracle_volume* vol, const double pw[3], int64
     int lo[3];
     double t[3];
     for (int k = 0; k < 3; ++k) {
-        const double q = pw[k] - vol->offset[k];
+        /* grid-local coordinate with the reference's boundary clamps
+         * (tsdf_volume.cu:53-64, BOUNDARY_EPS = 1e-4) */
+        double q = pw[k] - vol->offset[k];
+        const double bound = (double)vol->grid[k] * vol->voxel_size[k], eps = (double)1e-4f;
+        if (q < -eps) q = 0.0;
+        if (q > bound && q - bound < eps) q = bound - eps;
+        if (!(q <= bound)) return 0; /* beyond the grid, or NaN */
         int l = (int)floor(q / vol->voxel_size[k] - 0.5);
         if (l < 0) l = 0;
         if (l + 1 > vol->grid[k] - 1) return 0;

@@ -120,6 +120,11 @@ int64_t oracle_point_cloud_from_depth_f64(const float* depth, int32_t rows, int3
 void oracle_diff_forward_f64(const oracle_volume* vol, double* D, double* W, const float* depth,
                              int32_t rows, int32_t cols, const double* Rt, int32_t nframes,
                              int nthreads);
+/* The same forward in float32 in the GPU kernel's operation order; rt: 12
+ * floats per frame [R row-major | t].  D, W start as oracle_tsdf_init. */
+void oracle_diff_forward_f32(const oracle_volume* vol, float* D, float* W, const float* depth,
+                             int32_t rows, int32_t cols, const float* rt, int32_t nframes,
+                             int nthreads);
 /* Trilinear support of a world point (tsdf_volume.cu:46-104). */
 int oracle_trilinear_support(const oracle_volume* vol, const double pw[3], int64_t ids[8],
                              double coef[8]);

@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         sig.update({
             "oracle_diff_forward_f64": (None, [vp, vp, vp, vp, i32, i32, vp, i32, C.c_int]),
             "oracle_trilinear_support": (C.c_int, [vp, vp, vp, vp]),
+            "oracle_diff_forward_f32": (None, [vp, vp, vp, vp, i32, i32, vp, i32, C.c_int]),
             "oracle_surface_loss_f64": (C.c_double, [vp, vp, vp, vp, i64, vp]),
             "oracle_diff_adjoint_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp]),
             "oracle_stable_points": (None, [vp, vp, i32, i32, vp, i32, vp, i64, C.c_double,
@@ -213,6 +214,16 @@ def diff_forward(vol, depth, Rt, nthreads=0):
     Rt = np.ascontiguousarray(Rt, np.float64)
     lib().oracle_diff_forward_f64(C.byref(vol), _p(D), _p(W), _p(depth), rows, cols, _p(Rt), F,
                                   nthreads)
+    return D, W
+
+
+def diff_forward_f32(vol, depth, rt32, nthreads=0):
+    """Float32 bilinear forward from init (D = trunc, W = 0); rt32 (F, 12)."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    D, W = init(vol)
+    lib().oracle_diff_forward_f32(C.byref(vol), _p(D), _p(W), _p(depth), rows, cols,
+                                  _p(np.ascontiguousarray(rt32, np.float32)), F, nthreads)
     return D, W
 
 

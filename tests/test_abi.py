@@ -149,3 +149,37 @@ def test_bad_sizes_are_rejected_before_any_launch():
     st = lib.recon_mc_extract(C.c_void_p(1024), ffi.i32x3((4, 4, 4)), ffi.f32x3((1, 1, 1)),
                               ffi.f32x3((0, 0, 0)), C.c_void_p(4096), None, -1, C.byref(n), None)
     assert st == ffi.ERR_SIZE
+
+
+def test_differentiable_entry_points_reject_null_and_bad_sizes():
+    lib = ffi.load()
+    dv = ffi.DepthView(1024, 640 * 4, 480, 640)
+    p = C.c_void_p(4096)
+    d = C.c_double()
+    assert lib.recon_tsdf_diff_integrate(None, C.byref(dv), 0, 1, p, None) == ffi.ERR_NULL
+    assert lib.recon_tsdf_surface_loss(None, p, None, 1, None, C.byref(d), None, None,
+                                       None) == ffi.ERR_NULL
+    assert lib.recon_tsdf_diff_adjoint(None, C.byref(dv), 0, 1, p, p, p, 1, p, None,
+                                       None) == ffi.ERR_NULL
+    assert lib.recon_pose_to_matrix(None, 1, p) == ffi.ERR_NULL
+    assert lib.recon_pose_to_matrix(None, 0, None) == ffi.OK
+
+
+def test_pose_to_matrix_matches_oracle_narrowing():
+    """recon_pose_to_matrix narrows like the plain integrate (the oracle's
+    oracle_pose_matrix_f32), bit for bit."""
+    import numpy as np
+    import oracle_ffi as O
+    import recon
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(5)
+    q = Rotation.random(50, random_state=6).as_quat()  # x, y, z, w
+    poses = np.c_[q[:, 3], q[:, :3], rng.normal(size=(50, 3))]
+    rt = recon.pose_to_matrix(poses)
+    for k in range(50):
+        R, t = O.pose_matrix_f32(poses[k])
+        assert (rt[k, :9] == R.ravel()).all() and (rt[k, 9:] == t).all()
+    bad = poses[:2].copy()
+    bad[1, 2] = np.nan
+    with pytest.raises(recon.ReconError):
+        recon.pose_to_matrix(bad)

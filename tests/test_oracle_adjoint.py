@@ -65,6 +65,14 @@ def test_trilinear_support_reproduces_affine_fields_and_order():
         assert (_unravel(vol, ids[:1]) >= 0).all()
     # beyond the last centre the 2x2x2 support would leave the grid
     assert O.trilinear_support(vol, hi_c + 0.01) is None
+    # more than 1e-4 below the grid a coordinate moves onto the grid's face
+    # (tsdf_volume.cu:53-58); within 1e-4 it extrapolates
+    off = np.array([np.float64(np.float32(o)) for o in vol.offset])
+    p = np.array([off[0] - 0.03, 0.01, off[2] - 0.00005])
+    ids, coef = O.trilinear_support(vol, p)
+    ids0, coef0 = O.trilinear_support(vol, np.array([off[0], 0.01, off[2] - 0.00005]))
+    assert (ids == ids0).all() and np.allclose(coef, coef0, rtol=0, atol=1e-15)
+    assert coef[[0, 1, 4, 5]].sum() > 1.0  # z below the first centre: extrapolated
 
 
 def test_trilinear_matches_scipy_linear_interpolation():
