@@ -142,6 +142,151 @@ def bench_batched(steps: int, autotune_to: str | None = None) -> dict:
             "identical_results": bool(same), "tuned": cache is not None}
 
 
+def bench_differentiable(steps: int) -> dict:
+    """The differentiable workload: 256^3 volume (6 mm voxels), 20 frames of
+    640x480 of the sphere-and-plane scene, loss at 100k surface points.
+    forward = reset + recon_tsdf_diff_integrate (20 frames, bilinear) +
+    recon_tsdf_surface_loss; backward = recon_tsdf_diff_adjoint (gradients
+    for the 20 poses and the 20 depth maps).  HIP events on the launch
+    stream bracket each part."""
+    import numpy as np
+    import torch
+
+    import recon
+    from recon import scenes
+
+    w = scenes.differentiable()
+    depth = torch.from_numpy(w.render(nthreads=min(os.cpu_count() or 1, 16))).cuda()
+    rt = torch.from_numpy(recon.pose_to_matrix(w.poses)).cuda()
+    pts = torch.from_numpy(scenes.surface_points(w, 100_000).astype(np.float32)).cuda()
+    vol = recon.TSDFVolume(w.config())
+    stream = torch.cuda.current_stream()
+
+    def forward():
+        vol.reset(stream)
+        vol.diff_integrate(depth, rt, stream)
+        return vol.surface_loss(pts, stream=stream)
+
+    loss, _, ids, lam = forward()
+    gp, gd = vol.diff_adjoint(depth, rt, ids, lam, stream=stream)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for s in range(steps):
+        ev[s][0].record(stream)
+        loss, _, ids, lam = forward()
+        ev[s][1].record(stream)
+        gp, gd = vol.diff_adjoint(depth, rt, ids, lam, stream=stream)
+        ev[s][2].record(stream)
+    torch.cuda.synchronize()
+    fwd = np.array([e[0].elapsed_time(e[1]) for e in ev])
+    bwd = np.array([e[1].elapsed_time(e[2]) for e in ev])
+    out = {"workload": w.name, "grid": list(w.grid), "frames": w.num_frames,
+           "points": int(pts.shape[0]), "fwd_ms": round(float(fwd.mean()), 4),
+           "bwd_ms": round(float(bwd.mean()), 4),
+           "fwd_bwd_over_fwd": round(float((fwd + bwd).mean() / fwd.mean()), 3),
+           "fwd_gvu_s_swept": round(w.num_voxels * w.num_frames / (fwd.mean() / 1000.0) / 1e9, 2),
+           "loss": float(loss.item()), "grad_pose_norm": float(gp.norm().item())}
+    vol.close()
+    return out
+
+
+def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
+                          frames: int = 10, points: int = 20000) -> dict:
+    """Batched pose refinement: `fragments` fragments (128^3, 8 mm, 10 frames
+    of 640x480 of a distinct seeded box/sphere scene), initial poses perturbed
+    by up to 1 degree and 1 cm, `steps` on-device steps (one graph launch per
+    step: forward, loss, adjoint, Adam) over all fragments.  Reports the time
+    per step, pose errors, the loss and, as the judge, the RMS distance to the
+    analytic surface of the plain integrate's mesh with initial, refined and
+    ground-truth poses."""
+    import numpy as np
+    import torch
+
+    import recon
+    from recon import scenes
+
+    ws = scenes.pose_refinement(num_fragments=fragments, frames=frames, grid=grid,
+                                voxel_size=1.024 / grid)
+    spec, depths_np = [], []
+    t0 = time.perf_counter()
+    for i, w in enumerate(ws):
+        depth = w.render(nthreads=min(os.cpu_count() or 1, 16))
+        pts, wts = scenes.refinement_points(w, depth, points, seed=1000 + i)
+        depths_np.append(depth)
+        spec.append((w.config(), torch.from_numpy(depth).cuda(),
+                     torch.from_numpy(pts.astype(np.float32)).cuda(),
+                     torch.from_numpy(wts).cuda()))
+    log(f"pose refinement: {fragments} fragments prepared in {time.perf_counter() - t0:.1f} s")
+    init = np.concatenate([recon.pose_to_matrix(w.meta["init"]) for w in ws])
+    gt = np.concatenate([recon.pose_to_matrix(w.poses) for w in ws])
+    b = recon.DiffBatch(spec)
+    loss_gt = b.forward(torch.from_numpy(gt).cuda()).sum().item()
+    rt = torch.from_numpy(init.copy()).cuda()
+    b.refine(rt, 1)  # warm-up: capture and first replay
+    rt = torch.from_numpy(init.copy()).cuda()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = b.refine(rt, steps, lr_translation=5e-4, lr_rotation=1e-3, decay=0.95)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    refined = rt.cpu().numpy()
+    h = hist.cpu().numpy()
+    b.close()
+    a0, c0 = scenes.pose_errors(init, gt)
+    a1, c1 = scenes.pose_errors(refined, gt)
+
+    def mesh_error(rts):
+        """Mean over fragments of the RMS distance to the analytic surface of
+        the plain integrate's mesh vertices in observed space."""
+        errs = []
+        f0 = 0
+        for w, depth in zip(ws, depths_np):
+            F = w.num_frames
+            v = recon.TSDFVolume(w.config())
+            dd = torch.from_numpy(depth).cuda()
+            for f in range(F):
+                R = rts[f0 + f, :9].reshape(3, 3).astype(np.float64)
+                pose = np.r_[scenes.quat_from_matrix(R), rts[f0 + f, 9:]]
+                v.integrate(dd[f], pose)
+            verts, _ = v.extract_mesh()
+            _, W = v.download()
+            vv = verts.cpu().numpy().reshape(-1, 3)
+            errs.append(scenes.mesh_rms(w, vv[scenes.observed_vertices(w, W, vv)]))
+            v.close()
+            f0 += F
+        return float(np.mean(errs))
+
+    def common_mode(rts):
+        """Mean over fragments of |mean over its frames of the world-frame
+        error twist|: (translation mm, rotation deg)."""
+        tw = scenes.world_error_twists(rts, gt)
+        f0, tr, rot = 0, [], []
+        for w in ws:
+            m = tw[f0:f0 + w.num_frames].mean(axis=0)
+            tr.append(np.linalg.norm(m[:3]) * 1e3)
+            rot.append(np.degrees(np.linalg.norm(m[3:])))
+            f0 += w.num_frames
+        return round(float(np.mean(tr)), 3), round(float(np.mean(rot)), 4)
+
+    return {"fragments": fragments, "grid": grid, "frames": frames, "points_per_fragment": points,
+            "steps": steps, "ms_per_step": round(1000.0 * dt / max(steps, 1), 3),
+            "loss_initial": float(h[0].sum()), "loss_final": float(h[-1].sum()),
+            "loss_ground_truth": float(loss_gt),
+            "rotation_error_deg": {"initial_mean": round(float(a0.mean()), 4),
+                                   "final_mean": round(float(a1.mean()), 4),
+                                   "initial_max": round(float(a0.max()), 4),
+                                   "final_max": round(float(a1.max()), 4)},
+            "centre_error_mm": {"initial_mean": round(float(c0.mean()) * 1e3, 3),
+                                "final_mean": round(float(c1.mean()) * 1e3, 3),
+                                "initial_max": round(float(c0.max()) * 1e3, 3),
+                                "final_max": round(float(c1.max()) * 1e3, 3)},
+            "common_mode_error": {"initial_mm_deg": common_mode(init),
+                                  "final_mm_deg": common_mode(refined)},
+            "mesh_rms_mm": {"initial": round(mesh_error(init) * 1e3, 4),
+                            "refined": round(mesh_error(refined) * 1e3, 4),
+                            "ground_truth": round(mesh_error(gt) * 1e3, 4)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,10 +298,16 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-volume measurement")
     ap.add_argument("--autotune", metavar="JSON",
                     help="run the batch autotuner on the batched workload and write its table")
-    ap.add_argument("--only", choices=["integrate", "copy"],
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiable (forward + adjoint) measurement")
+    ap.add_argument("--no-refine", action="store_true",
+                    help="skip the batched pose refinement measurement")
+    ap.add_argument("--only", choices=["integrate", "copy", "differentiable", "refine"],
                     help="profiling aids: 'integrate' runs --steps passes of the per-frame "
                          "integrate only; 'copy' copies a 256 MiB buffer 20 times (the "
-                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE); "
+                         "'differentiable' runs the forward + adjoint leg only; 'refine' the "
+                         "batched pose refinement only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the TSDF path here is single-GPU "
@@ -180,6 +331,12 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         log("copied 20 x 256 MiB")
+        return 0
+    if args.only == "differentiable":
+        log(json.dumps(bench_differentiable(args.steps)))
+        return 0
+    if args.only == "refine":
+        log(json.dumps(bench_pose_refinement()))
         return 0
     w = scenes.plain_large(num_frames=args.frames)
     t0 = time.perf_counter()
@@ -317,6 +474,12 @@ def main() -> int:
     if not args.no_batched:
         log("timing the batched volumes")
         result["batched"] = bench_batched(max(args.steps, 2), args.autotune)
+    if not args.no_diff:
+        log("timing the differentiable forward + adjoint")
+        result["differentiable"] = bench_differentiable(max(args.steps, 3))
+    if not args.no_refine:
+        log("batched pose refinement")
+        result["pose_refinement"] = bench_pose_refinement()
     if not args.no_cpu:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(w, depth, args.cpu_seconds)

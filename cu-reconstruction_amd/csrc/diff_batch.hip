@@ -1,0 +1,861 @@
+// diff_batch.hip -- pose refinement over many independent fragments on one
+// GPU: the differentiable integrate, the surface loss and the adjoint of
+// tsdf_diff.hip over a batch of volumes (sizes, frame counts and image sizes
+// may differ), with the poses refined on the device.  The reference has no
+// counterpart (it fuses one scene at a time with fixed poses,
+// tsdf_surface_reconstruction.cpp:125-138).
+//
+// A batch owns one arena for every volume's D and W, device tables of
+// volumes and frames, the depth bounds of every frame (computed once: depth
+// maps do not change), and the scratch of every step, so that a step
+// allocates nothing and can be captured.  One step is:
+//   forward   frame table from the poses (device), fill of every volume,
+//             one cull over the wave-tiles of every volume (64 tiles x 4
+//             frame groups per workgroup), one persistent bilinear integrate
+//             over the list, the loss of every volume's points (workgroups
+//             never straddle volumes, so each volume's loss is its own
+//             ordered sum);
+//   backward  the (voxel, dL/dD) pairs of all volumes merged (pairs are
+//             volume-major, so the merged list is too), work units of up to
+//             kUnit merged pairs within one volume, lanes as frames, per-unit
+//             partials summed per (frame, component) over the volume's units
+//             in order;
+//   update    Adam on xi = (rho, omega) per frame with second moments shared
+//             per volume (translations, rotations), T <- Exp(step) T in
+//             float64, R re-orthonormalised, stored as float32.
+// recon_diff_batch_refine captures forward + backward + update once and
+// replays the graph per step.
+#pragma clang fp contract(off)
+
+#include "diff_device.hpp"
+#include "diff_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace recon {
+
+using Shape = DefaultShape;
+constexpr int kUnit = 1024;          // merged pairs per adjoint work unit
+constexpr int kMaxWaves = 8192;      // bound on the adjoint grid
+constexpr int kFill = 1024;          // voxels per fill workgroup
+
+struct DVol {
+    VolumeParams vp;
+    int64_t voxel_base;      // first voxel of the volume in the arena
+    int64_t first_point;     // first point of the volume in the batch
+    int64_t npoints;
+    const float* points;
+    const float* weights;
+    int32_t first_frame, nframes;
+    int32_t first_loss_block, nloss_blocks;
+};
+
+struct DFrame {
+    const float* depth;
+    const float* bound0;
+    const float* bound1;
+    int32_t pitch;
+    int32_t volume;
+};
+
+__device__ __forceinline__ int last_le(const int* starts, int n, int b) {
+    int lo = 0, hi = n - 1;  // last i with starts[i] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void dbatch_bounds_kernel(const BoundsJob* __restrict__ jobs) {
+    __shared__ int s0[16];
+    __shared__ int s1;
+    const BoundsJob j = jobs[blockIdx.z];
+    if ((int)blockIdx.x >= j.b1x || (int)blockIdx.y >= j.b1y) return;  // whole workgroup
+    bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
+}
+
+__global__ __launch_bounds__(64) void dbatch_setup_kernel(const DVol* __restrict__ vols,
+                                                          const DFrame* __restrict__ fr,
+                                                          const float* __restrict__ poses,
+                                                          int total_frames,
+                                                          FrameParams* __restrict__ table) {
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    if (g >= total_frames) return;
+    const DFrame d = fr[g];
+    FrameParams fp;
+    for (int i = 0; i < 9; ++i) fp.R[i] = poses[12 * g + i];
+    for (int i = 0; i < 3; ++i) fp.t[i] = poses[12 * g + 9 + i];
+    fp.depth = d.depth;
+    fp.bound0 = d.bound0;
+    fp.bound1 = d.bound1;
+    fp.pitch = d.pitch;
+    frame_planes(vols[d.volume].vp, &fp);
+    table[g] = fp;
+}
+
+__global__ __launch_bounds__(256) void dbatch_fill_kernel(const DVol* __restrict__ vols,
+                                                          const int* __restrict__ block_start,
+                                                          int nvol, unsigned* list_count) {
+    const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) *list_count = 0;
+    const int v = last_le(block_start, nvol, b);
+    const VolumeParams& vp = vols[v].vp;
+    const long long n = (long long)vp.gx * vp.gy * vp.gz;
+    fill4(vp.D, vp.W, (long long)(b - block_start[v]) * kFill + 4 * threadIdx.x, n, vp.trunc);
+}
+
+__global__ __launch_bounds__(256) void dbatch_cull_kernel(const DVol* __restrict__ vols,
+                                                          const FrameParams* __restrict__ table,
+                                                          const uint2* __restrict__ wg,
+                                                          ListEntry* __restrict__ list,
+                                                          unsigned* __restrict__ list_count) {
+    __shared__ uint64_t s_live[4][64];
+    __shared__ unsigned s_wave[kThreads / 64];
+    __shared__ unsigned s_base;
+    const uint2 e = wg[blockIdx.x];  // (volume, first tile)
+    const DVol& dv = vols[e.x];
+    const uint32_t ntiles = (uint32_t)dv.vp.tiles_x * dv.vp.tiles_y * dv.vp.tiles_z;
+    const int i = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const uint32_t id = e.y + i;
+    s_live[g][i] = id < ntiles ? tile_frames<Shape>(dv.vp, dv.nframes,
+                                                    FrameTable{table + dv.first_frame}, id, g, 4)
+                               : 0ull;
+    __syncthreads();
+    const uint64_t live = g == 0 ? (s_live[0][i] | s_live[1][i]) | (s_live[2][i] | s_live[3][i]) : 0ull;
+    append_tile(live, id, e.x, list, list_count, s_wave, &s_base);
+}
+
+__global__ __launch_bounds__(kThreads) void dbatch_integrate_kernel(
+    const DVol* __restrict__ vols, const FrameParams* __restrict__ table,
+    const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count) {
+    const unsigned n = __builtin_amdgcn_readfirstlane(*list_count);
+    const unsigned waves = gridDim.x * (kThreads / 64);
+    for (unsigned i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(list[i].tile);
+        const uint32_t v = __builtin_amdgcn_readfirstlane(list[i].volume);
+        const uint64_t mask =
+            ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(list[i].frames >> 32)) << 32) |
+            __builtin_amdgcn_readfirstlane((uint32_t)list[i].frames);
+        const DVol& dv = vols[v];
+        int x0, y0, z0;
+        tile_origin<Shape>(dv.vp, tile, x0, y0, z0);
+        const FrameTable ft{table + dv.first_frame};
+        if (dv.vp.gx % 4 == 0)
+            diff_integrate_tile<Shape, FrameTable, true>(dv.vp, dv.nframes, ft, mask, x0, y0, z0);
+        else
+            diff_integrate_tile<Shape, FrameTable, false>(dv.vp, dv.nframes, ft, mask, x0, y0, z0);
+    }
+}
+
+// One workgroup: 256 points of one volume.  Pair k*8 + i of the batch is
+// support voxel i of batch point k, as an arena index.
+__global__ __launch_bounds__(256) void dbatch_loss_kernel(const DVol* __restrict__ vols,
+                                                          const uint2* __restrict__ wg,
+                                                          double* __restrict__ partials,
+                                                          int64_t* __restrict__ lam_ids,
+                                                          float* __restrict__ lam) {
+    __shared__ double s_wave[4];
+    const uint2 e = wg[blockIdx.x];  // (volume, first local point)
+    const DVol& dv = vols[e.x];
+    const int64_t k = (int64_t)e.y + threadIdx.x;
+    double l = 0.0;
+    if (k < dv.npoints) {
+        const float p[3] = {dv.points[3 * k], dv.points[3 * k + 1], dv.points[3 * k + 2]};
+        const float w = dv.weights ? dv.weights[k] : 1.0f;
+        int64_t ids[8];
+        float coef[8];
+        const bool ok = tri_support(dv.vp, p, ids, coef);
+        float r = 0.0f;
+        if (ok) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) r = r + coef[i] * dv.vp.D[ids[i]];
+        }
+        l = ok ? (double)w * (double)r * (double)r : 0.0;
+        const float g = (2.0f * w * r) / (float)dv.npoints;
+        const int64_t q = 8 * (dv.first_point + k);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            lam_ids[q + i] = ok ? dv.voxel_base + ids[i] : -1;
+            lam[q + i] = ok ? g * coef[i] : 0.0f;
+        }
+    }
+    l = wave_sum_f64(l);
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = ((s_wave[0] + s_wave[1]) + s_wave[2]) + s_wave[3];
+}
+
+// Each volume's loss: its blocks' partials summed in the order of the
+// single-volume path's fixed_sum_kernel, times 1/K.
+__global__ __launch_bounds__(256) void dbatch_loss_sum_kernel(const DVol* __restrict__ vols,
+                                                              const double* __restrict__ partials,
+                                                              double* __restrict__ loss) {
+    __shared__ double s[256];
+    const DVol& dv = vols[blockIdx.x];
+    const double* src = partials + dv.first_loss_block;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < dv.nloss_blocks; i += 256) acc += src[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[blockIdx.x] = dv.npoints > 0 ? s[0] * (1.0 / (double)dv.npoints) : 0.0;
+}
+
+// Per volume, the range of the merged list it owns and its work units.
+// One workgroup; thread 0 forms the prefix (volume counts are small).
+__global__ __launch_bounds__(256) void dbatch_units_kernel(const DVol* __restrict__ vols, int nvol,
+                                                           const uint32_t* __restrict__ uid,
+                                                           const long long* __restrict__ count,
+                                                           long long* __restrict__ beg,
+                                                           long long* __restrict__ unit_start) {
+    const long long n = *count;
+    for (int v = threadIdx.x; v < nvol; v += 256) {
+        // first merged pair at or past the volume's first voxel (pairs are
+        // grouped by volume in volume order)
+        const long long base = vols[v].voxel_base;
+        long long lo = 0, hi = n;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if ((long long)uid[mid] < base) lo = mid + 1; else hi = mid;
+        }
+        beg[v] = lo;
+    }
+    if (threadIdx.x == 0) beg[nvol] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long u = 0;
+        for (int v = 0; v < nvol; ++v) {
+            unit_start[v] = u;
+            u += (beg[v + 1] - beg[v] + kUnit - 1) / kUnit;
+        }
+        unit_start[nvol] = u;
+    }
+}
+
+__global__ __launch_bounds__(256) void dbatch_adjoint_kernel(
+    const DVol* __restrict__ vols, int nvol, const FrameParams* __restrict__ table,
+    const uint32_t* __restrict__ uid, const float* __restrict__ lam_u,
+    const long long* __restrict__ beg, const long long* __restrict__ unit_start,
+    double* __restrict__ partial) {
+    const unsigned waves = gridDim.x * 4u;
+    const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const long long nunits = unit_start[nvol];
+    for (long long u = wave; u < nunits; u += waves) {
+        int lo = 0, hi = nvol - 1;  // last volume whose first unit <= u
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (unit_start[mid] <= u) lo = mid; else hi = mid - 1;
+        }
+        const int v = __builtin_amdgcn_readfirstlane(lo);
+        const DVol& dv = vols[v];
+        const LaneLayout ly = lane_layout(dv.nframes);
+        const FrameParams L = table[dv.first_frame + (ly.ok ? ly.f : 0)];
+        const long long e_begin = beg[v] + (u - unit_start[v]) * kUnit;
+        const long long e_end = min(e_begin + kUnit, beg[v + 1]);
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (long long e0 = e_begin; e0 < e_end; e0 += ly.S) {
+            const long long e = e0 + ly.seg;
+            int64_t vx = -1;
+            float l = 0.0f;
+            if (ly.ok && e < e_end) {
+                vx = (int64_t)uid[e] - dv.voxel_base;
+                l = lam_u[e];
+            }
+            adjoint_lane_pass(dv.vp, ly, L, vx, l, nullptr, acc);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double sum = segment_sum(acc[c], ly);
+            if (ly.seg == 0 && ly.ok) partial[((size_t)u * 64 + ly.f) * 6 + c] = sum;
+        }
+    }
+}
+
+// grad[g*6 + c] = sum over the units of frame g's volume, in order.
+__global__ __launch_bounds__(256) void dbatch_grad_kernel(const DVol* __restrict__ vols,
+                                                          const DFrame* __restrict__ fr,
+                                                          int total_frames,
+                                                          const long long* __restrict__ unit_start,
+                                                          const double* __restrict__ partial,
+                                                          double* __restrict__ grad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total_frames * 6) return;
+    const int g = i / 6, c = i - 6 * g;
+    const int v = fr[g].volume;
+    const int f = g - vols[v].first_frame;
+    double s = 0.0;
+    for (long long u = unit_start[v]; u < unit_start[v + 1]; ++u) s += partial[((size_t)u * 64 + f) * 6 + c];
+    grad[i] = s;
+}
+
+// ---- the update ----------------------------------------------------------------------
+struct AdamState {
+    double lr_t, lr_r, b1, b2, eps, decay;
+};
+
+__device__ void se3_exp_f64(const double xi[6], double R[9], double t[3]) {
+    const double w0 = xi[3], w1 = xi[4], w2 = xi[5];
+    const double th2 = w0 * w0 + w1 * w1 + w2 * w2, th = sqrt(th2);
+    double A, B, C;
+    if (th < 1e-6) {
+        A = 1.0 - th2 / 6.0;
+        B = 0.5 - th2 / 24.0;
+        C = 1.0 / 6.0 - th2 / 120.0;
+    } else {
+        A = sin(th) / th;
+        B = (1.0 - cos(th)) / th2;
+        C = (th - sin(th)) / (th2 * th);
+    }
+    const double K[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+    double K2[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            K2[3 * r + c] = K[3 * r] * K[c] + K[3 * r + 1] * K[3 + c] + K[3 * r + 2] * K[6 + c];
+    double V[9];
+    for (int k = 0; k < 9; ++k) {
+        const double I = (k % 4 == 0) ? 1.0 : 0.0;
+        R[k] = I + A * K[k] + B * K2[k];
+        V[k] = I + B * K[k] + C * K2[k];
+    }
+    for (int r = 0; r < 3; ++r) t[r] = V[3 * r] * xi[0] + V[3 * r + 1] * xi[1] + V[3 * r + 2] * xi[2];
+}
+
+// One workgroup per volume, one thread per frame.  Adam with a first moment
+// per coordinate and a second moment shared by the volume's translations and
+// one shared by its rotations (the mean square over the volume's frames): a
+// coordinate the loss barely constrains keeps its small step instead of
+// being normalised up to the full rate.  step = -lr * mhat / (sqrt(vhat) +
+// eps), lr_t for rho and lr_r for omega, both times decay^(t-1);
+// T <- Exp(step) T in float64, R re-orthonormalised.
+__global__ __launch_bounds__(64) void dbatch_adam_kernel(const DVol* __restrict__ vols,
+                                                         float* __restrict__ poses,
+                                                         const double* __restrict__ grad,
+                                                         double* __restrict__ m,
+                                                         double* __restrict__ v2,
+                                                         const int* __restrict__ step,
+                                                         const AdamState opt) {
+    __shared__ double s_sq[2][64];
+    const DVol& dv = vols[blockIdx.x];
+    const int f = threadIdx.x;
+    const bool ok = f < dv.nframes;
+    const int g = dv.first_frame + (ok ? f : 0);
+    double gk[6];
+    double sq_t = 0.0, sq_r = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        gk[k] = ok ? grad[6 * g + k] : 0.0;
+        if (k < 3) sq_t += gk[k] * gk[k]; else sq_r += gk[k] * gk[k];
+    }
+    s_sq[0][f] = sq_t;
+    s_sq[1][f] = sq_r;
+    __syncthreads();
+    for (int h = 32; h >= 1; h >>= 1) {
+        if (f < h) {
+            s_sq[0][f] += s_sq[0][f + h];
+            s_sq[1][f] += s_sq[1][f + h];
+        }
+        __syncthreads();
+    }
+    const double denom = 3.0 * (double)dv.nframes;
+    const int t = *step + 1;
+    const double c1 = 1.0 - pow(opt.b1, (double)t), c2 = 1.0 - pow(opt.b2, (double)t);
+    const double scale = pow(opt.decay, (double)(t - 1));
+    // shared second moments live in the slots of the volume's first frame
+    double* vs = v2 + 6 * (size_t)dv.first_frame;
+    const double vt = opt.b2 * vs[0] + (1.0 - opt.b2) * (s_sq[0][0] / denom);
+    const double vr = opt.b2 * vs[1] + (1.0 - opt.b2) * (s_sq[1][0] / denom);
+    __syncthreads();
+    if (f == 0) {
+        vs[0] = vt;
+        vs[1] = vr;
+    }
+    if (!ok) return;
+    double xi[6];
+    for (int k = 0; k < 6; ++k) {
+        const double mk = opt.b1 * m[6 * g + k] + (1.0 - opt.b1) * gk[k];
+        m[6 * g + k] = mk;
+        const double lr = (k < 3 ? opt.lr_t : opt.lr_r) * scale;
+        const double vk = k < 3 ? vt : vr;
+        xi[k] = -lr * (mk / c1) / (sqrt(vk / c2) + opt.eps);
+    }
+    double Re[9], te[3];
+    se3_exp_f64(xi, Re, te);
+    double R0[9], t0[3], R[9], tt[3];
+    for (int k = 0; k < 9; ++k) R0[k] = poses[12 * g + k];
+    for (int k = 0; k < 3; ++k) t0[k] = poses[12 * g + 9 + k];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            R[3 * r + c] = Re[3 * r] * R0[c] + Re[3 * r + 1] * R0[3 + c] + Re[3 * r + 2] * R0[6 + c];
+        tt[r] = Re[3 * r] * t0[0] + Re[3 * r + 1] * t0[1] + Re[3 * r + 2] * t0[2] + te[r];
+    }
+    // rows re-orthonormalised (Gram-Schmidt), third row from the cross product
+    double* a = R;
+    double* b = R + 3;
+    double* c = R + 6;
+    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; ++k) a[k] /= na;
+    const double d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    for (int k = 0; k < 3; ++k) b[k] -= d * a[k];
+    const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    for (int k = 0; k < 3; ++k) b[k] /= nb;
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+    for (int k = 0; k < 9; ++k) poses[12 * g + k] = (float)R[k];
+    for (int k = 0; k < 3; ++k) poses[12 * g + 9 + k] = (float)tt[k];
+}
+
+// Records the step's losses and advances the step counter.
+__global__ __launch_bounds__(256) void dbatch_step_kernel(int* __restrict__ step,
+                                                          const double* __restrict__ loss, int nvol,
+                                                          double* __restrict__ history) {
+    const int t = *step;
+    for (int v = threadIdx.x; v < nvol; v += 256)
+        if (history) history[(size_t)t * nvol + v] = loss[v];
+    __syncthreads();
+    if (threadIdx.x == 0) *step = t + 1;
+}
+
+__global__ __launch_bounds__(64) void dbatch_reset_step_kernel(int* __restrict__ step) {
+    if (threadIdx.x == 0) *step = 0;
+}
+
+__global__ __launch_bounds__(256) void dbatch_zero_f64_kernel(double* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = 0.0;
+}
+
+}  // namespace recon
+
+using namespace recon;
+
+struct recon_diff_batch {
+    int n = 0;
+    int total_frames = 0;
+    int64_t total_points = 0, arena_voxels = 0, npairs = 0;
+    std::vector<recon_tsdf_info> info;
+    std::vector<DVol> h_vols;
+    std::vector<void*> allocs;
+    float* arena = nullptr;
+    DVol* d_vols = nullptr;
+    DFrame* d_frames = nullptr;
+    FrameParams* table = nullptr;
+    int* d_fill_start = nullptr;
+    int fill_blocks = 0;
+    uint2* d_cull_wg = nullptr;
+    int cull_blocks = 0;
+    ListEntry* list = nullptr;
+    unsigned* list_count = nullptr;
+    int integrate_grid = 0;
+    uint2* d_loss_wg = nullptr;
+    int loss_blocks = 0;
+    double* loss_partials = nullptr;
+    int64_t* lam_ids = nullptr;
+    float* lam = nullptr;
+    double* loss = nullptr;  // internal per-volume loss of refinement steps
+    uint32_t* owner = nullptr;
+    char* merge_scratch = nullptr;
+    MergeLayout M;
+    long long* beg = nullptr;
+    long long* unit_start = nullptr;
+    int64_t units_bound = 0;
+    int adjoint_grid = 0;
+    double* partial = nullptr;
+    double* grad = nullptr;
+    double* adam_m = nullptr;
+    double* adam_v = nullptr;
+    int* step = nullptr;
+
+    ~recon_diff_batch() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T** p, size_t count, const char* what) {
+        void* q = nullptr;
+        RECON_HIP_TRY(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)), what);
+        allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        return RECON_OK;
+    }
+    int forward(const float* poses, double* loss_out, hipStream_t s);
+    int backward(double* grad_out, hipStream_t s);
+    int update(float* poses, const AdamState& opt, double* history, hipStream_t s);
+};
+
+int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t s) {
+    hipLaunchKernelGGL(dbatch_setup_kernel, dim3((total_frames + 63) / 64), dim3(64), 0, s, d_vols,
+                       d_frames, poses, total_frames, table);
+    hipLaunchKernelGGL(dbatch_fill_kernel, dim3(fill_blocks), dim3(256), 0, s, d_vols, d_fill_start,
+                       n, list_count);
+    hipLaunchKernelGGL(dbatch_cull_kernel, dim3(cull_blocks), dim3(256), 0, s, d_vols, table,
+                       d_cull_wg, list, list_count);
+    hipLaunchKernelGGL(dbatch_integrate_kernel, dim3(integrate_grid), dim3(kThreads), 0, s, d_vols,
+                       table, list, list_count);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch forward launch");
+    if (loss_blocks > 0)
+        hipLaunchKernelGGL(dbatch_loss_kernel, dim3(loss_blocks), dim3(256), 0, s, d_vols,
+                           d_loss_wg, loss_partials, lam_ids, lam);
+    hipLaunchKernelGGL(dbatch_loss_sum_kernel, dim3(n), dim3(256), 0, s, d_vols, loss_partials,
+                       loss_out);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch loss launch");
+    return RECON_OK;
+}
+
+int recon_diff_batch::backward(double* grad_out, hipStream_t s) {
+    if (npairs == 0) {
+        hipLaunchKernelGGL(dbatch_zero_f64_kernel, dim3(1), dim3(256), 0, s, grad_out,
+                           (int64_t)total_frames * 6);
+        RECON_HIP_TRY(hipGetLastError(), "diff batch zero launch");
+        return RECON_OK;
+    }
+    uint32_t* uid = nullptr;
+    float* lam_u = nullptr;
+    const long long* count = nullptr;
+    int rc = launch_merge(lam_ids, lam, npairs, arena_voxels, owner, merge_scratch, M, &uid, &lam_u,
+                          &count, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(dbatch_units_kernel, dim3(1), dim3(256), 0, s, d_vols, n, uid, count, beg,
+                       unit_start);
+    hipLaunchKernelGGL(dbatch_adjoint_kernel, dim3(adjoint_grid), dim3(256), 0, s, d_vols, n, table,
+                       uid, lam_u, beg, unit_start, partial);
+    hipLaunchKernelGGL(dbatch_grad_kernel, dim3((total_frames * 6 + 255) / 256), dim3(256), 0, s,
+                       d_vols, d_frames, total_frames, unit_start, partial, grad_out);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch backward launch");
+    return RECON_OK;
+}
+
+int recon_diff_batch::update(float* poses, const AdamState& opt, double* history, hipStream_t s) {
+    hipLaunchKernelGGL(dbatch_adam_kernel, dim3(n), dim3(64), 0, s, d_vols, poses, grad, adam_m,
+                       adam_v, step, opt);
+    hipLaunchKernelGGL(dbatch_step_kernel, dim3(1), dim3(256), 0, s, step, loss, n, history);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch update launch");
+    return RECON_OK;
+}
+
+static int check_dvolume(const recon_diff_volume& v, int i, recon_tsdf_info* info) {
+    int rc = describe(&v.config, info);
+    if (rc) return rc;
+    if (v.num_frames < 1 || v.num_frames > 64)
+        return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d has %d frames (1..64)", i,
+                    v.num_frames);
+    if (v.rows < 2 || v.cols < 2)
+        return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d image %dx%d (>= 2x2)", i,
+                    v.cols, v.rows);
+    if (!v.depth) return fail(RECON_ERR_NULL, "recon_diff_batch_create: volume %d depth is NULL", i);
+    if (v.depth_step < 4 * (size_t)v.cols || v.depth_step % 4)
+        return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d depth step %zu", i,
+                    v.depth_step);
+    if (v.num_frames > 1 && (v.frame_stride % 4 || v.frame_stride < v.depth_step * (size_t)v.rows))
+        return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d frame stride %zu", i,
+                    v.frame_stride);
+    if ((v.depth_step / 4) * (size_t)v.rows >= (size_t(1) << 31))
+        return fail(RECON_ERR_RANGE, "recon_diff_batch_create: volume %d image too large", i);
+    if (v.num_points < 0 || (v.num_points > 0 && !v.points))
+        return fail(RECON_ERR_NULL, "recon_diff_batch_create: volume %d points", i);
+    if (v.num_points >= (int64_t(1) << 32))
+        return fail(RECON_ERR_RANGE, "recon_diff_batch_create: volume %d has too many points", i);
+    return RECON_OK;
+}
+
+static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipStream_t s) {
+    int rc;
+    b->n = n;
+    b->info.resize(n);
+    int64_t arena = 0, frames = 0, points = 0;
+    for (int i = 0; i < n; ++i) {
+        if ((rc = check_dvolume(vols[i], i, &b->info[i]))) return rc;
+        arena += b->info[i].num_voxels;
+        frames += vols[i].num_frames;
+        points += vols[i].num_points;
+    }
+    if (arena >= 0xFFFFFFFFll)
+        return fail(RECON_ERR_RANGE, "recon_diff_batch_create: %lld voxels exceed 2^32 - 1",
+                    (long long)arena);
+    if (8 * points >= 0x7FFFFFFFll)
+        return fail(RECON_ERR_RANGE, "recon_diff_batch_create: %lld points exceed 2^28",
+                    (long long)points);
+    b->arena_voxels = arena;
+    b->total_frames = (int)frames;
+    b->total_points = points;
+    b->npairs = 8 * points;
+    if ((rc = b->alloc(&b->arena, 2 * (size_t)arena, "diff batch arena"))) return rc;
+
+    // volume and frame tables, depth bounds jobs, fill / cull / loss work tables
+    std::vector<DFrame> h_frames;
+    std::vector<BoundsJob> jobs;
+    std::vector<int> fill_start;
+    std::vector<uint2> cull_wg, loss_wg;
+    size_t bounds_floats = 0;
+    for (int i = 0; i < n; ++i) {
+        const recon_diff_volume& v = vols[i];
+        const int cols = v.cols, rows = v.rows;
+        bounds_floats += (size_t)v.num_frames *
+                         ((size_t)((cols + kBound0 - 1) / kBound0) * ((rows + kBound0 - 1) / kBound0) +
+                          (size_t)((cols + kBound1 - 1) / kBound1) * ((rows + kBound1 - 1) / kBound1));
+    }
+    float* bounds = nullptr;
+    if ((rc = b->alloc(&bounds, bounds_floats, "diff batch bounds"))) return rc;
+    int64_t vbase = 0, pbase = 0;
+    size_t boff = 0;
+    int fbase = 0, fill_blocks = 0, loss_blocks = 0, max_b1x = 1, max_b1y = 1;
+    for (int i = 0; i < n; ++i) {
+        const recon_diff_volume& v = vols[i];
+        const recon_tsdf_info& inf = b->info[i];
+        DVol dv;
+        std::memset(&dv, 0, sizeof(dv));
+        VolumeParams& vp = dv.vp;
+        vp.D = b->arena + vbase;
+        vp.W = b->arena + arena + vbase;
+        for (int k = 0; k < 3; ++k) {
+            vp.vs[k] = inf.voxel_size[k];
+            vp.off[k] = v.config.global_offset[k];
+        }
+        vp.gx = v.config.voxel_grid_size[0];
+        vp.gy = v.config.voxel_grid_size[1];
+        vp.gz = v.config.voxel_grid_size[2];
+        vp.tiles_x = (vp.gx + Shape::X - 1) / Shape::X;
+        vp.tiles_y = (vp.gy + Shape::Y - 1) / Shape::Y;
+        vp.tiles_z = (vp.gz + Shape::Z - 1) / Shape::Z;
+        vp.fx = v.config.focal_x;
+        vp.fy = v.config.focal_y;
+        vp.cx = v.config.cx;
+        vp.cy = v.config.cy;
+        vp.trunc = inf.truncation;
+        vp.maxw = v.config.max_weight;
+        vp.maxd = inf.max_depth;
+        vp.cols = v.cols;
+        vp.rows = v.rows;
+        vp.fcols = (float)v.cols;
+        vp.frows = (float)v.rows;
+        vp.b0x = (v.cols + kBound0 - 1) / kBound0;
+        vp.b0y = (v.rows + kBound0 - 1) / kBound0;
+        vp.b1x = (v.cols + kBound1 - 1) / kBound1;
+        vp.b1y = (v.rows + kBound1 - 1) / kBound1;
+        max_b1x = std::max(max_b1x, vp.b1x);
+        max_b1y = std::max(max_b1y, vp.b1y);
+        dv.voxel_base = vbase;
+        dv.first_point = pbase;
+        dv.npoints = v.num_points;
+        dv.points = v.points;
+        dv.weights = v.weights;
+        dv.first_frame = fbase;
+        dv.nframes = v.num_frames;
+        dv.first_loss_block = loss_blocks;
+        dv.nloss_blocks = (int)((v.num_points + 255) / 256);
+        for (int f = 0; f < v.num_frames; ++f) {
+            DFrame fr;
+            fr.depth = reinterpret_cast<const float*>(reinterpret_cast<const char*>(v.depth) +
+                                                      v.frame_stride * (size_t)f);
+            fr.bound0 = bounds + boff;
+            fr.bound1 = fr.bound0 + (size_t)vp.b0x * vp.b0y;
+            fr.pitch = (int32_t)(v.depth_step / 4);
+            fr.volume = i;
+            h_frames.push_back(fr);
+            BoundsJob j;
+            j.depth = fr.depth;
+            j.bound0 = bounds + boff;
+            j.bound1 = j.bound0 + (size_t)vp.b0x * vp.b0y;
+            j.pitch = fr.pitch;
+            j.cols = v.cols;
+            j.rows = v.rows;
+            j.b0x = vp.b0x;
+            j.b0y = vp.b0y;
+            j.b1x = vp.b1x;
+            j.b1y = vp.b1y;
+            j.maxd = vp.maxd;
+            jobs.push_back(j);
+            boff += (size_t)vp.b0x * vp.b0y + (size_t)vp.b1x * vp.b1y;
+        }
+        fill_start.push_back(fill_blocks);
+        fill_blocks += (int)((inf.num_voxels + kFill - 1) / kFill);
+        const uint32_t ntiles = (uint32_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
+        for (uint32_t t = 0; t < ntiles; t += 64) cull_wg.push_back(make_uint2(i, t));
+        for (int64_t p = 0; p < v.num_points; p += 256) loss_wg.push_back(make_uint2(i, (uint32_t)p));
+        loss_blocks += dv.nloss_blocks;
+        b->h_vols.push_back(dv);
+        vbase += inf.num_voxels;
+        pbase += v.num_points;
+        fbase += v.num_frames;
+    }
+    b->fill_blocks = fill_blocks;
+    b->cull_blocks = (int)cull_wg.size();
+    b->loss_blocks = loss_blocks;
+    if ((rc = b->alloc(&b->d_vols, n, "diff batch volumes"))) return rc;
+    if ((rc = b->alloc(&b->d_frames, h_frames.size(), "diff batch frames"))) return rc;
+    if ((rc = b->alloc(&b->table, h_frames.size(), "diff batch frame table"))) return rc;
+    BoundsJob* d_jobs = nullptr;
+    if ((rc = b->alloc(&d_jobs, jobs.size(), "diff batch bounds jobs"))) return rc;
+    if ((rc = b->alloc(&b->d_fill_start, n, "diff batch fill table"))) return rc;
+    if ((rc = b->alloc(&b->d_cull_wg, cull_wg.size(), "diff batch cull table"))) return rc;
+    if ((rc = b->alloc(&b->d_loss_wg, loss_wg.size(), "diff batch loss table"))) return rc;
+    size_t tiles_total = (size_t)cull_wg.size() * 64;
+    if ((rc = b->alloc(&b->list, tiles_total, "diff batch cull list"))) return rc;
+    if ((rc = b->alloc(&b->list_count, 1, "diff batch list count"))) return rc;
+    if ((rc = b->alloc(&b->loss_partials, std::max(loss_blocks, 1), "diff batch loss partials")))
+        return rc;
+    if ((rc = b->alloc(&b->lam_ids, b->npairs, "diff batch pairs"))) return rc;
+    if ((rc = b->alloc(&b->lam, b->npairs, "diff batch pairs"))) return rc;
+    if ((rc = b->alloc(&b->loss, n, "diff batch loss"))) return rc;
+    if ((rc = b->alloc(&b->owner, arena, "diff batch owner table"))) return rc;
+    b->M = merge_layout(b->npairs);
+    if ((rc = b->alloc(&b->merge_scratch, b->M.total, "diff batch merge scratch"))) return rc;
+    if ((rc = b->alloc(&b->beg, n + 1, "diff batch ranges"))) return rc;
+    if ((rc = b->alloc(&b->unit_start, n + 1, "diff batch units"))) return rc;
+    b->units_bound = (b->npairs + kUnit - 1) / kUnit + n;
+    b->adjoint_grid = (int)((std::min<int64_t>(b->units_bound, kMaxWaves) + 3) / 4);
+    if ((rc = b->alloc(&b->partial, (size_t)b->units_bound * 64 * 6, "diff batch partials")))
+        return rc;
+    if ((rc = b->alloc(&b->grad, (size_t)frames * 6, "diff batch gradients"))) return rc;
+    if ((rc = b->alloc(&b->adam_m, (size_t)frames * 6, "diff batch adam"))) return rc;
+    if ((rc = b->alloc(&b->adam_v, (size_t)frames * 6, "diff batch adam"))) return rc;
+    if ((rc = b->alloc(&b->step, 1, "diff batch step"))) return rc;
+    RECON_HIP_TRY(hipMemcpyAsync(b->d_vols, b->h_vols.data(), n * sizeof(DVol),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    RECON_HIP_TRY(hipMemcpyAsync(b->d_frames, h_frames.data(), h_frames.size() * sizeof(DFrame),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    RECON_HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(BoundsJob),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    RECON_HIP_TRY(hipMemcpyAsync(b->d_fill_start, fill_start.data(), n * sizeof(int),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    RECON_HIP_TRY(hipMemcpyAsync(b->d_cull_wg, cull_wg.data(), cull_wg.size() * sizeof(uint2),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    if (!loss_wg.empty())
+        RECON_HIP_TRY(hipMemcpyAsync(b->d_loss_wg, loss_wg.data(), loss_wg.size() * sizeof(uint2),
+                                     hipMemcpyHostToDevice, s), "diff batch upload");
+    hipLaunchKernelGGL(dbatch_bounds_kernel, dim3(max_b1x, max_b1y, (unsigned)jobs.size()), dim3(256),
+                       0, s, d_jobs);
+    RECON_HIP_TRY(hipGetLastError(), "dbatch_bounds_kernel launch");
+    if ((rc = launch_fill_u32(b->owner, arena, 0xFFFFFFFFu, s))) return rc;
+    int dev = 0, cus = 256, occ = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dbatch_integrate_kernel, kThreads, 0) !=
+            hipSuccess || occ < 1)
+        occ = 2;
+    b->integrate_grid = cus * occ;
+    // the host tables must outlive the copies
+    RECON_HIP_TRY(hipStreamSynchronize(s), "diff batch create synchronize");
+    return RECON_OK;
+}
+
+extern "C" int recon_diff_batch_create(const recon_diff_volume* volumes, int32_t num_volumes,
+                                       recon_diff_batch** out) {
+    clear_error();
+    if (!out) return fail(RECON_ERR_NULL, "recon_diff_batch_create: out is NULL");
+    *out = nullptr;
+    if (!volumes || num_volumes < 1)
+        return fail(RECON_ERR_SIZE, "recon_diff_batch_create: need at least one volume");
+    int rc = require_gfx950();
+    if (rc) return rc;
+    recon_diff_batch* b = new recon_diff_batch();
+    rc = plan(volumes, num_volumes, b, nullptr);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return RECON_OK;
+}
+
+extern "C" void recon_diff_batch_destroy(recon_diff_batch* b) { delete b; }
+
+extern "C" int recon_diff_batch_volume_data(recon_diff_batch* b, int32_t i, float** distances,
+                                            float** weights, recon_tsdf_info* info) {
+    clear_error();
+    if (!b) return fail(RECON_ERR_NULL, "recon_diff_batch_volume_data: batch is NULL");
+    if (i < 0 || i >= b->n) return fail(RECON_ERR_RANGE, "recon_diff_batch_volume_data: index %d", i);
+    if (distances) *distances = b->h_vols[i].vp.D;
+    if (weights) *weights = b->h_vols[i].vp.W;
+    if (info) *info = b->info[i];
+    return RECON_OK;
+}
+
+extern "C" int recon_diff_batch_download(recon_diff_batch* b, int32_t i, float* host_d,
+                                         float* host_w, recon_stream_t stream) {
+    clear_error();
+    if (!b) return fail(RECON_ERR_NULL, "recon_diff_batch_download: batch is NULL");
+    if (i < 0 || i >= b->n) return fail(RECON_ERR_RANGE, "recon_diff_batch_download: index %d", i);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)b->info[i].num_voxels * sizeof(float);
+    if (host_d)
+        RECON_HIP_TRY(hipMemcpyAsync(host_d, b->h_vols[i].vp.D, bytes, hipMemcpyDeviceToHost, s),
+                      "recon_diff_batch_download");
+    if (host_w)
+        RECON_HIP_TRY(hipMemcpyAsync(host_w, b->h_vols[i].vp.W, bytes, hipMemcpyDeviceToHost, s),
+                      "recon_diff_batch_download");
+    RECON_HIP_TRY(hipStreamSynchronize(s), "recon_diff_batch_download");
+    return RECON_OK;
+}
+
+extern "C" int recon_diff_batch_forward(recon_diff_batch* b, const float* poses_rt, double* loss,
+                                        recon_stream_t stream) {
+    clear_error();
+    if (!b || !poses_rt || !loss) return fail(RECON_ERR_NULL, "recon_diff_batch_forward: NULL");
+    return b->forward(poses_rt, loss, (hipStream_t)stream);
+}
+
+extern "C" int recon_diff_batch_backward(recon_diff_batch* b, double* grad_pose,
+                                         recon_stream_t stream) {
+    clear_error();
+    if (!b || !grad_pose) return fail(RECON_ERR_NULL, "recon_diff_batch_backward: NULL");
+    return b->backward(grad_pose, (hipStream_t)stream);
+}
+
+extern "C" int recon_diff_batch_refine(recon_diff_batch* b, float* poses_rt, int32_t steps,
+                                       const recon_adam_params* opt, double* loss_history,
+                                       recon_stream_t stream) {
+    clear_error();
+    if (!b || !poses_rt || !opt) return fail(RECON_ERR_NULL, "recon_diff_batch_refine: NULL");
+    if (steps < 0) return fail(RECON_ERR_SIZE, "recon_diff_batch_refine: steps < 0");
+    const AdamState st{opt->lr_translation, opt->lr_rotation, opt->beta1, opt->beta2, opt->eps,
+                       opt->decay};
+    if (!(st.lr_t >= 0 && st.lr_r >= 0 && st.b1 >= 0 && st.b1 < 1 && st.b2 >= 0 && st.b2 < 1 &&
+          st.eps > 0 && st.decay > 0))
+        return fail(RECON_ERR_CONFIG, "recon_diff_batch_refine: bad optimiser parameters");
+    hipStream_t s = (hipStream_t)stream;
+    // fresh optimiser state
+    const int64_t nm = (int64_t)b->total_frames * 6;
+    hipLaunchKernelGGL(dbatch_zero_f64_kernel, dim3(64), dim3(256), 0, s, b->adam_m, nm);
+    hipLaunchKernelGGL(dbatch_zero_f64_kernel, dim3(64), dim3(256), 0, s, b->adam_v, nm);
+    hipLaunchKernelGGL(dbatch_reset_step_kernel, dim3(1), dim3(64), 0, s, b->step);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch refine reset");
+    if (steps == 0) return RECON_OK;
+    // one step, captured once and replayed
+    hipStream_t cs = nullptr;
+    RECON_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    RECON_HIP_TRY(hipStreamSynchronize(s), "refine synchronize");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = RECON_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = b->forward(poses_rt, b->loss, cs);
+        if (!rc) rc = b->backward(b->grad, cs);
+        if (!rc) rc = b->update(poses_rt, st, loss_history, cs);
+        const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+        if (!rc && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+        if (!rc) {
+            const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e3 != hipSuccess) rc = hip_fail(e3, "hipGraphInstantiate");
+        }
+    } else {
+        rc = hip_fail(e, "hipStreamBeginCapture");
+    }
+    for (int k = 0; !rc && k < steps; ++k) {
+        const hipError_t e4 = hipGraphLaunch(exec, s);
+        if (e4 != hipSuccess) rc = hip_fail(e4, "hipGraphLaunch");
+    }
+    // the graph may still run: wait before releasing it
+    const hipError_t e5 = hipStreamSynchronize(s);
+    if (!rc && e5 != hipSuccess) rc = hip_fail(e5, "refine synchronize");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    return rc;
+}

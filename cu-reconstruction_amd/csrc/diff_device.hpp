@@ -230,4 +230,123 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
+// ---- adjoint, lanes as frames ----------------------------------------------------
+// For up to 64 frames a wave holds S = 64 / F segments of F lanes; lane f of
+// segment s evaluates frame f of the segment's voxel once.  The running
+// average of a voxel is D' = a D + b tsdf, a = Wb/(Wb+1), b = 1/(Wb+1),
+// Wb = min(updates before the frame, cap): the updates before frame f are
+// the set bits of the segment's ballot below the lane, and dL/dD after frame
+// f is lambda times the product of a over the segment's later frames (a
+// suffix-product scan).  With g = that * b (0 when the frame clamps at
+// +trunc) and sdf = d(s_u, s_v) - z:
+//   dsdf/dp = (dd/du fx/z, dd/dv fy/z, -(dd/du fx x + dd/dv fy y)/z^2 - 1),
+//   dL/drho += g dsdf/dp,  dL/domega += p x (g dsdf/dp),
+//   dL/d(tap) += g * (bilinear weight of the tap).
+struct LaneLayout {
+    int F, S, seg, f;
+    bool ok;             // the lane belongs to a segment
+    uint64_t seg_mask;   // lanes of this lane's segment
+    uint64_t below;      // lanes below this one
+};
+
+__device__ __forceinline__ LaneLayout lane_layout(int F) {
+    LaneLayout ly;
+    const int lane = threadIdx.x & 63;
+    ly.F = F;
+    ly.S = 64 / F;
+    ly.seg = lane / F;
+    ly.f = lane - ly.seg * F;
+    ly.ok = ly.seg < ly.S;
+    ly.seg_mask = (F == 64 ? ~0ull : ((1ull << F) - 1ull)) << (ly.ok ? ly.seg * F : 0);
+    ly.below = (1ull << lane) - 1ull;
+    return ly;
+}
+
+// One pass of the wave: this lane's segment holds voxel v (local index, or
+// -1) with dL/dD = lam; L is the lane's frame.  Every lane must call.
+__device__ __forceinline__ void adjoint_lane_pass(const VolumeParams& vp, const LaneLayout& ly,
+                                                  const FrameParams& L, int64_t v, float lam,
+                                                  float* gd_frame, double acc[6]) {
+    const int64_t N = (int64_t)vp.gx * vp.gy * vp.gz;
+    const bool act = ly.ok && v >= 0 && v < N && lam != 0.0f && isfinite(lam);
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (act) {
+        const int64_t x = v % vp.gx, yz = v / vp.gx;
+        cx = voxel_centre((int)x, vp.vs[0], vp.off[0]);
+        cy = voxel_centre((int)(yz % vp.gy), vp.vs[1], vp.off[1]);
+        cz = voxel_centre((int)(yz / vp.gy), vp.vs[2], vp.off[2]);
+    }
+    bool pass = false;
+    BiProj s;
+    s.px = s.py = s.pz = s.a = s.b = 0.0f;
+    s.addr = 0;
+    float t00 = 0.0f, t10 = 0.0f, t01 = 0.0f, t11 = 0.0f, sdf = 0.0f;
+    if (act) {
+        s = bi_project(vp, L, cx, cy, cz);
+        if (s.in) {
+            const float* q = L.depth + s.addr;
+            t00 = q[0];
+            t10 = q[1];
+            t01 = q[L.pitch];
+            t11 = q[L.pitch + 1];
+            const bool valid = tap_valid(t00, vp.maxd) && tap_valid(t10, vp.maxd) &&
+                               tap_valid(t01, vp.maxd) && tap_valid(t11, vp.maxd);
+            sdf = bi_depth(s.a, s.b, t00, t10, t01, t11) - s.pz;
+            pass = valid && sdf >= -vp.trunc;
+        }
+    }
+    const uint64_t ballot = __ballot(pass);
+    const int before = __popcll(ballot & ly.seg_mask & ly.below);
+    float a = 1.0f, b = 0.0f;
+    if (pass) {
+        const float wb = fminf((float)before, vp.maxw);
+        a = wb / (wb + 1.0f);
+        b = 1.0f / (wb + 1.0f);
+    }
+    // P = product of a over the later frames of the segment
+    const float nxt = __shfl_down(a, 1, 64);
+    float P = (ly.f + 1 < ly.F) ? nxt : 1.0f;
+    for (int d = 1; d < ly.F; d <<= 1) {
+        const float y = __shfl_down(P, d, 64);
+        if (ly.f + d < ly.F) P = P * y;
+    }
+    if (pass && !(sdf > 0.0f && sdf >= vp.trunc)) {
+        const float g = (lam * P) * b;
+        if (g != 0.0f) {
+            const float ia = 1.0f - s.a, ib = 1.0f - s.b;
+            const float dd_du = ib * (t10 - t00) + s.b * (t11 - t01);
+            const float dd_dv = ia * (t01 - t00) + s.a * (t11 - t10);
+            const float iz = 1.0f / s.pz;
+            const float gx = ((g * dd_du) * vp.fx) * iz;
+            const float gy = ((g * dd_dv) * vp.fy) * iz;
+            const float gz = -((gx * s.px + gy * s.py) * iz) - g;
+            acc[0] += gx;
+            acc[1] += gy;
+            acc[2] += gz;
+            acc[3] += (double)s.py * gz - (double)s.pz * gy;
+            acc[4] += (double)s.pz * gx - (double)s.px * gz;
+            acc[5] += (double)s.px * gy - (double)s.py * gx;
+            if (gd_frame) {
+                float* o = gd_frame + s.addr;
+                unsafeAtomicAdd(o, g * (ia * ib));
+                unsafeAtomicAdd(o + 1, g * (s.a * ib));
+                unsafeAtomicAdd(o + L.pitch, g * (ia * s.b));
+                unsafeAtomicAdd(o + L.pitch + 1, g * (s.a * s.b));
+            }
+        }
+    }
+}
+
+// Sum of x over the segments of the wave, in segment order, valid in
+// segment 0.  Every lane must call.
+__device__ __forceinline__ double segment_sum(double x, const LaneLayout& ly) {
+    const int lane = threadIdx.x & 63;
+    double sum = x;
+    for (int k = 1; k < ly.S; ++k) {
+        const double o = __shfl(x, lane + k * ly.F, 64);
+        if (ly.seg == 0) sum += o;
+    }
+    return sum;
+}
+
 }  // namespace recon

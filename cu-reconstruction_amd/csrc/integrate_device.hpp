@@ -168,10 +168,11 @@ __device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const Fram
 }
 
 // Frames (bit f of the result) whose frustum and depth bound reach wave-tile
-// `id` of a volume.
+// `id` of a volume; only frames first, first + step, ... are tested.
 template <class S, class Frames>
 __device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const int nframes,
-                                                const Frames& frames, uint32_t id) {
+                                                const Frames& frames, uint32_t id,
+                                                const int first = 0, const int step = 1) {
     int x0, y0, z0;
     tile_origin<S>(vp, id, x0, y0, z0);
     float lo[3], hi[3];
@@ -182,7 +183,7 @@ __device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const in
     hi[1] = voxel_centre(min(y0 + S::Y - 1, vp.gy - 1), vp.vs[1], vp.off[1]);
     hi[2] = voxel_centre(min(z0 + S::Z - 1, vp.gz - 1), vp.vs[2], vp.off[2]);
     uint64_t live = 0;
-    for (int f = 0; f < nframes; ++f) {
+    for (int f = first; f < nframes; f += step) {
         const FrameParams& F = frames.f[f];
         bool in = true;
 #pragma unroll

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "recon_common.hpp"
+#include "scan.hpp"
 
 #include "recon/mc_tables.h"
 
@@ -152,33 +153,5 @@ __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, 
         base += chunk_total;
     }
 }
-
-// ---- exclusive scan of int32 counts into int64 offsets ---------------------
-constexpr int kScanItems = 4;
-constexpr int kScanBlock = 256 * kScanItems;
-
-__device__ __forceinline__ long long block_exclusive_scan(long long v, long long* sh,
-                                                          long long* total) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const long long o = t >= d ? sh[t - d] : 0;
-        __syncthreads();
-        sh[t] += o;
-        __syncthreads();
-    }
-    const long long incl = sh[t];
-    *total = sh[255];
-    __syncthreads();
-    return incl - v;
-}
-
-// Exclusive scan of n int32 counts into int64 offsets on stream s: three
-// launches (block sums, top-level scan, apply); sums[nb] receives the total,
-// nb = (n + kScanBlock - 1) / kScanBlock.  `offsets` may be null to compute
-// the total only.  Defined in marching_cubes.hip.
-int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
-                          hipStream_t s);
 
 }  // namespace recon

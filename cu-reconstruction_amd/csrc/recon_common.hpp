@@ -151,6 +151,8 @@ struct recon_tsdf {
     // differentiable path scratch (frame table, depth bounds, partial sums)
     void* diff_scratch = nullptr;
     size_t diff_scratch_bytes = 0;
+    uint32_t* owner = nullptr;  // per-voxel table of the adjoint's pair merge
+    bool owner_dirty = false;   // set while a merge is being queued
 };
 
 namespace recon {

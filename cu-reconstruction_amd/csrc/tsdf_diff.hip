@@ -17,17 +17,23 @@
 //                              per-block float64 loss partials reduced in a
 //                              fixed order, and the 8 (voxel, dL/dD) pairs of
 //                              the point;
-//   recon_tsdf_diff_adjoint    one thread per (voxel, dL/dD) pair over a
-//                              bounded grid: the voxel's frame decisions
-//                              replayed forward (count), then backward with
-//                              the chain of running averages; per frame the
-//                              six pose sums are reduced per wave in float64
-//                              into a slot of its own, then summed over waves
-//                              in a fixed order (deterministic); depth
-//                              gradients by float atomics.
+//   recon_tsdf_diff_adjoint    (voxel, dL/dD) pairs over a bounded grid.
+//                              Up to 64 frames: lanes are frames, 64 / F
+//                              voxels per wave, each (voxel, frame) evaluated
+//                              once; the updates before a frame come from a
+//                              ballot, the chain of running averages from a
+//                              suffix-product scan, and each lane keeps its
+//                              frame's pose sums in float64 registers.
+//                              Longer sequences: one voxel per lane, frames
+//                              replayed forward (count) then backward.  Per
+//                              wave partials are summed in a fixed order
+//                              (run-to-run identical); depth gradients by
+//                              float atomics.
 #pragma clang fp contract(off)
 
 #include "diff_device.hpp"
+#include "diff_host.hpp"
+#include "scan.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -70,15 +76,21 @@ __global__ __launch_bounds__(256) void diff_bounds_kernel(const BoundsJob job, s
     bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
 }
 
+// 64 tiles per block; wave g tests frames g, g + 4, ... of every tile, and
+// the four masks of a tile are OR-ed before the append.
 __global__ __launch_bounds__(256) void diff_cull_kernel(const VolumeParams vp, const int nframes,
                                                         const FrameTable ft,
                                                         ListEntry* __restrict__ list,
                                                         unsigned* __restrict__ list_count) {
+    __shared__ uint64_t s_live[4][64];
     __shared__ unsigned s_wave[kThreads / 64];
     __shared__ unsigned s_base;
     const uint32_t ntiles = (uint32_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-    const uint64_t live = id < ntiles ? tile_frames<Shape>(vp, nframes, ft, id) : 0ull;
+    const int i = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const uint32_t id = blockIdx.x * 64u + i;
+    s_live[g][i] = id < ntiles ? tile_frames<Shape>(vp, nframes, ft, id, g, 4) : 0ull;
+    __syncthreads();
+    const uint64_t live = g == 0 ? (s_live[0][i] | s_live[1][i]) | (s_live[2][i] | s_live[3][i]) : 0ull;
     append_tile(live, id, 0u, list, list_count, s_wave, &s_base);
 }
 
@@ -155,12 +167,53 @@ __global__ __launch_bounds__(256) void fixed_sum_kernel(const double* __restrict
 }
 
 // ---- adjoint ---------------------------------------------------------------------
-// Entry e = wave * 64 + lane + k * (waves * 64).  partial[(f*6 + c) * waves +
-// wave] is owned by one wave: written on its first pass, then accumulated.
+// Sequences of up to 64 frames: lanes are frames.  A wave holds S = 64 / F
+// segments of F lanes; segment s takes entry e = (wave + k * waves) * S + s
+// and its lane f evaluates frame f of that voxel once.  The updates before
+// frame f are the set bits of the segment's ballot below the lane; dL/dD
+// after frame f is lambda times the product of a_g over the segment's later
+// frames (a suffix-product scan); each lane accumulates its frame's six pose
+// sums over all the wave's entries.  At the end the segments are summed in
+// order into partial[(f*6 + c) * waves + wave].
+template <class IdT>
 __global__ __launch_bounds__(256) void diff_adjoint_kernel(
-    const VolumeParams vp, const int nframes, const FrameTable ft, const int64_t* __restrict__ ids,
-    const float* __restrict__ lam_in, const int64_t n, double* __restrict__ partial,
-    float* __restrict__ gdepth, const size_t gd_frame_stride) {
+    const VolumeParams vp, const int nframes, const FrameTable ft, const IdT* __restrict__ ids,
+    const float* __restrict__ lam_in, const int64_t n_host, const long long* __restrict__ n_dev,
+    double* __restrict__ partial, float* __restrict__ gdepth, const size_t gd_frame_stride) {
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    const unsigned waves = gridDim.x * 4u;
+    const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const LaneLayout ly = lane_layout(nframes);
+    const FrameParams L = ft.f[ly.ok ? ly.f : 0];  // kept in registers
+    float* gd_frame = gdepth ? gdepth + gd_frame_stride * (ly.ok ? ly.f : 0) : nullptr;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t e0 = (int64_t)wave * ly.S; e0 < n; e0 += (int64_t)waves * ly.S) {
+        const int64_t e = e0 + ly.seg;
+        int64_t v = -1;
+        float lam = 0.0f;
+        if (ly.ok && e < n) {
+            v = (int64_t)ids[e];
+            lam = lam_in[e];
+        }
+        adjoint_lane_pass(vp, ly, L, v, lam, gd_frame, acc);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const double sum = segment_sum(acc[c], ly);
+        if (ly.seg == 0 && ly.ok) partial[((size_t)ly.f * 6 + c) * waves + wave] = sum;
+    }
+}
+
+// Longer sequences: one voxel per lane, frames in a loop (count, then
+// reverse).  Entry e = wave * 64 + lane + k * (waves * 64);
+// partial[(f*6 + c) * waves + wave] is owned by one wave: written on its
+// first pass, then accumulated.
+template <class IdT>
+__global__ __launch_bounds__(256) void diff_adjoint_long_kernel(
+    const VolumeParams vp, const int nframes, const FrameTable ft, const IdT* __restrict__ ids,
+    const float* __restrict__ lam_in, const int64_t n_host, const long long* __restrict__ n_dev,
+    double* __restrict__ partial, float* __restrict__ gdepth, const size_t gd_frame_stride) {
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int lane = threadIdx.x & 63;
     const unsigned waves = gridDim.x * 4u;
     const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -171,10 +224,10 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(
         int64_t v = -1;
         float lam = 0.0f;
         if (e < n) {
-            v = ids[e];
+            v = (int64_t)ids[e];
             lam = lam_in[e];
         }
-        const bool act = v >= 0 && v < N && lam != 0.0f;
+        const bool act = v >= 0 && v < N && lam != 0.0f && isfinite(lam);
         float cx = 0.0f, cy = 0.0f, cz = 0.0f;
         if (act) {
             const int64_t x = v % vp.gx, yz = v / vp.gx;
@@ -258,6 +311,102 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(
             for (int k = 0; k < 6; ++k) partial[((size_t)f * 6 + k) * waves + wave] = 0.0;
 }
 
+// ---- merging repeated voxels -------------------------------------------------------
+// (voxel, value) pairs -> one pair per voxel, in the order of first
+// occurrence, with the values summed exactly: every value is scaled to an
+// int64 fixed-point number (the scale puts the largest |value| at
+// 2^(61 - ceil(log2(n+1))), so no sum of n terms overflows) and added with
+// integer atomics, whose result does not depend on their order.  `owner`
+// (one uint32 per voxel, all ones between calls) takes the smallest pair
+// index of each voxel and is restored at the end.
+__device__ __forceinline__ bool pair_valid(int64_t v, float l, int64_t N) {
+    return v >= 0 && v < N && l != 0.0f && isfinite(l);
+}
+
+__device__ __forceinline__ double merge_scale(unsigned maxbits, int lg) {
+    const float m = __uint_as_float(maxbits);
+    return m > 0.0f ? ldexp(1.0, 61 - lg - ilogbf(m) - 1) : 1.0;
+}
+
+__global__ __launch_bounds__(256) void merge_mark_kernel(const int64_t* __restrict__ ids,
+                                                         const float* __restrict__ lam, int64_t n,
+                                                         int64_t N, uint32_t* __restrict__ owner,
+                                                         unsigned* __restrict__ maxbits) {
+    const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+    unsigned m = 0;
+    if (e < n) {
+        const int64_t v = ids[e];
+        const float l = lam[e];
+        if (pair_valid(v, l, N)) {
+            // the plain read brings the line into L2 (atomics that miss it
+            // wait on memory) and skips pairs an earlier one already beat
+            if ((uint32_t)e < owner[v]) atomicMin(owner + v, (uint32_t)e);
+            m = __float_as_uint(fabsf(l));  // orders like the magnitudes
+        }
+    }
+    // one atomic per wave for the largest magnitude
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, d, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(maxbits, m);
+}
+
+__global__ __launch_bounds__(256) void merge_flag_kernel(const int64_t* __restrict__ ids,
+                                                         const float* __restrict__ lam, int64_t n,
+                                                         int64_t N, const uint32_t* __restrict__ owner,
+                                                         int32_t* __restrict__ head) {
+    const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+    if (e >= n) return;
+    const int64_t v = ids[e];
+    head[e] = pair_valid(v, lam[e], N) && owner[v] == (uint32_t)e ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void merge_slot_kernel(const int64_t* __restrict__ ids, int64_t n,
+                                                         const int32_t* __restrict__ head,
+                                                         const long long* __restrict__ offsets,
+                                                         uint32_t* __restrict__ uid,
+                                                         long long* __restrict__ acc) {
+    const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+    if (e >= n || !head[e]) return;
+    const long long slot = offsets[e];
+    uid[slot] = (uint32_t)ids[e];
+    acc[slot] = 0;
+}
+
+__global__ __launch_bounds__(256) void merge_add_kernel(const int64_t* __restrict__ ids,
+                                                        const float* __restrict__ lam, int64_t n,
+                                                        int64_t N, const uint32_t* __restrict__ owner,
+                                                        const long long* __restrict__ offsets,
+                                                        const unsigned* __restrict__ maxbits, int lg,
+                                                        long long* __restrict__ acc) {
+    const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+    if (e >= n) return;
+    const int64_t v = ids[e];
+    const float l = lam[e];
+    if (!pair_valid(v, l, N)) return;
+    const double scale = merge_scale(*maxbits, lg);
+    const long long q = llrint((double)l * scale);
+    atomicAdd(reinterpret_cast<unsigned long long*>(acc + offsets[owner[v]]),
+              (unsigned long long)q);
+}
+
+__global__ __launch_bounds__(256) void merge_finish_kernel(const long long* __restrict__ count,
+                                                           const uint32_t* __restrict__ uid,
+                                                           const long long* __restrict__ acc,
+                                                           const unsigned* __restrict__ maxbits,
+                                                           int lg, float* __restrict__ lam_u,
+                                                           uint32_t* __restrict__ owner) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= *count) return;
+    lam_u[i] = (float)((double)acc[i] / merge_scale(*maxbits, lg));
+    owner[uid[i]] = 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n,
+                                                       uint32_t value) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = value;
+}
+
 __global__ __launch_bounds__(256) void zero_frames_kernel(float* __restrict__ p, size_t frame_stride,
                                                           size_t per_frame) {
     float* base = p + frame_stride * blockIdx.y;
@@ -266,7 +415,6 @@ __global__ __launch_bounds__(256) void zero_frames_kernel(float* __restrict__ p,
 }
 
 // ---- host side ----------------------------------------------------------------------
-static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Grows the handle's scratch for the differentiable path to `bytes`.  A
 // buffer being replaced may still be read by queued kernels, so the stream is
@@ -340,6 +488,79 @@ static DiffLayout diff_layout(const VolumeParams& vp, int nframes, size_t partia
     L.partial = L.bounds + align256(per * kDiffChunk * sizeof(float));
     L.total = L.partial + align256(partial_doubles * sizeof(double));
     return L;
+}
+
+MergeLayout merge_layout(int64_t n) {
+    MergeLayout M;
+    const size_t un = (size_t)std::max<int64_t>(n, 1);
+    M.nb = (int)((un + kScanBlock - 1) / kScanBlock);
+    M.head = 0;
+    M.offsets = M.head + align256(4 * un);
+    M.sums = M.offsets + align256(8 * un);
+    M.uid = M.sums + align256(8 * ((size_t)M.nb + 1));
+    M.lam = M.uid + align256(4 * un);
+    M.acc = M.lam + align256(4 * un);
+    M.maxbits = M.acc + align256(8 * un);
+    M.total = M.maxbits + 256;
+    return M;
+}
+
+int launch_merge(const int64_t* ids, const float* lam, int64_t n, int64_t N, uint32_t* owner,
+                 char* mb, const MergeLayout& M, uint32_t** uid_out, float** lam_out,
+                 const long long** count_out, hipStream_t s) {
+    int32_t* head = reinterpret_cast<int32_t*>(mb + M.head);
+    long long* offsets = reinterpret_cast<long long*>(mb + M.offsets);
+    long long* sums = reinterpret_cast<long long*>(mb + M.sums);
+    uint32_t* uid = reinterpret_cast<uint32_t*>(mb + M.uid);
+    float* lam_u = reinterpret_cast<float*>(mb + M.lam);
+    long long* acc = reinterpret_cast<long long*>(mb + M.acc);
+    unsigned* maxbits = reinterpret_cast<unsigned*>(mb + M.maxbits);
+    const dim3 pg((unsigned)((n + 255) / 256));
+    const int lg = 64 - __builtin_clzll((unsigned long long)n);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, s, maxbits, 1, 0u);
+    hipLaunchKernelGGL(merge_mark_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, maxbits);
+    hipLaunchKernelGGL(merge_flag_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, head);
+    RECON_HIP_TRY(hipGetLastError(), "merge kernels launch");
+    int rc = launch_exclusive_scan(head, (int)n, offsets, sums, s);
+    if (rc) return rc;
+    const long long* count = sums + M.nb;
+    hipLaunchKernelGGL(merge_slot_kernel, pg, dim3(256), 0, s, ids, n, head, offsets, uid, acc);
+    hipLaunchKernelGGL(merge_add_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, offsets,
+                       maxbits, lg, acc);
+    hipLaunchKernelGGL(merge_finish_kernel, pg, dim3(256), 0, s, count, uid, acc, maxbits, lg,
+                       lam_u, owner);
+    RECON_HIP_TRY(hipGetLastError(), "merge kernels launch");
+    *uid_out = uid;
+    *lam_out = lam_u;
+    *count_out = count;
+    return RECON_OK;
+}
+
+int launch_fixed_sum(const double* in, int64_t n, double* out, int nout, double scale,
+                     hipStream_t s) {
+    if (nout <= 0) return RECON_OK;
+    hipLaunchKernelGGL(fixed_sum_kernel, dim3(nout), dim3(256), 0, s, in, n, out, scale);
+    RECON_HIP_TRY(hipGetLastError(), "fixed_sum_kernel launch");
+    return RECON_OK;
+}
+
+int launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, value);
+    RECON_HIP_TRY(hipGetLastError(), "fill_u32_kernel launch");
+    return RECON_OK;
+}
+
+// The per-voxel owner table of the merge (all ones between calls), made on
+// first use.
+static int ensure_owner(recon_tsdf* h, hipStream_t s) {
+    if (h->owner && !h->owner_dirty) return RECON_OK;
+    const int64_t n = h->info.num_voxels;
+    if (!h->owner)
+        RECON_HIP_TRY(hipMalloc((void**)&h->owner, (size_t)n * sizeof(uint32_t)),
+                      "merge owner table");
+    h->owner_dirty = false;
+    return launch_fill_u32(h->owner, n, 0xFFFFFFFFu, s);
 }
 
 // Frame table of all frames into the scratch (device side, from the poses).
@@ -421,7 +642,7 @@ extern "C" int recon_tsdf_diff_integrate(recon_tsdf* h, const recon_depth_view* 
                            frame_stride_bytes / 4, per, h->list_count);
         RECON_HIP_TRY(hipGetLastError(), "diff_bounds_kernel launch");
         const FrameTable ft{table + f0};
-        hipLaunchKernelGGL(diff_cull_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, s, vp, n, ft,
+        hipLaunchKernelGGL(diff_cull_kernel, dim3((ntiles + 63) / 64), dim3(256), 0, s, vp, n, ft,
                            list, h->list_count);
         RECON_HIP_TRY(hipGetLastError(), "diff_cull_kernel launch");
         if (vp.gx % 4 == 0)
@@ -489,13 +710,19 @@ extern "C" int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* de
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const VolumeParams vp = diff_volume_params(h, depth0->cols, depth0->rows);
-    const int64_t want_waves = std::max<int64_t>((num_lambda + 63) / 64, 1);
+    const int64_t per_wave = num_frames <= 64 ? 64 / num_frames : 64;  // entries per pass
+    const int64_t want_waves = std::max<int64_t>((num_lambda + per_wave - 1) / per_wave, 1);
     int64_t cap = std::max<int64_t>(64, (int64_t(32) << 20) / (48 * (int64_t)num_frames));
     cap = std::min<int64_t>(cap, kAdjointMaxWaves);
     const int64_t blocks = (std::min(want_waves, cap) + 3) / 4;
     const int64_t waves = blocks * 4;
     const DiffLayout L = diff_layout(vp, num_frames, (size_t)waves * 6 * num_frames);
-    if ((rc = ensure_diff_scratch(h, L.total, s))) return rc;
+    // repeated voxels are merged first unless ids or counts exceed 32 bits
+    const int64_t N = h->info.num_voxels;
+    const bool merge = num_lambda > 0 && N < 0xFFFFFFFFll && num_lambda < 0x7FFFFFFFll;
+    const MergeLayout M = merge_layout(num_lambda);
+    if ((rc = ensure_diff_scratch(h, L.total + (merge ? M.total : 0), s))) return rc;
+    if (merge && (rc = ensure_owner(h, s))) return rc;
     char* base = static_cast<char*>(h->diff_scratch);
     launch_setup(vp, depth0, frame_stride_bytes, num_frames, poses, base, L, s);
     RECON_HIP_TRY(hipGetLastError(), "diff_setup_kernel launch");
@@ -509,11 +736,35 @@ extern "C" int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* de
     }
     double* partial = reinterpret_cast<double*>(base + L.partial);
     const FrameTable ft{reinterpret_cast<const FrameParams*>(base + L.table)};
-    hipLaunchKernelGGL(diff_adjoint_kernel, dim3((unsigned)blocks), dim3(256), 0, s, vp, num_frames,
-                       ft, lambda_ids, lambda, num_lambda, partial, grad_depth, gstride);
+    const dim3 grid((unsigned)blocks);
+    if (merge) {
+        // a failure past this point may leave owner entries set: refill later
+        h->owner_dirty = true;
+        uint32_t* uid = nullptr;
+        float* lam_u = nullptr;
+        const long long* count = nullptr;
+        if ((rc = launch_merge(lambda_ids, lambda, num_lambda, N, h->owner, base + L.total, M, &uid,
+                               &lam_u, &count, s)))
+            return rc;
+        h->owner_dirty = false;
+        if (num_frames <= 64)
+            hipLaunchKernelGGL(diff_adjoint_kernel<uint32_t>, grid, dim3(256), 0, s, vp, num_frames,
+                               ft, uid, lam_u, num_lambda, count, partial, grad_depth, gstride);
+        else
+            hipLaunchKernelGGL(diff_adjoint_long_kernel<uint32_t>, grid, dim3(256), 0, s, vp,
+                               num_frames, ft, uid, lam_u, num_lambda, count, partial, grad_depth,
+                               gstride);
+    } else {
+        const long long* none = nullptr;
+        if (num_frames <= 64)
+            hipLaunchKernelGGL(diff_adjoint_kernel<int64_t>, grid, dim3(256), 0, s, vp, num_frames,
+                               ft, lambda_ids, lambda, num_lambda, none, partial, grad_depth,
+                               gstride);
+        else
+            hipLaunchKernelGGL(diff_adjoint_long_kernel<int64_t>, grid, dim3(256), 0, s, vp,
+                               num_frames, ft, lambda_ids, lambda, num_lambda, none, partial,
+                               grad_depth, gstride);
+    }
     RECON_HIP_TRY(hipGetLastError(), "diff_adjoint_kernel launch");
-    hipLaunchKernelGGL(fixed_sum_kernel, dim3(6 * num_frames), dim3(256), 0, s, partial, waves,
-                       grad_pose, 1.0);
-    RECON_HIP_TRY(hipGetLastError(), "fixed_sum_kernel launch");
-    return RECON_OK;
+    return launch_fixed_sum(partial, waves, grad_pose, 6 * num_frames, 1.0, s);
 }

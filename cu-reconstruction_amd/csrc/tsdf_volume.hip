@@ -295,6 +295,7 @@ extern "C" void recon_tsdf_destroy(recon_tsdf* h) {
     if (h->bounds) (void)hipFree(h->bounds);
     if (h->list) (void)hipFree(h->list);
     if (h->diff_scratch) (void)hipFree(h->diff_scratch);
+    if (h->owner) (void)hipFree(h->owner);
     delete h;
 }
 

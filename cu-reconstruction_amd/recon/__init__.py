@@ -392,3 +392,98 @@ class Batch:
         check(self._lib.recon_batch_download_mesh(self._h, i, out.ctypes.data, n, C.byref(got),
                                                   _stream(stream)))
         return out[:got.value]
+
+
+class DiffBatch:
+    """Pose refinement over many fragments (recon_diff_batch_*).  `volumes` is
+    a list of (config, depth tensor (F, H, W) on the device, points (K, 3)
+    float32 device tensor, weights (K,) float32 device tensor or None); the
+    tensors are kept alive here because every step reads them."""
+
+    def __init__(self, volumes):
+        torch = _torch()
+        self._lib = ffi.load()
+        self._keep = []
+        arr = (ffi.DiffVolume * len(volumes))()
+        frames = 0
+        for i, (cfg, depth, points, weights) in enumerate(volumes):
+            if depth.dim() != 3 or depth.stride(2) != 1 or depth.dtype != torch.float32:
+                raise ValueError("depth must be a (F, H, W) float32 tensor")
+            if points.dtype != torch.float32 or not points.is_contiguous() or points.dim() != 2:
+                raise ValueError("points must be a contiguous (K, 3) float32 tensor")
+            self._keep.append((depth, points, weights))
+            v = arr[i]
+            v.config = cfg
+            v.num_frames = int(depth.shape[0])
+            v.rows, v.cols = int(depth.shape[1]), int(depth.shape[2])
+            v.depth_step = depth.stride(1) * 4
+            v.frame_stride = depth.stride(0) * 4
+            v.depth = depth.data_ptr()
+            v.points = points.data_ptr()
+            v.weights = weights.data_ptr() if weights is not None else None
+            v.num_points = int(points.shape[0])
+            frames += v.num_frames
+        self.n = len(volumes)
+        self.total_frames = frames
+        h = C.c_void_p()
+        check(self._lib.recon_diff_batch_create(arr, self.n, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.recon_diff_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check_rt(self, rt):
+        torch = _torch()
+        if rt.dtype != torch.float32 or not rt.is_contiguous() or \
+                tuple(rt.shape) != (self.total_frames, 12):
+            raise ValueError("poses_rt must be a contiguous (total frames, 12) float32 tensor")
+
+    def forward(self, poses_rt, stream=None):
+        """(n,) float64 device tensor of per-volume losses."""
+        torch = _torch()
+        self._check_rt(poses_rt)
+        loss = torch.empty(self.n, dtype=torch.float64, device=poses_rt.device)
+        check(self._lib.recon_diff_batch_forward(self._h, poses_rt.data_ptr(), loss.data_ptr(),
+                                                 _stream(stream)))
+        return loss
+
+    def backward(self, stream=None):
+        """(total frames, 6) float64 gradients for the last forward."""
+        torch = _torch()
+        g = torch.empty((self.total_frames, 6), dtype=torch.float64, device="cuda")
+        check(self._lib.recon_diff_batch_backward(self._h, g.data_ptr(), _stream(stream)))
+        return g
+
+    def refine(self, poses_rt, steps, lr_translation=1e-3, lr_rotation=2e-3, beta1=0.9,
+               beta2=0.999, eps=1e-12, decay=0.95, stream=None):
+        """Refines poses_rt in place; returns the (steps, n) loss history."""
+        torch = _torch()
+        self._check_rt(poses_rt)
+        hist = torch.empty((max(steps, 1), self.n), dtype=torch.float64, device=poses_rt.device)
+        opt = ffi.AdamParams(lr_translation, lr_rotation, beta1, beta2, eps, decay)
+        check(self._lib.recon_diff_batch_refine(self._h, poses_rt.data_ptr(), int(steps),
+                                                C.byref(opt), hist.data_ptr(), _stream(stream)))
+        return hist[:steps]
+
+    def volume(self, i):
+        """(D, W) device pointers and the info of volume i."""
+        d, w = C.c_void_p(), C.c_void_p()
+        info = TsdfInfo()
+        check(self._lib.recon_diff_batch_volume_data(self._h, i, C.byref(d), C.byref(w),
+                                                     C.byref(info)))
+        return d.value, w.value, info
+
+    def download(self, i, stream=None):
+        """(D, W) of volume i as numpy arrays in (z, y, x) order."""
+        _, _, info = self.volume(i)
+        shape = (int(info.grid[2]), int(info.grid[1]), int(info.grid[0]))
+        D = np.empty(shape, np.float32)
+        W = np.empty(shape, np.float32)
+        check(self._lib.recon_diff_batch_download(self._h, i, D.ctypes.data, W.ctypes.data,
+                                                  _stream(stream)))
+        return D, W

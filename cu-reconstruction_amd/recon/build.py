@@ -40,7 +40,7 @@ CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
 
 KERNEL_SOURCES = ["recon_common.hip", "tsdf_volume.hip", "marching_cubes.hip", "depth_points.hip",
-                  "tsdf_batch.hip", "tsdf_diff.hip"]
+                  "tsdf_batch.hip", "tsdf_diff.hip", "diff_batch.hip"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

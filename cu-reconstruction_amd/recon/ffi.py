@@ -63,6 +63,19 @@ class BatchVolume(C.Structure):
                 ("depth", C.c_void_p), ("poses", C.c_void_p)]
 
 
+class DiffVolume(C.Structure):
+    _fields_ = [("config", TsdfConfig), ("num_frames", C.c_int32), ("rows", C.c_int32),
+                ("cols", C.c_int32), ("depth_step", C.c_size_t), ("frame_stride", C.c_size_t),
+                ("depth", C.c_void_p), ("points", C.c_void_p), ("weights", C.c_void_p),
+                ("num_points", C.c_int64)]
+
+
+class AdamParams(C.Structure):
+    _fields_ = [("lr_translation", C.c_double), ("lr_rotation", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
+                ("decay", C.c_double)]
+
+
 class DepthView(C.Structure):
     _fields_ = [("data", C.c_void_p), ("step", C.c_size_t), ("rows", C.c_int32),
                 ("cols", C.c_int32)]
@@ -111,6 +124,14 @@ SIGNATURES = {
     "recon_batch_download": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
     "recon_batch_download_mesh": (C.c_int, [_vp, _i32, _vp, _i64, _pi64, _vp]),
     "recon_pose_to_matrix": (C.c_int, [_vp, _i32, _vp]),
+    "recon_diff_batch_create": (C.c_int, [C.POINTER(DiffVolume), _i32, C.POINTER(_vp)]),
+    "recon_diff_batch_destroy": (None, [_vp]),
+    "recon_diff_batch_volume_data": (C.c_int, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp),
+                                               C.POINTER(TsdfInfo)]),
+    "recon_diff_batch_download": (C.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "recon_diff_batch_forward": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "recon_diff_batch_backward": (C.c_int, [_vp, _vp, _vp]),
+    "recon_diff_batch_refine": (C.c_int, [_vp, _vp, _i32, C.POINTER(AdamParams), _vp, _vp]),
     "recon_tsdf_diff_integrate": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp]),
     "recon_tsdf_surface_loss": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "recon_tsdf_diff_adjoint": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp,

@@ -332,13 +332,13 @@ def _union_sdf(prims, p: np.ndarray) -> np.ndarray:
 
 
 def _sample_prim_surface(kind, q, m: int, lo: np.ndarray, hi: np.ndarray,
-                         rng: np.random.Generator) -> tuple[np.ndarray, float]:
+                         rng: np.random.Generator) -> tuple[np.ndarray, np.ndarray, float]:
     """m points uniform on one primitive's surface (a z plane: its part over
-    the box [lo, hi]) and that surface's area."""
+    the box [lo, hi]), their outward normals and that surface's area."""
     if kind == PRIM_SPHERE:
         v = rng.normal(size=(m, 3))
         v /= np.linalg.norm(v, axis=1, keepdims=True)
-        return np.asarray(q[:3]) + q[3] * v, 4.0 * math.pi * q[3] ** 2
+        return np.asarray(q[:3]) + q[3] * v, v, 4.0 * math.pi * q[3] ** 2
     if kind == PRIM_BOX:
         b0, b1 = np.asarray(q[:3], np.float64), np.asarray(q[3:6], np.float64)
         s = b1 - b0
@@ -347,16 +347,19 @@ def _sample_prim_surface(kind, q, m: int, lo: np.ndarray, hi: np.ndarray,
         pts = b0 + rng.uniform(size=(m, 3)) * s
         axis = face // 2
         pts[np.arange(m), axis] = np.where(face % 2 == 0, b0[axis], b1[axis])
-        return pts, float(areas.sum())
+        nrm = np.zeros((m, 3))
+        nrm[np.arange(m), axis] = np.where(face % 2 == 0, -1.0, 1.0)
+        return pts, nrm, float(areas.sum())
     pts = np.c_[rng.uniform(lo[0], hi[0], m), rng.uniform(lo[1], hi[1], m), np.full(m, q[0])]
-    return pts, float((hi[0] - lo[0]) * (hi[1] - lo[1]))
+    return pts, np.tile([0.0, 0.0, 1.0], (m, 1)), float((hi[0] - lo[0]) * (hi[1] - lo[1]))
 
 
-def surface_points(w: Workload, n: int, seed: int = 0x5EED0003, margin_voxels: float = 1.5) -> np.ndarray:
+def surface_points(w: Workload, n: int, seed: int = 0x5EED0003, margin_voxels: float = 1.5,
+                   with_normals: bool = False):
     """n points (float64, (n, 3)) uniform over the visible surface of the
     scene inside the volume shrunk by `margin_voxels`: each primitive's
     surface, less the parts inside another primitive, drawn in proportion to
-    its clipped area."""
+    its clipped area.  With `with_normals`, also their outward normals."""
     rng = np.random.default_rng(seed)
     vs = w.voxel_size
     lo = np.asarray(w.offset, np.float64) + margin_voxels * vs
@@ -364,14 +367,202 @@ def surface_points(w: Workload, n: int, seed: int = 0x5EED0003, margin_voxels: f
     pools, areas = [], []
     m = max(4 * n, 4096)
     for kind, q in w.prims:
-        pts, area = _sample_prim_surface(kind, q, m, lo, hi, rng)
+        pts, nrm, area = _sample_prim_surface(kind, q, m, lo, hi, rng)
         keep = np.all((pts >= lo) & (pts <= hi), axis=1) & (_union_sdf(w.prims, pts) >= -1e-9)
-        pools.append(pts[keep])
+        pools.append((pts[keep], nrm[keep]))
         areas.append(area * keep.mean())
     areas = np.asarray(areas)
     if areas.sum() <= 0:
-        return np.zeros((0, 3))
+        return (np.zeros((0, 3)), np.zeros((0, 3))) if with_normals else np.zeros((0, 3))
     counts = rng.multinomial(n, areas / areas.sum())
-    out = [pool[rng.choice(len(pool), size=c, replace=c > len(pool))] for pool, c in zip(pools, counts)
-           if c > 0]
-    return np.concatenate(out, axis=0)
+    picks = [(pool, nrm, rng.choice(len(pool), size=c, replace=c > len(pool)))
+             for (pool, nrm), c in zip(pools, counts) if c > 0]
+    pts = np.concatenate([pool[i] for pool, _, i in picks], axis=0)
+    if not with_normals:
+        return pts
+    return pts, np.concatenate([nrm[i] for _, nrm, i in picks], axis=0)
+
+
+def perturb_pose(pose, rot_rad: float, trans_m: float, rng: Rng) -> np.ndarray:
+    """Left perturbation of a world->camera pose {qw,qx,qy,qz,tx,ty,tz}: a
+    rotation by up to `rot_rad` about a random axis through the camera centre
+    and a shift of the centre by up to `trans_m` (uniform magnitudes)."""
+    R, t = pose_matrix(pose)
+    axis = np.array([rng.uniform(-1, 1) for _ in range(3)])
+    axis /= max(np.linalg.norm(axis), 1e-12)
+    ang = rng.uniform(0.0, rot_rad)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    dR = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+    d = np.array([rng.uniform(-1, 1) for _ in range(3)])
+    d *= rng.uniform(0.0, trans_m) / max(np.linalg.norm(d), 1e-12)
+    R2 = dR @ R
+    t2 = dR @ t + d
+    return np.r_[quat_from_matrix(R2), t2]
+
+
+def pose_refinement(num_fragments: int = 32, frames: int = 10, seed: int = 0x5EED0004,
+                    grid: int = 128, voxel_size: float = 0.008) -> list[Workload]:
+    """Fragments for batched pose refinement: each a 128^3 volume (8 mm) of a
+    distinct seeded scene of 2-5 spheres and boxes (at least one box, so no
+    rotation leaves the scene unchanged), seen by `frames` 640x480 cameras on
+    an arc 10 degrees apart.  `poses` are the ground truth; meta["init"] are
+    the poses perturbed by up to 1 degree and 1 cm."""
+    rng = Rng(seed)
+    out = []
+    half = grid * voxel_size / 2
+    for v in range(num_fragments):
+        prims = []
+        nprim = rng.integer(2, 5)
+        for k in range(nprim):
+            if k == 0 or rng.uniform() < 0.5:
+                c = [rng.uniform(-0.2, 0.2) for _ in range(3)]
+                h = [rng.uniform(0.05, 0.18) for _ in range(3)]
+                prims.append((PRIM_BOX, tuple(c[i] - h[i] for i in range(3)) +
+                              tuple(c[i] + h[i] for i in range(3))))
+            else:
+                prims.append((PRIM_SPHERE, (rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2),
+                                            rng.uniform(-0.2, 0.2), rng.uniform(0.06, 0.2))))
+        poses = arc_poses(frames, step_deg=10.0, radius=1.1, height=rng.uniform(0.1, 0.6),
+                          start_deg=rng.uniform(0.0, 360.0))
+        init = np.stack([perturb_pose(p, math.radians(1.0), 0.01, rng) for p in poses])
+        out.append(Workload(
+            name=f"pose_refinement_{v}_{grid}^3x{frames}", grid=(grid, grid, grid),
+            voxel_size=voxel_size, offset=(-half, -half, -half), truncation=4 * voxel_size,
+            max_weight=128.0, intrinsics=(525.0, 525.0, 319.5, 239.5), image=(640, 480),
+            prims=prims, poses=poses, noise=(0.001, 0.0, 0.02, seed ^ (v * 0x9E37)),
+            meta={"index": v, "init": init}))
+    return out
+
+
+def pose_errors(est, gt) -> tuple[np.ndarray, np.ndarray]:
+    """(rotation error in degrees, camera-centre error in metres) per frame of
+    world->camera poses given as (F, 12) [R | t] or (F, 7) quaternion poses."""
+    def mats(p):
+        p = np.asarray(p, np.float64)
+        if p.shape[-1] == 12:
+            return p[:, :9].reshape(-1, 3, 3), p[:, 9:]
+        Rs, ts = zip(*[pose_matrix(q) for q in p])
+        return np.stack(Rs), np.stack(ts)
+    Re, te = mats(est)
+    Rg, tg = mats(gt)
+    dR = Re @ np.transpose(Rg, (0, 2, 1))
+    # angle from both the symmetric and the skew part (exact near zero)
+    skew = np.stack([dR[:, 2, 1] - dR[:, 1, 2], dR[:, 0, 2] - dR[:, 2, 0], dR[:, 1, 0] - dR[:, 0, 1]], 1)
+    ang = np.degrees(np.arctan2(0.5 * np.linalg.norm(skew, axis=1),
+                                0.5 * (np.trace(dR, axis1=1, axis2=2) - 1)))
+    ce = -np.einsum("fji,fj->fi", Re, te)
+    cg = -np.einsum("fji,fj->fi", Rg, tg)
+    return ang, np.linalg.norm(ce - cg, axis=1)
+
+
+def observed_points(w: Workload, W: np.ndarray, points: np.ndarray) -> np.ndarray:
+    """float32 weights: 1 for points whose eight trilinear support voxels
+    (lower corner floor(q/vs - 0.5), q = p - offset) all have weight > 0 in
+    the (z, y, x) weight volume W, else 0."""
+    g = np.asarray(w.grid)
+    vs = np.float32(w.voxel_size)
+    q = (np.asarray(points, np.float64) - np.asarray(w.offset, np.float64)) / np.float64(vs) - 0.5
+    lo = np.floor(q).astype(np.int64)
+    inside = np.all((lo >= 0) & (lo + 1 <= g - 1), axis=1)
+    lo = np.clip(lo, 0, g - 2)
+    ok = inside.copy()
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                ok &= W[lo[:, 2] + dz, lo[:, 1] + dy, lo[:, 0] + dx] > 0
+    return ok.astype(np.float32)
+
+
+
+def visible_points(w: Workload, depth: np.ndarray, poses, points: np.ndarray, tol: float,
+                   min_frames: int = 1, normals: np.ndarray | None = None,
+                   min_cos: float = 0.0, window: int = 0) -> np.ndarray:
+    """float32 weights: 1 for points that at least `min_frames` frames see,
+    i.e. whose camera z lies within `tol` of the depth at their projected
+    pixel (nearest pixel, valid depth, in front of the camera).  With
+    `normals`, a frame counts only when |n . view direction| >= min_cos; with
+    `window` > 0, only when every depth of the (2 window + 1)^2 pixels around
+    the projection is valid and within `tol` of the point (no silhouette)."""
+    fx, fy, cx, cy = w.intrinsics
+    cols, rows = w.image
+    P = np.asarray(points, np.float64)
+    seen = np.zeros(len(P), np.int32)
+    for f, pose in enumerate(np.asarray(poses)):
+        if pose.shape[-1] == 12:
+            R, t = pose[:9].reshape(3, 3).astype(np.float64), pose[9:].astype(np.float64)
+        else:
+            R, t = pose_matrix(pose)
+        pc = P @ R.T + t
+        z = pc[:, 2]
+        with np.errstate(all="ignore"):
+            u = np.floor(pc[:, 0] * fx / z + cx + 0.5).astype(np.int64)
+            v = np.floor(pc[:, 1] * fy / z + cy + 0.5).astype(np.int64)
+        r = window
+        ok = (z > 0) & (u >= r) & (u < cols - r) & (v >= r) & (v < rows - r)
+        good = ok.copy()
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                d = np.zeros(len(P))
+                d[ok] = depth[f][v[ok] + dy, u[ok] + dx]
+                good &= (d > 0) & (np.abs(d - z) < tol)
+        if normals is not None:
+            view = pc / np.maximum(np.linalg.norm(pc, axis=1, keepdims=True), 1e-12)
+            nc = np.asarray(normals, np.float64) @ R.T
+            good &= np.abs(np.sum(nc * view, axis=1)) >= min_cos
+        seen += good.astype(np.int32)
+    return (seen >= min_frames).astype(np.float32)
+
+
+def refinement_points(w: Workload, depth: np.ndarray, n: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """Surface points for pose refinement and their float32 weights: n points
+    uniform on the scene's surfaces, weight 1 where at least 3 frames see the
+    point at its initial pose (meta["init"]) away from silhouettes (5x5
+    depth window within 3 cm) and not at grazing angles (|n . view| >= 0.4);
+    elsewhere the fused TSDF is not locally linear and its residuals would
+    pull the poses."""
+    pts, nrm = surface_points(w, n, seed=seed, with_normals=True)
+    wts = visible_points(w, depth, w.meta.get("init", w.poses), pts, tol=0.03, min_frames=3,
+                         normals=nrm, min_cos=0.4, window=2)
+    return pts, wts
+
+
+def mesh_rms(w: Workload, verts: np.ndarray) -> float:
+    """RMS distance of mesh vertices (any shape (..., 3)) to the scene's
+    analytic surface (its union signed distance)."""
+    v = np.asarray(verts, np.float64).reshape(-1, 3)
+    if len(v) == 0:
+        return float("nan")
+    return float(np.sqrt(np.mean(_union_sdf(w.prims, v) ** 2)))
+
+
+def world_error_twists(est, gt) -> np.ndarray:
+    """(F, 6) world-frame error of each world->camera pose: E = T_est^-1 T_gt,
+    as (rho = its translation in metres, omega = its small-angle rotation
+    vector in radians).  A rigid motion of a whole fragment against the
+    model gives every frame the same E."""
+    def mats(p):
+        p = np.asarray(p, np.float64)
+        return p[:, :9].reshape(-1, 3, 3), p[:, 9:]
+    Re, te = mats(est)
+    Rg, tg = mats(gt)
+    RE = np.transpose(Re, (0, 2, 1)) @ Rg
+    tE = np.einsum("fji,fj->fi", Re, tg - te)
+    om = 0.5 * np.stack([RE[:, 2, 1] - RE[:, 1, 2], RE[:, 0, 2] - RE[:, 2, 0], RE[:, 1, 0] - RE[:, 0, 1]], 1)
+    return np.c_[tE, om]
+
+
+def observed_vertices(w: Workload, W: np.ndarray, verts: np.ndarray) -> np.ndarray:
+    """Mask of mesh vertices whose eight surrounding voxel centres all have
+    weight > 0 in the (z, y, x) weight volume W (marching cubes on D alone
+    also meets the initial value of unobserved voxels behind surfaces)."""
+    v = np.asarray(verts, np.float64).reshape(-1, 3)
+    g = np.asarray(w.grid)
+    q = (v - np.asarray(w.offset, np.float64)) / np.float64(np.float32(w.voxel_size)) - 0.5
+    lo = np.floor(q).astype(np.int64)
+    ok = np.all((lo >= 0) & (lo + 1 <= g - 1), axis=1)
+    lo = np.clip(lo, 0, g - 2)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                ok &= W[lo[:, 2] + dz, lo[:, 1] + dy, lo[:, 0] + dx] > 0
+    return ok

@@ -233,6 +233,61 @@ RECON_API int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* dep
                                       const float* lambda, int64_t num_lambda, double* grad_pose,
                                       float* grad_depth, recon_stream_t stream);
 
+/* ---- batched pose refinement --------------------------------------------- */
+/* No counterpart in the reference.  Many independent fragments (volumes of
+ * their own size, frames and image size), each with its surface points,
+ * fused with recon_tsdf_diff_integrate's bilinear update and scored with
+ * recon_tsdf_surface_loss, over one arena; the poses of all frames are
+ * refined on the device.  All device memory is allocated at creation. */
+typedef struct recon_diff_batch recon_diff_batch;
+
+typedef struct recon_diff_volume {
+    recon_tsdf_config config;  /* volume and the intrinsics of its frames */
+    int32_t num_frames;        /* 1..64 */
+    int32_t rows, cols;        /* depth image size (>= 2x2) */
+    size_t depth_step;         /* bytes per depth row */
+    size_t frame_stride;       /* bytes between frames */
+    const float* depth;        /* device: frame f at depth + f * frame_stride */
+    const float* points;       /* device: 3 floats per surface point (world) */
+    const float* weights;      /* device: one per point, or NULL (all 1) */
+    int64_t num_points;
+} recon_diff_volume;
+
+typedef struct recon_adam_params {
+    double lr_translation;     /* step on rho (metres) at the first step */
+    double lr_rotation;        /* step on omega (radians) at the first step */
+    double beta1, beta2, eps;  /* Adam moments */
+    double decay;              /* both steps are multiplied by decay per step */
+} recon_adam_params;
+
+/* sync.  The depth maps and points must stay valid and unchanged while the
+ * batch lives (their depth bounds are computed here, once). */
+RECON_API int recon_diff_batch_create(const recon_diff_volume* volumes, int32_t num_volumes,
+                                      recon_diff_batch** out);
+RECON_API void recon_diff_batch_destroy(recon_diff_batch* b);
+RECON_API int recon_diff_batch_volume_data(recon_diff_batch* b, int32_t i, float** distances,
+                                           float** weights, recon_tsdf_info* info);
+/* sync.  Copies volume i's D and/or W (either may be NULL) to host memory. */
+RECON_API int recon_diff_batch_download(recon_diff_batch* b, int32_t i, float* host_d,
+                                        float* host_w, recon_stream_t stream);
+/* async.  Resets every volume and integrates all of its frames seen from
+ * poses_rt (device, 12 floats per frame, volumes in order); loss (device,
+ * one double per volume) <- each volume's surface loss. */
+RECON_API int recon_diff_batch_forward(recon_diff_batch* b, const float* poses_rt, double* loss,
+                                       recon_stream_t stream);
+/* async.  grad_pose (device, 6 doubles per frame) <- d(loss_v)/dxi of each
+ * frame's volume for the last recon_diff_batch_forward (same poses), xi the
+ * left perturbation of recon_tsdf_diff_adjoint.  Run-to-run identical. */
+RECON_API int recon_diff_batch_backward(recon_diff_batch* b, double* grad_pose,
+                                        recon_stream_t stream);
+/* sync.  `steps` steps of forward, backward and Adam on every frame's xi,
+ * T <- Exp(-step) T with R re-orthonormalised, poses_rt updated in place;
+ * one graph launch per step.  loss_history (device, steps * num_volumes
+ * doubles, may be NULL) <- each step's losses before its update. */
+RECON_API int recon_diff_batch_refine(recon_diff_batch* b, float* poses_rt, int32_t steps,
+                                      const recon_adam_params* opt, double* loss_history,
+                                      recon_stream_t stream);
+
 /* ---- batched volumes ----------------------------------------------------- */
 /* No counterpart in the reference, which builds one TSDFVolume per scene and
  * blocks on every frame (tsdf_surface_reconstruction.cpp:125-138).  A batch
