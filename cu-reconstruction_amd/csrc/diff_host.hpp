@@ -11,7 +11,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Scratch of the pair merge for up to n pairs (byte offsets from its base).
 struct MergeLayout {
-    size_t head, offsets, sums, uid, lam, acc, maxbits, total;
+    size_t head, offsets, sums, uid, lam, acc, maxbits, block_max, total;
     int nb;
 };
 MergeLayout merge_layout(int64_t n);

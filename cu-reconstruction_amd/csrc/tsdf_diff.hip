@@ -331,23 +331,41 @@ __device__ __forceinline__ double merge_scale(unsigned maxbits, int lg) {
 __global__ __launch_bounds__(256) void merge_mark_kernel(const int64_t* __restrict__ ids,
                                                          const float* __restrict__ lam, int64_t n,
                                                          int64_t N, uint32_t* __restrict__ owner,
-                                                         unsigned* __restrict__ maxbits) {
+                                                         unsigned* __restrict__ block_max) {
+    __shared__ unsigned s_max[4];
     const int64_t e = blockIdx.x * 256ll + threadIdx.x;
     unsigned m = 0;
     if (e < n) {
         const int64_t v = ids[e];
         const float l = lam[e];
         if (pair_valid(v, l, N)) {
-            // the plain read brings the line into L2 (atomics that miss it
-            // wait on memory) and skips pairs an earlier one already beat
+            // the plain read skips pairs an earlier one already beat
             if ((uint32_t)e < owner[v]) atomicMin(owner + v, (uint32_t)e);
             m = __float_as_uint(fabsf(l));  // orders like the magnitudes
         }
     }
-    // one atomic per wave for the largest magnitude
+    // the block's largest magnitude into a slot of its own (atomics on one
+    // address from every wave would serialise)
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, d, 64));
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(maxbits, m);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) block_max[blockIdx.x] = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
+// One workgroup: the largest of the block maxima.
+__global__ __launch_bounds__(256) void merge_max_kernel(const unsigned* __restrict__ block_max,
+                                                        int nblocks, unsigned* __restrict__ maxbits) {
+    __shared__ unsigned s[256];
+    unsigned m = 0;
+    for (int i = threadIdx.x; i < nblocks; i += 256) m = max(m, block_max[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *maxbits = s[0];
 }
 
 __global__ __launch_bounds__(256) void merge_flag_kernel(const int64_t* __restrict__ ids,
@@ -501,7 +519,8 @@ MergeLayout merge_layout(int64_t n) {
     M.lam = M.uid + align256(4 * un);
     M.acc = M.lam + align256(4 * un);
     M.maxbits = M.acc + align256(8 * un);
-    M.total = M.maxbits + 256;
+    M.block_max = M.maxbits + 256;
+    M.total = M.block_max + align256(4 * ((un + 255) / 256));
     return M;
 }
 
@@ -515,10 +534,12 @@ int launch_merge(const int64_t* ids, const float* lam, int64_t n, int64_t N, uin
     float* lam_u = reinterpret_cast<float*>(mb + M.lam);
     long long* acc = reinterpret_cast<long long*>(mb + M.acc);
     unsigned* maxbits = reinterpret_cast<unsigned*>(mb + M.maxbits);
-    const dim3 pg((unsigned)((n + 255) / 256));
+    unsigned* block_max = reinterpret_cast<unsigned*>(mb + M.block_max);
+    const int nblocks = (int)((n + 255) / 256);
+    const dim3 pg((unsigned)nblocks);
     const int lg = 64 - __builtin_clzll((unsigned long long)n);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, s, maxbits, 1, 0u);
-    hipLaunchKernelGGL(merge_mark_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, maxbits);
+    hipLaunchKernelGGL(merge_mark_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, block_max);
+    hipLaunchKernelGGL(merge_max_kernel, dim3(1), dim3(256), 0, s, block_max, nblocks, maxbits);
     hipLaunchKernelGGL(merge_flag_kernel, pg, dim3(256), 0, s, ids, lam, n, N, owner, head);
     RECON_HIP_TRY(hipGetLastError(), "merge kernels launch");
     int rc = launch_exclusive_scan(head, (int)n, offsets, sums, s);

@@ -566,3 +566,21 @@ def observed_vertices(w: Workload, W: np.ndarray, verts: np.ndarray) -> np.ndarr
             for dx in (0, 1):
                 ok &= W[lo[:, 2] + dz, lo[:, 1] + dy, lo[:, 0] + dx] > 0
     return ok
+
+
+def morton_order(w: Workload, points: np.ndarray) -> np.ndarray:
+    """Permutation putting points in 3-D Morton (Z-curve) order of their voxel
+    cell: consecutive points then share support voxels, which keeps the
+    loss's (voxel, value) pairs, their merge and the adjoint's accesses
+    local."""
+    q = np.floor((np.asarray(points, np.float64) - np.asarray(w.offset)) / w.voxel_size)
+    c = np.clip(q, 0, 1023).astype(np.uint64)
+
+    def spread(x):
+        x = (x | (x << np.uint64(16))) & np.uint64(0x030000FF)
+        x = (x | (x << np.uint64(8))) & np.uint64(0x0300F00F)
+        x = (x | (x << np.uint64(4))) & np.uint64(0x030C30C3)
+        x = (x | (x << np.uint64(2))) & np.uint64(0x09249249)
+        return x
+    key = spread(c[:, 0]) | (spread(c[:, 1]) << np.uint64(1)) | (spread(c[:, 2]) << np.uint64(2))
+    return np.argsort(key, kind="stable")
