@@ -39,6 +39,8 @@ HIP_FLAGS = [
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
 
+# host-side pieces of the drop-in beside tsdf_volume.cpp (librecon_host.so)
+HOST_SOURCES = ["config.cpp", "png_reader.cpp", "tsdf_surface.cpp", "driver_capi.cpp"]
 KERNEL_SOURCES = ["recon_common.hip", "tsdf_volume.hip", "marching_cubes.hip", "depth_points.hip",
                   "tsdf_batch.hip", "tsdf_diff.hip", "diff_batch.hip"]
 
@@ -85,9 +87,17 @@ def build(verbose: bool = False, jobs: int = 8) -> dict[str, Path]:
     host_src = PKG / "host" / "tsdf_volume.cpp"
     if host_src.exists():
         host_so = LIB / "librecon_host.so"
-        _run([HIPCC, *HIP_FLAGS, "-shared", host_src, f"-L{LIB}", "-lrecon",
+        host_srcs = [host_src] + [PKG / "host" / f for f in HOST_SOURCES
+                                  if (PKG / "host" / f).exists()]
+        _run([HIPCC, *HIP_FLAGS, "-shared", *host_srcs, f"-L{LIB}", "-lrecon", "-lz",
               "-Wl,-rpath,$ORIGIN", "-o", host_so], verbose)
         out["host"] = host_so
+        main_src = PKG / "host" / "tsdf_surface_main.cpp"
+        if main_src.exists():
+            exe = LIB / "tsdf_surface"
+            _run([HIPCC, *HIP_FLAGS, main_src, f"-L{LIB}", "-lrecon_host", "-lrecon",
+                  "-Wl,-rpath,$ORIGIN", "-o", exe], verbose)
+            out["tsdf_surface"] = exe
         test_src = ROOT / "tests" / "cpp" / "tsdf_wrapper_test.cpp"
         if test_src.exists():
             exe = LIB / "tsdf_wrapper_test"
