@@ -81,6 +81,26 @@ bool TSDFSurface::reconstruct(const PtrStepSz<float>& depth, const unsigned char
     return true;
 }
 
+bool TSDFSurface::reconstruct_surface(PointCloud& cloud) {
+    const int rows = (int)cfg_.camera_height, cols = (int)cfg_.camera_width;
+    if (rows <= 0 || cols <= 0) {
+        std::fprintf(stderr, "[cuphoto] TSDFSurface::reconstruct_surface() -- camera size is zero\n");
+        return false;
+    }
+    float* depth = nullptr;
+    unsigned char* color = nullptr;
+    const size_t n = (size_t)rows * cols;
+    bool ok = hipMalloc((void**)&depth, 4 * n) == hipSuccess &&
+              hipMalloc((void**)&color, 3 * n) == hipSuccess;
+    if (!ok) std::fprintf(stderr, "[cuphoto] TSDFSurface::reconstruct_surface() -- out of device memory\n");
+    PtrStepSz<float> d{depth, 4 * (size_t)cols, rows, cols};
+    PtrStepSz<unsigned char> c{color, 3 * (size_t)cols, rows, cols};
+    ok = ok && cloud.project_to_color_depth(c, d, global_pose) && reconstruct(d, color, 3 * (size_t)cols);
+    if (depth) (void)hipFree(depth);
+    if (color) (void)hipFree(color);
+    return ok;
+}
+
 bool Mesh::store_to_ply(const std::string& ply_filepath) const {
     return write_mesh_to_ply(ply_filepath, vertices, polygons);
 }

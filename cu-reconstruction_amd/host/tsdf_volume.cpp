@@ -166,6 +166,37 @@ PointCloud::~PointCloud() {
     if (xyz_) (void)hipFree(xyz_);
     if (valid_) (void)hipFree(valid_);
     if (rgb_) (void)hipFree(rgb_);
+    if (zbuffer_) (void)hipFree(zbuffer_);
+}
+
+bool PointCloud::render_raw(float* depth, size_t step, int rows, int cols, uint8_t* color,
+                            size_t color_step, const std::array<r64, 7>& pose, const char* who) {
+    if (!depth) {  // cuda_point_cloud.cu:231-234
+        std::fprintf(stderr, "[cuphoto] %s -- depth map is null pointer\n", who);
+        return false;
+    }
+    if (rows <= 0 || cols <= 0) {  // :239-242
+        std::fprintf(stderr, "[cuphoto] %s -- depth width/height parameters are zero\n", who);
+        return false;
+    }
+    const size_t words = (size_t)rows * (size_t)cols;
+    if (words > zbuffer_words_) {
+        if (zbuffer_) (void)hipFree(zbuffer_);
+        zbuffer_ = nullptr;
+        zbuffer_words_ = 0;
+        if (hipMalloc((void**)&zbuffer_, 8 * words) != hipSuccess) {
+            std::fprintf(stderr, "[cuphoto] %s -- can't allocate the z-buffer\n", who);
+            return false;
+        }
+        zbuffer_words_ = words;
+    }
+    if (recon_points_render(xyz_, valid_, rgb_, (int64_t)num_, pose.data(), K_[0], K_[4], K_[6],
+                            K_[7], depth, step, rows, cols, color, color_step, zbuffer_,
+                            stream_) != RECON_OK) {
+        log_error(who);
+        return false;
+    }
+    return sync(stream_, who);
 }
 
 bool PointCloud::extract_raw(const float* depth, size_t step, int rows, int cols,

@@ -304,6 +304,45 @@ def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, s
     return (xyz, valid, colors) if with_colors else (xyz, valid)
 
 
+def render_points(xyz, pose, intrinsics, image, valid=None, colors=None, with_colors=False,
+                  depth_out=None, color_out=None, zbuffer=None, stream=None):
+    """cudaPointCloud::project_to_depth / project_to_color_depth
+    (cuda_point_cloud.cu:229-353): z-buffer render of world points xyz (N, 3)
+    float32 at the world->camera `pose` into a (rows, cols) depth map and,
+    with with_colors, a (rows, cols, 3) uint8 image.  `image` is (cols, rows).
+    depth_out / color_out may be row-pitched views (unit inner stride)."""
+    torch = _torch()
+    cols, rows = int(image[0]), int(image[1])
+    dev = xyz.device
+    if xyz.dtype != torch.float32 or xyz.dim() != 2 or xyz.shape[1] != 3 or not xyz.is_contiguous():
+        raise ValueError("xyz must be a contiguous (N, 3) float32 tensor")
+    n = xyz.shape[0]
+    for name, t, shape in (("valid", valid, (n,)), ("colors", colors, (n, 3))):
+        if t is not None and (t.dtype != torch.uint8 or tuple(t.shape) != shape or not t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous uint8 tensor of shape {shape}")
+    if depth_out is None:
+        depth_out = torch.empty((rows, cols), dtype=torch.float32, device=dev)
+    if with_colors and color_out is None:
+        color_out = torch.empty((rows, cols, 3), dtype=torch.uint8, device=dev)
+    if depth_out.shape != (rows, cols) or depth_out.stride(1) != 1 or depth_out.dtype != torch.float32:
+        raise ValueError("depth_out must be (rows, cols) float32 with unit column stride")
+    if color_out is not None and (color_out.shape != (rows, cols, 3) or color_out.stride(2) != 1
+                                  or color_out.stride(1) != 3 or color_out.dtype != torch.uint8):
+        raise ValueError("color_out must be (rows, cols, 3) uint8 with packed pixels")
+    if zbuffer is None:
+        zbuffer = torch.empty((rows * cols,), dtype=torch.int64, device=dev)
+    elif zbuffer.dtype != torch.int64 or zbuffer.numel() < rows * cols:
+        raise ValueError("zbuffer must be an int64 tensor of at least rows*cols elements")
+    fx, fy, cx, cy = [float(v) for v in intrinsics]
+    check(ffi.load().recon_points_render(
+        xyz.data_ptr() if n else None, valid.data_ptr() if valid is not None else None,
+        colors.data_ptr() if colors is not None else None, n, ffi.pose_array(pose), fx, fy, cx, cy,
+        depth_out.data_ptr(), depth_out.stride(0) * 4, rows, cols,
+        color_out.data_ptr() if color_out is not None else None,
+        color_out.stride(0) if color_out is not None else 0, zbuffer.data_ptr(), _stream(stream)))
+    return (depth_out, color_out) if with_colors or color_out is not None else depth_out
+
+
 class Batch:
     """Many volumes integrated and meshed by one graph launch
     (recon_batch_*).  `volumes` is a list of (config, depth tensor (F, H, W)

@@ -113,6 +113,9 @@ SIGNATURES = {
     "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
     "recon_depth_to_points": (C.c_int, [C.POINTER(DepthView), C.c_double, C.c_double, C.c_double,
                                         C.c_double, _d7, _vp, C.c_size_t, _vp, _vp, _vp, _vp]),
+    "recon_points_render": (C.c_int, [_vp, _vp, _vp, _i64, _d7, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, _vp, C.c_size_t, _i32, _i32, _vp,
+                                      C.c_size_t, _vp, _vp]),
     "recon_batch_create": (C.c_int, [C.POINTER(BatchVolume), _i32, _i64, C.c_char_p,
                                      C.POINTER(_vp)]),
     "recon_batch_destroy": (None, [_vp]),

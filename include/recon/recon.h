@@ -185,6 +185,28 @@ RECON_API int recon_depth_to_points(const recon_depth_view* depth, double fx, do
                           float* xyz, uint8_t* valid, uint8_t* colors_out,
                           recon_stream_t stream);
 
+/* ---- point cloud -> depth / colour z-buffer render ------------------------ */
+/* async.  Replaces cudaPointCloud::project_to_depth / project_to_color_depth
+ * with filter_image_if (cuda_point_cloud.cu:229-353, 89-99), which splat on
+ * the host.  Point i (xyz[3i..3i+2], world) is moved to the camera with
+ * camera_pose = {qw,qx,qy,qz,tx,ty,tz} (world->camera, float32 as integrate
+ * narrows it) and lands on pixel (trunc((X*fx)/Z + cx), trunc((Y*fy)/Z + cy))
+ * in float32 with the intrinsics narrowed to float; points with Z == 0, with
+ * valid[i] == 0 (valid may be NULL) or off the image are skipped.  A pixel
+ * keeps the smallest Z, the lowest index among equal Z, and that point's
+ * colours[3i..3i+2] (255,255,255 when colours is NULL); other pixels get depth
+ * 0 and colour 0.  Results do not depend on the order of execution.  depth is
+ * rows x depth_step bytes of float; color_out (may be NULL) rows x color_step
+ * bytes of rgb triples; zbuffer is device scratch of rows*cols 8-byte words.
+ * The reference reads cx, cy from K[2], K[5], 0 in the column-major K it
+ * fills; here they are explicit.  num_points < 2^32. */
+RECON_API int recon_points_render(const float* xyz, const uint8_t* valid, const uint8_t* colors,
+                                  int64_t num_points, const double camera_pose[7], double fx,
+                                  double fy, double cx, double cy, float* depth,
+                                  size_t depth_step, int32_t rows, int32_t cols,
+                                  uint8_t* color_out, size_t color_step,
+                                  unsigned long long* zbuffer, recon_stream_t stream);
+
 /* ---- differentiable integrate -------------------------------------------- */
 /* No counterpart in the reference.  An extension of cu_tsdf_integrate
  * (tsdf_volume.cu:144-190) whose depth lookup is bilinear, so that the

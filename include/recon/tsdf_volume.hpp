@@ -21,6 +21,7 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <string>
 #include <vector>
@@ -131,6 +132,30 @@ public:
                            depth.cols, reinterpret_cast<const uint8_t*>(colors.data), colors.step,
                            transform, frame_idx);
     }
+    // cuda_point_cloud.cu:229-353 with filter_image_if (:89-99): z-buffer
+    // render of the first get_num_points() points (invalid ones skipped) at
+    // camera_pose (world->camera) into a pitched float depth map and, for
+    // project_to_color_depth, a pitched rgb image.  A pixel keeps the nearest
+    // point (the first of equal depths) and its colour; empty pixels get 0.
+    // Runs on the device; blocks until done, as the reference does.  cx, cy
+    // come from K[6], K[7]: the reference reads K[2], K[5], which are 0 in the
+    // column-major K it fills (:248-251).
+    template <typename Depth>
+    bool project_to_depth(Depth& depth, const std::array<r64, 7>& camera_pose) {
+        return render_raw(reinterpret_cast<float*>(depth.data), depth.step, depth.rows, depth.cols,
+                          nullptr, 0, camera_pose, "PointCloud::project_to_depth()");
+    }
+    template <typename Color, typename Depth>
+    bool project_to_color_depth(Color& color, Depth& depth, const std::array<r64, 7>& camera_pose) {
+        if (!color.data) {
+            std::fprintf(stderr, "[cuphoto] PointCloud::project_to_color_depth() -- color map is "
+                                 "null pointer\n");
+            return false;
+        }
+        return render_raw(reinterpret_cast<float*>(depth.data), depth.step, depth.rows, depth.cols,
+                          reinterpret_cast<uint8_t*>(color.data), color.step, camera_pose,
+                          "PointCloud::project_to_color_depth()");
+    }
     float* xyz() const { return xyz_; }
     uint8_t* valid() const { return valid_; }
     uint8_t* colors() const { return rgb_; }
@@ -143,7 +168,11 @@ public:
 private:
     bool extract_raw(const float* depth, size_t step, int rows, int cols, const uint8_t* rgb,
                      size_t rgb_step, const std::array<r64, 7>& T, int frame_idx);
+    bool render_raw(float* depth, size_t step, int rows, int cols, uint8_t* color,
+                    size_t color_step, const std::array<r64, 7>& pose, const char* who);
     std::array<r64, 9> K_;
+    unsigned long long* zbuffer_ = nullptr;  // grows to rows*cols words
+    size_t zbuffer_words_ = 0;
     size_t total_ = 0, num_ = 0;
     float* xyz_ = nullptr;
     uint8_t* valid_ = nullptr;

@@ -21,7 +21,10 @@
  *   - marching cubes uses the layer stride gx*gy (marching_cubes.cu:211
  *     uses gx+gy, correct only for 2x2xN grids);
  *   - the trilinear sampler weights neighbour (1,1,1) by xyz and (0,1,1) by
- *     (1-x)yz (tsdf_volume.cu:100-101 swap the two).
+ *     (1-x)yz (tsdf_volume.cu:100-101 swap the two);
+ *   - the z-buffer render takes fx, fy, cx, cy explicitly
+ *     (cuda_point_cloud.cu:248-251 read cx, cy from K[2], K[5], which are 0
+ *     in the column-major K the point extraction reads).
  */
 #ifndef RECON_ORACLE_H
 #define RECON_ORACLE_H
@@ -100,6 +103,19 @@ void oracle_depth_to_points(const float* depth, int32_t rows, int32_t cols, int6
                             const double transform[7],
                             const uint8_t* rgb, int64_t rgb_pitch,
                             float* xyz, uint8_t* valid, uint8_t* colors_out);
+
+/* project_to_depth / project_to_color_depth + filter_image_if
+ * (cuda_point_cloud.cu:229-353, 89-99): points in index order, world to
+ * camera with the float32 pose matrix, pixel = trunc((X*fx)/Z + cx) in
+ * float32, Z == 0 skipped, a pixel keeps the smallest Z (the first of equal
+ * ones) and that point's colour (255,255,255 when colors is NULL); unwritten
+ * pixels get depth 0 and colour 0.  valid (may be NULL) skips points whose
+ * byte is 0.  depth is rows x pitch floats; color_out (may be NULL) is rows x
+ * color_pitch bytes of rgb triples. */
+void oracle_render_points(const float* xyz, const uint8_t* valid, const uint8_t* colors,
+                          int64_t num_points, const double pose[7], float fx, float fy, float cx,
+                          float cy, int32_t rows, int32_t cols, int64_t pitch, float* depth,
+                          uint8_t* color_out, int64_t color_pitch);
 
 /* point_cloud_from_depth (point_cloud_utility.cpp:76-115): all-float64 path,
  * keeps min < d < max, transform applied in float64 (the caller's Sophus SE3

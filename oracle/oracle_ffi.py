@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
             "oracle_diff_adjoint_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp]),
             "oracle_stable_points": (None, [vp, vp, i32, i32, vp, i32, vp, i64, C.c_double,
                                             C.c_double, vp]),
+            "oracle_render_points": (None, [vp, vp, vp, i64, vp, C.c_float, C.c_float, C.c_float,
+                                            C.c_float, i32, i32, i64, vp, vp, i64]),
         })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -176,6 +178,22 @@ def depth_to_points(depth, intrinsics, transform, rgb=None):
                                  _p(pose_arr(transform)), _p(rgbp), cols * 3, _p(xyz), _p(valid),
                                  _p(colors))
     return xyz, valid, colors
+
+
+def render_points(xyz, pose, intrinsics, image, valid=None, colors=None, want_color=True):
+    """Z-buffer render of world points (N, 3) float32 into (rows, cols) depth
+    and (rows, cols, 3) colour (None when want_color is False)."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    n = xyz.shape[0]
+    cols, rows = image
+    fx, fy, cx, cy = [float(np.float32(v)) for v in intrinsics]
+    depth = np.empty((rows, cols), np.float32)
+    color = np.empty((rows, cols, 3), np.uint8) if want_color else None
+    valid = np.ascontiguousarray(valid, np.uint8) if valid is not None else None
+    colors = np.ascontiguousarray(colors, np.uint8) if colors is not None else None
+    lib().oracle_render_points(_p(xyz), _p(valid), _p(colors), n, _p(pose_arr(pose)), fx, fy, cx,
+                               cy, rows, cols, cols, _p(depth), _p(color), cols * 3)
+    return depth, color
 
 
 def point_cloud_from_depth_f64(depth, intrinsics, transform, dmin=0.0, dmax=1e30):

@@ -7,9 +7,16 @@
  *       (cuda_point_cloud.cu:44-47, passed here as fx, fy, cx, cy).
  *   oracle_point_cloud_from_depth_f64: point_cloud_from_depth
  *       (modules/src/point_cloud_utility.cpp:76-115).
+ *   oracle_render_points: the host z-buffer of
+ *       cudaPointCloud::project_to_depth / project_to_color_depth
+ *       (cuda_point_cloud.cu:229-353) followed by filter_image_if
+ *       (cuda_point_cloud.cu:89-99), with explicit intrinsics in place of
+ *       K[2], K[5] (cuda_point_cloud.cu:248-251, which read 0 from the
+ *       column-major K the kernel above uses).
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 
 void oracle_depth_to_points(const float* depth, int32_t rows, int32_t cols, int64_t pitch,
@@ -75,4 +82,49 @@ int64_t oracle_point_cloud_from_depth_f64(const float* depth, int32_t rows, int3
             ++count;
         }
     return count;
+}
+
+void oracle_render_points(const float* xyz, const uint8_t* valid, const uint8_t* colors,
+                          int64_t num_points, const double pose[7], float fx, float fy, float cx,
+                          float cy, int32_t rows, int32_t cols, int64_t pitch, float* depth,
+                          uint8_t* color_out, int64_t color_pitch) {
+    float R[9], t[3];
+    oracle_pose_matrix_f32(pose, R, t); /* SE3<r32>(quat, trans), :246 */
+    /* cpu_depth.setTo(FLT_MAX); cpu_color.setTo(0) (:254-255, :318-319) */
+    for (int32_t y = 0; y < rows; ++y)
+        for (int32_t x = 0; x < cols; ++x) {
+            depth[(int64_t)y * pitch + x] = FLT_MAX;
+            if (color_out)
+                for (int k = 0; k < 3; ++k) color_out[(int64_t)y * color_pitch + 3 * x + k] = 0;
+        }
+    for (int64_t i = 0; i < num_points; ++i) {
+        if (valid && !valid[i]) continue; /* points without a depth sample */
+        const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+        /* transform * v->pos, world to camera (se3.cuh:82-100) */
+        const float X = ((R[0] * px + R[1] * py) + R[2] * pz) + t[0];
+        const float Y = ((R[3] * px + R[4] * py) + R[5] * pz) + t[1];
+        const float Z = ((R[6] * px + R[7] * py) + R[8] * pz) + t[2];
+        if (Z == 0.0f) continue; /* :259-260 */
+        /* static_cast<i32>((x * fx) / z + cx) (:262-263): truncation toward
+         * zero; a value outside int32 or NaN is treated as off the image
+         * (x86's cvttss2si gives INT_MIN there, which the bounds test drops) */
+        const float u = (X * fx) / Z + cx;
+        const float v = (Y * fy) / Z + cy;
+        if (!(u > -1.0f && u < 2147483648.0f) || !(v > -1.0f && v < 2147483648.0f)) continue;
+        const int32_t ix = (int32_t)u, iy = (int32_t)v;
+        if (ix < 0 || ix >= cols || iy < 0 || iy >= rows) continue; /* :264-265 */
+        float* d = &depth[(int64_t)iy * pitch + ix];
+        if (*d > Z) { /* strict: the first of equal depths stays (:266-269) */
+            *d = Z;
+            if (color_out)
+                for (int k = 0; k < 3; ++k)
+                    color_out[(int64_t)iy * color_pitch + 3 * ix + k] =
+                        colors ? colors[3 * i + k] : 255;
+        }
+    }
+    /* filter_image_if(depth, FLT_MAX, 0): |d - FLT_MAX| < EPSILON only holds
+     * for FLT_MAX itself (EPSILON = 1e-10, util.cu:6-10; :89-99, :275-278) */
+    for (int32_t y = 0; y < rows; ++y)
+        for (int32_t x = 0; x < cols; ++x)
+            if (depth[(int64_t)y * pitch + x] == FLT_MAX) depth[(int64_t)y * pitch + x] = 0.0f;
 }

@@ -1,8 +1,9 @@
 // tsdf_wrapper_test.cpp -- drives the C++ drop-in (include/recon/tsdf_volume.hpp)
 // the way the reference's TSDFSurface::reconstruct_surface does
 // (tsdf_surface_reconstruction.cpp:122-153): create a volume, integrate
-// frames, extract the mesh, back-project a frame, write the PLY; every result
-// is compared bit-for-bit with the CPU oracle (test infrastructure).
+// frames, extract the mesh, back-project frames, render the point cloud,
+// write the PLY; every result is compared bit-for-bit with the CPU oracle
+// (test infrastructure).
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "oracle.h"
+#include "recon/tsdf_surface.hpp"
 #include "recon/tsdf_volume.hpp"
 #include "scene.h"
 
@@ -171,6 +173,106 @@ int main() {
     CHECK(std::memcmp(xyz.data(), oxyz.data(), xyz.size() * 4) == 0, "points differ from the oracle");
     CHECK(valid == ovalid, "validity differs");
     CHECK(!pc.extract_points(d2, no_color, poses[2], 2), "frame 2 does not fit and must fail");
+
+    // z-buffer render (project_to_color_depth / project_to_depth) of a
+    // two-frame cloud: slot 0 gets frame 1 in world coordinates
+    {
+        const std::array<double, 7>& P = poses[1];
+        const double w = P[0], x = P[1], y = P[2], z = P[3];
+        const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                             2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                             2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
+        std::array<double, 7> inv{w, -x, -y, -z, 0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            inv[4 + i] = -(R[i] * P[4] + R[3 + i] * P[5] + R[6 + i] * P[6]);  // -R^T t
+        cuphoto::PtrStepSzf d1{ddepth + (size_t)rows * cols, (size_t)cols * 4, rows, cols};
+        CHECK(pc.extract_points(d1, no_color, inv, 0), "extract_points slot 0");
+    }
+    const size_t np = pc.get_num_points(), n1 = (size_t)rows * cols;
+    CHECK(np == 2 * n1, "points %zu", np);
+    std::vector<float> all_xyz(3 * np);
+    std::vector<uint8_t> all_valid(np), all_rgb(3 * np);
+    CHECK(hipMemcpy(all_xyz.data(), pc.xyz(), 12 * np, hipMemcpyDeviceToHost) == hipSuccess, "xyz");
+    CHECK(hipMemcpy(all_valid.data(), pc.valid(), np, hipMemcpyDeviceToHost) == hipSuccess, "valid");
+    CHECK(hipMemcpy(all_rgb.data(), pc.colors(), 3 * np, hipMemcpyDeviceToHost) == hipSuccess, "rgb");
+    const size_t dstep = 4 * ((size_t)cols + 7), cstep = 3 * (size_t)cols + 5;
+    float* rdepth = nullptr;
+    uint8_t* rcolor = nullptr;
+    CHECK(hipMalloc((void**)&rdepth, dstep * rows) == hipSuccess, "hipMalloc");
+    CHECK(hipMalloc((void**)&rcolor, cstep * rows) == hipSuccess, "hipMalloc");
+    cuphoto::PtrStepSzf rd{rdepth, dstep, rows, cols};
+    cuphoto::PtrStepSzb rc{rcolor, cstep, rows, cols};
+    CHECK(pc.project_to_color_depth(rc, rd, poses[3]), "project_to_color_depth");
+    std::vector<float> gdepth(dstep / 4 * rows), odepth(dstep / 4 * rows);
+    std::vector<uint8_t> gcolor(cstep * rows), ocolor(cstep * rows);
+    CHECK(hipMemcpy(gdepth.data(), rdepth, dstep * rows, hipMemcpyDeviceToHost) == hipSuccess, "dl");
+    CHECK(hipMemcpy(gcolor.data(), rcolor, cstep * rows, hipMemcpyDeviceToHost) == hipSuccess, "dl");
+    oracle_render_points(all_xyz.data(), all_valid.data(), all_rgb.data(), (int64_t)np,
+                         poses[3].data(), (float)fx, (float)fy, (float)ccx, (float)ccy, rows, cols,
+                         (int64_t)(dstep / 4), odepth.data(), ocolor.data(), (int64_t)cstep);
+    size_t hits = 0;
+    bool same = true;
+    for (int r = 0; r < rows; ++r) {
+        same &= std::memcmp(&gdepth[r * (dstep / 4)], &odepth[r * (dstep / 4)], 4 * (size_t)cols) == 0;
+        same &= std::memcmp(&gcolor[r * cstep], &ocolor[r * cstep], 3 * (size_t)cols) == 0;
+        for (int c = 0; c < cols; ++c) hits += odepth[r * (dstep / 4) + c] > 0.f;
+    }
+    CHECK(same, "render differs from the oracle");
+    CHECK(hits > n1 / 4, "render hits %zu", hits);
+    CHECK(pc.project_to_depth(rd, poses[3]), "project_to_depth");
+    CHECK(hipMemcpy(gdepth.data(), rdepth, dstep * rows, hipMemcpyDeviceToHost) == hipSuccess, "dl");
+    same = true;
+    for (int r = 0; r < rows; ++r)
+        same &= std::memcmp(&gdepth[r * (dstep / 4)], &odepth[r * (dstep / 4)], 4 * (size_t)cols) == 0;
+    CHECK(same, "project_to_depth differs from the oracle");
+    cuphoto::PtrStepSzf null_rd{nullptr, dstep, rows, cols};
+    CHECK(!pc.project_to_depth(null_rd, poses[3]), "null depth must fail");
+    (void)hipFree(rdepth);
+    (void)hipFree(rcolor);
+
+    // TSDFSurface::reconstruct_surface(cloud): render at the global pose,
+    // integrate, marching cubes (tsdf_surface_reconstruction.cpp:94-120,
+    // whose integrate and meshing are commented out)
+    {
+        char text[1024];
+        std::snprintf(text, sizeof text,
+                      "%%YAML:1.0\n"
+                      "tsdf.volume.voxel_grid_size.x: 60\ntsdf.volume.voxel_grid_size.y: 52\n"
+                      "tsdf.volume.voxel_grid_size.z: 44\ntsdf.volume.physical_size.x: 0.9\n"
+                      "tsdf.volume.physical_size.y: 0.78\ntsdf.volume.physical_size.z: 0.66\n"
+                      "tsdf.volume.global_offset.x: -0.45\ntsdf.volume.global_offset.y: -0.39\n"
+                      "tsdf.volume.global_offset.z: -0.33\ntsdf.volume.max_weight: 3\n"
+                      "tsdf.volume.truncation: 0.04\n"
+                      "camera.fx: %.17g\ncamera.fy: %.17g\ncamera.cx: %.17g\ncamera.cy: %.17g\n"
+                      "camera.width: %d\ncamera.height: %d\n",
+                      fx, fy, ccx, ccy, cols, rows);
+        cuphoto::Config scfg;
+        CHECK(scfg.parse(text), "parse");
+        cuphoto::TSDFSurface surface(scfg);
+        surface.set_global_transformation(poses[3]);
+        CHECK(surface.reconstruct_surface(pc), "reconstruct_surface(cloud)");
+        const cuphoto::Mesh& mesh = surface.get_mesh();
+        // oracle: the contiguous render, one integrate, marching cubes
+        std::vector<float> cdepth(n1);
+        std::vector<uint8_t> ccolor(3 * n1);
+        oracle_render_points(all_xyz.data(), all_valid.data(), all_rgb.data(), (int64_t)np,
+                             poses[3].data(), (float)fx, (float)fy, (float)ccx, (float)ccy, rows,
+                             cols, cols, cdepth.data(), ccolor.data(), 3 * (int64_t)cols);
+        std::vector<float> Ds(nvox), Ws(nvox);
+        oracle_tsdf_init(&ov, Ds.data(), Ws.data());
+        oracle_tsdf_integrate_f32(&ov, Ds.data(), Ws.data(), cdepth.data(), rows, cols, cols,
+                                  poses[3].data(), nullptr, 0);
+        const int64_t ns = oracle_mc_count(Ds.data(), ov.grid, nullptr, 0);
+        std::vector<float> svs(9 * (size_t)ns + 9);
+        std::vector<int32_t> sts(3 * (size_t)ns + 3);
+        oracle_mc_extract(Ds.data(), ov.grid, ov.voxel_size, ov.offset, svs.data(), sts.data(),
+                          nullptr, nullptr, ns, 0);
+        CHECK((int64_t)mesh.polygons.size() == ns && ns > 100, "cloud mesh %zu vs %lld triangles",
+              mesh.polygons.size(), (long long)ns);
+        CHECK(mesh.vertices.size() == 3 * (size_t)ns &&
+                  std::memcmp(mesh.vertices.data(), svs.data(), 36 * (size_t)ns) == 0,
+              "cloud mesh vertices differ from the oracle");
+    }
 
     (void)hipFree(ddepth);
     if (g_fail) {

@@ -183,3 +183,28 @@ def test_pose_to_matrix_matches_oracle_narrowing():
     bad[1, 2] = np.nan
     with pytest.raises(recon.ReconError):
         recon.pose_to_matrix(bad)
+
+
+def test_render_arguments_rejected_before_any_launch():
+    lib = ffi.load()
+    pose = ffi.pose_array([1, 0, 0, 0, 0, 0, 0])
+    p = C.c_void_p(4096)
+
+    def call(xyz=p, n=10, pose=pose, fx=500.0, cx=320.0, depth=p, step=2560, rows=480, cols=640,
+             color=None, cstep=0, zb=p):
+        return lib.recon_points_render(xyz, None, None, n, pose, fx, 500.0, cx, 240.0, depth, step,
+                                       rows, cols, color, cstep, zb, None)
+    assert call(xyz=None) == ffi.ERR_NULL
+    assert call(pose=None) == ffi.ERR_NULL
+    assert call(depth=None) == ffi.ERR_NULL
+    assert call(zb=None) == ffi.ERR_NULL
+    assert call(n=-1) == ffi.ERR_SIZE
+    assert call(n=1 << 32) == ffi.ERR_RANGE
+    for rows, cols, step in [(0, 640, 2560), (480, 0, 2560), (480, 640, 2556), (480, 640, 2562)]:
+        assert call(rows=rows, cols=cols, step=step) == ffi.ERR_SIZE, (rows, cols, step)
+    assert call(rows=1 << 18, cols=4, step=16) == ffi.ERR_RANGE
+    assert call(color=p, cstep=1919) == ffi.ERR_SIZE
+    for fx in (0.0, float("nan"), float("inf"), 1e39):
+        assert call(fx=fx) == ffi.ERR_CONFIG, fx
+    assert call(cx=float("nan")) == ffi.ERR_CONFIG
+    assert call(pose=ffi.pose_array([1, 0, 0, float("inf"), 0, 0, 0])) == ffi.ERR_CONFIG
