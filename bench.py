@@ -190,6 +190,86 @@ def bench_differentiable(steps: int) -> dict:
     return out
 
 
+def bench_render(w, depth, steps: int, frames: int = 30, cpu: bool = True) -> dict:
+    """Point-cloud z-buffer render (recon_points_render, the device version of
+    cudaPointCloud::project_to_color_depth): frames 0..frames-1 of the
+    workload back-projected on the device into one cloud (valid mask and
+    colours kept), rendered at the pose of the next frame into a depth map
+    and an rgb image.  HIP events on the launch stream bracket each call
+    (clear + splat + resolve).  The CPU leg runs the oracle's restatement of
+    the reference's host loop on the same cloud, one thread, as the
+    reference does."""
+    import numpy as np
+    import torch
+
+    import recon
+    from recon import scenes
+
+    def inverse(pose):
+        R, t = scenes.pose_matrix(pose)
+        return np.r_[pose[0], -np.asarray(pose[1:4]), -(R.T @ t)]
+
+    cols, rows = w.image
+    n1 = rows * cols
+    xyz = torch.empty((frames * n1, 3), dtype=torch.float32, device="cuda")
+    valid = torch.empty((frames * n1,), dtype=torch.uint8, device="cuda")
+    colors = torch.empty((frames * n1, 3), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(7)
+    for f in range(frames):
+        rgb = torch.from_numpy(rng.integers(0, 256, (rows, cols, 3), dtype=np.uint8)).cuda()
+        p, v, c = recon.depth_to_points(torch.from_numpy(depth[f]).cuda(), w.intrinsics,
+                                        inverse(w.poses[f]), rgb=rgb, with_colors=True)
+        xyz[f * n1:(f + 1) * n1] = p
+        valid[f * n1:(f + 1) * n1] = v
+        colors[f * n1:(f + 1) * n1] = c
+    pose = w.poses[frames]
+    stream = torch.cuda.current_stream()
+    dout = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
+    cout = torch.empty((rows, cols, 3), dtype=torch.uint8, device="cuda")
+    zb = torch.empty((n1,), dtype=torch.int64, device="cuda")
+
+    def call():
+        recon.render_points(xyz, pose, w.intrinsics, w.image, valid=valid, colors=colors,
+                            depth_out=dout, color_out=cout, zbuffer=zb, stream=stream)
+
+    call()
+    torch.cuda.synchronize()
+    reps = max(steps, 1) * 10
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for r in range(reps):
+        call()
+        ev[r + 1].record(stream)
+    torch.cuda.synchronize()
+    ms = np.array([ev[r].elapsed_time(ev[r + 1]) for r in range(reps)])
+    n = xyz.shape[0]
+    hits = int((dout > 0).sum().item())
+    # algorithmic bytes per call: 12 B xyz + 1 B valid per point; per pixel
+    # 8 B z-buffer clear, 8 B z-buffer read, 4 B depth, 3 B colour written;
+    # 3 B colour gathered per covered pixel
+    alg = 13 * n + 23 * n1 + 3 * hits
+    out = {"workload": f"{w.name} frames 0-{frames - 1} as points, rendered at frame {frames}",
+           "points": int(n), "valid_points": int(valid.sum().item()), "image": [cols, rows],
+           "covered_pixels": hits, "ms_per_render": round(float(ms.mean()), 4),
+           "gpoints_s": round(n / (ms.mean() / 1000.0) / 1e9, 2),
+           "algorithmic_bytes": int(alg),
+           "achieved_gb_s": round(alg / (ms.mean() / 1000.0) / 1e9, 1)}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_ffi as O
+        hx, hv, hc = xyz.cpu().numpy(), valid.cpu().numpy(), colors.cpu().numpy()
+        t0 = time.perf_counter()
+        od, oc = O.render_points(hx, pose, w.intrinsics, w.image, valid=hv, colors=hc)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(od.view(np.uint32), dout.cpu().numpy().view(np.uint32)) and
+                    np.array_equal(oc, cout.cpu().numpy()))
+        out["cpu_baseline"] = {"value": round(n / dt / 1e9, 4), "unit": "Gpoints/s", "cores": 1,
+                               "kind": "port", "ms": round(dt * 1000.0, 1),
+                               "sample": f"oracle host z-buffer loop over the same {n} points"}
+        out["identical_to_oracle"] = same
+    return out
+
+
 def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
                           frames: int = 10, points: int = 20000) -> dict:
     """Batched pose refinement: `fragments` fragments (128^3, 8 mm, 10 frames
@@ -302,12 +382,14 @@ def main() -> int:
                     help="skip the differentiable (forward + adjoint) measurement")
     ap.add_argument("--no-refine", action="store_true",
                     help="skip the batched pose refinement measurement")
-    ap.add_argument("--only", choices=["integrate", "copy", "differentiable", "refine"],
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the point-cloud z-buffer render measurement")
+    ap.add_argument("--only", choices=["integrate", "copy", "differentiable", "refine", "render"],
                     help="profiling aids: 'integrate' runs --steps passes of the per-frame "
                          "integrate only; 'copy' copies a 256 MiB buffer 20 times (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE); "
                          "'differentiable' runs the forward + adjoint leg only; 'refine' the "
-                         "batched pose refinement only")
+                         "batched pose refinement only; 'render' the point-cloud render only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the TSDF path here is single-GPU "
@@ -347,6 +429,9 @@ def main() -> int:
     vol = recon.TSDFVolume(w.config())
     stream = torch.cuda.current_stream()
     F = w.num_frames
+    if args.only == "render":
+        log(json.dumps(bench_render(w, depth, args.steps, cpu=False)))
+        return 0
     if args.only == "integrate":
         for s in range(args.steps):
             vol.reset(stream)
@@ -471,6 +556,10 @@ def main() -> int:
     vol.close()
     del dev
     torch.cuda.empty_cache()
+    if not args.no_render:
+        log("timing the point-cloud render")
+        result["render"] = bench_render(w, depth, args.steps, cpu=not args.no_cpu)
+        torch.cuda.empty_cache()
     if not args.no_batched:
         log("timing the batched volumes")
         result["batched"] = bench_batched(max(args.steps, 2), args.autotune)

@@ -57,31 +57,48 @@ __global__ __launch_bounds__(256) void render_clear_kernel(unsigned long long* _
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) zb[i] = kEmptyKey;
 }
 
-__global__ __launch_bounds__(256) void render_splat_kernel(const RenderParams p,
-                                                           unsigned long long* __restrict__ zb) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.num_points) return;
-    if (p.valid && !p.valid[i]) return;
-    const float px = p.xyz[3 * i + 0], py = p.xyz[3 * i + 1], pz = p.xyz[3 * i + 2];
+// Camera-space z and z-buffer slot of one point; false when the point does
+// not land on the image.
+__device__ __forceinline__ bool splat_target(const RenderParams& p, float px, float py, float pz,
+                                             float* z_out, int64_t* slot) {
     // world -> camera (se3.cuh:82-100)
     const float X = ((p.R[0] * px + p.R[1] * py) + p.R[2] * pz) + p.t[0];
     const float Y = ((p.R[3] * px + p.R[4] * py) + p.R[5] * pz) + p.t[1];
     const float Z = ((p.R[6] * px + p.R[7] * py) + p.R[8] * pz) + p.t[2];
     // z == 0 is skipped (cuda_point_cloud.cu:259-260); a z that is not below
     // FLT_MAX (FLT_MAX, +inf, NaN) never beats the FLT_MAX fill (:266-267)
-    if (Z == 0.0f || !(Z < FLT_MAX)) return;
+    if (Z == 0.0f || !(Z < FLT_MAX)) return false;
     // static_cast<i32>((x * fx) / z + cx) truncates toward zero, so u in
     // (-1, cols) lands on columns 0..cols-1 (:262-265)
     const float u = (X * p.fx) / Z + p.cx;
     const float v = (Y * p.fy) / Z + p.cy;
-    if (!(u > -1.0f && u < (float)p.cols && v > -1.0f && v < (float)p.rows)) return;
-    const int ix = (int)u, iy = (int)v;
-    const unsigned long long key = ((unsigned long long)depth_key(Z) << 32) | (uint32_t)i;
-    unsigned long long* slot = zb + (int64_t)iy * p.cols + ix;
+    if (!(u > -1.0f && u < (float)p.cols && v > -1.0f && v < (float)p.rows)) return false;
+    *z_out = Z;
+    *slot = (int64_t)(int)v * p.cols + (int)u;
+    return true;
+}
+
+// One point per lane.  The point and its mask byte are loaded together (the
+// mask is a template switch, not a branch before the load).  The kernel is
+// bound by the atomics: a pixel receives one per new minimum in arrival
+// order (about ln k + 0.58 of k candidates for a random order), and wider
+// lanes (several points each) measured slower, fewer waves hiding the
+// atomics' latency.
+template <bool kMask>
+__global__ __launch_bounds__(256) void render_splat_kernel(const RenderParams p,
+                                                           unsigned long long* __restrict__ zb) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.num_points) return;
+    const float px = p.xyz[3 * i + 0], py = p.xyz[3 * i + 1], pz = p.xyz[3 * i + 2];
+    const uint8_t ok = kMask ? p.valid[i] : uint8_t(1);
+    float z;
+    int64_t slot;
+    if (!ok || !splat_target(p, px, py, pz, &z, &slot)) return;
+    const unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (uint32_t)i;
     // Keys only decrease during the pass, so a stale read is >= the current
     // minimum and skipping when it is <= key is safe; occluded points then
     // cost a load instead of an atomic.
-    if (*slot > key) atomicMin(slot, key);
+    if (zb[slot] > key) atomicMin(zb + slot, key);
 }
 
 __global__ __launch_bounds__(256) void render_resolve_kernel(
@@ -168,8 +185,11 @@ extern "C" int recon_points_render(const float* xyz, const uint8_t* valid, const
     hipLaunchKernelGGL(render_clear_kernel, dim3(clear_blocks), dim3(256), 0, s, zbuffer, pixels);
     RECON_HIP_TRY(hipGetLastError(), "render_clear_kernel launch");
     if (num_points > 0) {
-        hipLaunchKernelGGL(render_splat_kernel, dim3((unsigned)((num_points + 255) / 256)), dim3(256),
-                           0, s, p, zbuffer);
+        const dim3 grid((unsigned)((num_points + 255) / 256));
+        if (valid)
+            hipLaunchKernelGGL(render_splat_kernel<true>, grid, dim3(256), 0, s, p, zbuffer);
+        else
+            hipLaunchKernelGGL(render_splat_kernel<false>, grid, dim3(256), 0, s, p, zbuffer);
         RECON_HIP_TRY(hipGetLastError(), "render_splat_kernel launch");
     }
     hipLaunchKernelGGL(render_resolve_kernel, dim3((cols + 63) / 64, (rows + 3) / 4), dim3(256), 0,
