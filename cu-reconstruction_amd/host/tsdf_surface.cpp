@@ -57,6 +57,7 @@ TSDFSurface::TSDFSurface(const Config& cfg) {
     // additions of this library (0: the reference's constants)
     cfg_.truncation = cfg.get<float>("tsdf.volume.truncation", 0.f);
     cfg_.max_depth = cfg.get<float>("tsdf.volume.max_depth", 0.f);
+    welded_ = cfg.get<int>("tsdf.mesh.welded", 0) != 0;
     global_pose = euler_pose(cfg.get<float>("tsdf.volume.camera.pose.euler.roll", 0.f),
                              cfg.get<float>("tsdf.volume.camera.pose.euler.yaw", 0.f),
                              cfg.get<float>("tsdf.volume.camera.pose.euler.pitch", 0.f),
@@ -73,10 +74,14 @@ bool TSDFSurface::reconstruct(const PtrStepSz<float>& depth, const unsigned char
     if (!volume.ok()) return false;
     PtrStepSz<unsigned char> color{const_cast<unsigned char*>(rgb), rgb_step, depth.rows, depth.cols};
     if (!volume.integrate(color, depth, global_pose)) return false;
-    std::vector<float3> verts;
+    std::vector<float3> verts, normals;
     std::vector<int3> triangles;
-    generate_triangular_surface(volume, verts, triangles);
+    if (welded_)
+        generate_welded_surface(volume, verts, normals, triangles);
+    else
+        generate_triangular_surface(volume, verts, triangles);
     surface_mesh.vertices = std::move(verts);
+    surface_mesh.normals = std::move(normals);
     surface_mesh.polygons = std::move(triangles);
     return true;
 }
@@ -102,6 +107,7 @@ bool TSDFSurface::reconstruct_surface(PointCloud& cloud) {
 }
 
 bool Mesh::store_to_ply(const std::string& ply_filepath) const {
+    if (!normals.empty()) return write_mesh_to_ply(ply_filepath, vertices, normals, polygons);
     return write_mesh_to_ply(ply_filepath, vertices, polygons);
 }
 

@@ -247,6 +247,51 @@ bool PointCloud::save_ply(const std::string& path) const {
     return (bool)of;
 }
 
+void generate_welded_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
+                             std::vector<float3>& normals, std::vector<int3>& triangles) {
+    recon_tsdf* h = tsdf_volume.handle();
+    hipStream_t s = tsdf_volume.stream();
+    int64_t nv = 0, nt = 0;
+    if (!h || recon_tsdf_extract_welded_mesh(h, nullptr, nullptr, 0, nullptr, 0, &nv, &nt, s) != RECON_OK) {
+        log_error("generate_welded_surface()");
+        return;
+    }
+    if (nv == 0 || nt == 0) return;
+    float* dv = nullptr;
+    float* dn = nullptr;
+    int32_t* dt = nullptr;
+    bool ok = hipMalloc((void**)&dv, 12 * (size_t)nv) == hipSuccess &&
+              hipMalloc((void**)&dn, 12 * (size_t)nv) == hipSuccess &&
+              hipMalloc((void**)&dt, 12 * (size_t)nt) == hipSuccess;
+    if (!ok) std::fprintf(stderr, "[cuphoto] generate_welded_surface() -- out of device memory\n");
+    ok = ok && recon_tsdf_extract_welded_mesh(h, dv, dn, nv, dt, nt, &nv, &nt, s) == RECON_OK;
+    if (!ok) log_error("generate_welded_surface()");
+    const size_t v0 = verts.size(), t0 = triangles.size();
+    if (ok) {
+        verts.resize(v0 + (size_t)nv);
+        normals.resize(v0 + (size_t)nv);
+        triangles.resize(t0 + (size_t)nt);
+        ok = hipMemcpy(verts.data() + v0, dv, 12 * (size_t)nv, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(normals.data() + v0, dn, 12 * (size_t)nv, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(triangles.data() + t0, dt, 12 * (size_t)nt, hipMemcpyDeviceToHost) == hipSuccess;
+        if (!ok) {
+            std::fprintf(stderr, "[cuphoto] generate_welded_surface() -- copy failed\n");
+            verts.resize(v0);
+            normals.resize(v0);
+            triangles.resize(t0);
+        } else if (v0) {
+            for (size_t t = t0; t < triangles.size(); ++t) {
+                triangles[t].x += (int)v0;
+                triangles[t].y += (int)v0;
+                triangles[t].z += (int)v0;
+            }
+        }
+    }
+    if (dv) (void)hipFree(dv);
+    if (dn) (void)hipFree(dn);
+    if (dt) (void)hipFree(dt);
+}
+
 bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
                        const std::vector<int3>& triangles) {
     std::ofstream of(file_name);
@@ -261,6 +306,27 @@ bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& 
        << '\n' << "property list uchar int vertex_indices"
        << '\n' << "end_header\n";
     for (const float3& v : vertices) of << v.x << " " << v.y << " " << v.z << "\n";
+    for (const int3& t : triangles) of << "3 " << t.x << " " << t.y << " " << t.z << "\n";
+    return (bool)of;
+}
+
+}  // namespace cuphoto
+
+namespace cuphoto {
+
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<float3>& normals, const std::vector<int3>& triangles) {
+    if (normals.size() != vertices.size()) return false;
+    std::ofstream of(file_name);
+    if (!of.is_open()) return false;
+    of << "ply\nformat ascii 1.0\nelement vertex " << vertices.size()
+       << "\nproperty float x\nproperty float y\nproperty float z"
+          "\nproperty float nx\nproperty float ny\nproperty float nz"
+          "\nelement face " << triangles.size()
+       << "\nproperty list uchar int vertex_indices\nend_header\n";
+    for (size_t i = 0; i < vertices.size(); ++i)
+        of << vertices[i].x << " " << vertices[i].y << " " << vertices[i].z << " " << normals[i].x
+           << " " << normals[i].y << " " << normals[i].z << "\n";
     for (const int3& t : triangles) of << "3 " << t.x << " " << t.y << " " << t.z << "\n";
     return (bool)of;
 }

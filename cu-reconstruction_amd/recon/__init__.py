@@ -240,6 +240,32 @@ class TSDFVolume:
         t = int(n.value)
         return verts[:t], (tris[:t] if tris is not None else None)
 
+    def extract_welded_mesh(self, with_normals=True, stream=None):
+        """Welded mesh of the zero level set (recon_tsdf_extract_welded_mesh):
+        vertices (V, 3), normals (V, 3) or None, triangles (T, 3) int32."""
+        return _welded(lambda *a: self._lib.recon_tsdf_extract_welded_mesh(self._h, *a),
+                       with_normals, stream)
+
+
+def _welded(call, with_normals, stream):
+    torch = _torch()
+    nv, nt = C.c_int64(), C.c_int64()
+    check(call(None, None, 0, None, 0, C.byref(nv), C.byref(nt), _stream(stream)))
+    V, T = int(nv.value), int(nt.value)
+    verts = torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda")
+    normals = torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda") if with_normals else None
+    tris = torch.empty((max(T, 1), 3), dtype=torch.int32, device="cuda")
+    check(call(verts.data_ptr(), normals.data_ptr() if normals is not None else None, V,
+               tris.data_ptr(), T, C.byref(nv), C.byref(nt), _stream(stream)))
+    return verts[:V], (normals[:V] if normals is not None else None), tris[:T]
+
+
+def mc_extract_welded(field, grid, voxel_size, offset, with_normals=True, stream=None):
+    """Welded marching cubes of a dense field (recon_mc_extract_welded)."""
+    g, vs, off = _grid_args(grid, voxel_size, offset)
+    return _welded(lambda *a: ffi.load().recon_mc_extract_welded(field.data_ptr(), g, vs, off, *a),
+                   with_normals, stream)
+
 
 def _grid_args(grid, voxel_size, offset):
     return ffi.i32x3(grid), ffi.f32x3(voxel_size), ffi.f32x3(offset)

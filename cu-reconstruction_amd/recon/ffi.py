@@ -111,6 +111,11 @@ SIGNATURES = {
     "recon_mc_extract": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
                                    C.POINTER(C.c_float), _vp, _vp, _i64, _pi64, _vp]),
     "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
+    "recon_mc_extract_welded": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float), _vp, _vp, _i64, _vp, _i64, _pi64,
+                                          _pi64, _vp]),
+    "recon_tsdf_extract_welded_mesh": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _pi64, _pi64,
+                                                 _vp]),
     "recon_depth_to_points": (C.c_int, [C.POINTER(DepthView), C.c_double, C.c_double, C.c_double,
                                         C.c_double, _d7, _vp, C.c_size_t, _vp, _vp, _vp, _vp]),
     "recon_points_render": (C.c_int, [_vp, _vp, _vp, _i64, _d7, C.c_double, C.c_double,

@@ -170,6 +170,34 @@ RECON_API int recon_mc_extract(const float* field, const int32_t grid[3], const 
 RECON_API int recon_mc_class_histogram(const float* field, const int32_t grid[3],
                              unsigned long long* hist_device, recon_stream_t stream);
 
+/* ---- welded mesh ---------------------------------------------------------- */
+/* sync.  No counterpart in the reference, whose extraction emits a triangle
+ * soup (marching_cubes.cu:171-179 never shares a vertex): the same surface
+ * with one vertex per grid edge whose end values differ in sign (v < 0
+ * against not), about a third of the soup's vertices.  Vertices are numbered
+ * by (owning lower-end voxel x + gx*(y + gy*z), axis x, y, z); each runs from
+ * the negative end N to the other end M, p = sN + r*(sM - sN) with
+ * r = (-dN)/(dM - dN), the soup's formula (bit-identical to the soup's
+ * vertex except where dM is exactly +-0).  Normals (may be NULL) are the
+ * field's gradient -- central differences over 2*vs inside the grid,
+ * one-sided over vs on its border -- lerped from N to M with r and
+ * normalised, (0,0,0) where it vanishes.  Triangles (may be NULL) are the
+ * soup's, in its order, as int32 vertex indices.  Always returns both
+ * counts; with every output NULL it only counts.  Fails with
+ * RECON_ERR_CAPACITY, writing nothing, when a requested output does not fit,
+ * and with RECON_ERR_RANGE above 2^31-1 vertices. */
+RECON_API int recon_mc_extract_welded(const float* field, const int32_t grid[3],
+                                      const float voxel_size[3], const float offset[3],
+                                      float* vertices, float* normals, int64_t vertex_capacity,
+                                      int32_t* triangles, int64_t triangle_capacity,
+                                      int64_t* num_vertices, int64_t* num_triangles,
+                                      recon_stream_t stream);
+/* sync.  The same on a volume's TSDF (the handle's scratch is reused). */
+RECON_API int recon_tsdf_extract_welded_mesh(recon_tsdf* h, float* vertices, float* normals,
+                                             int64_t vertex_capacity, int32_t* triangles,
+                                             int64_t triangle_capacity, int64_t* num_vertices,
+                                             int64_t* num_triangles, recon_stream_t stream);
+
 /* ---- depth -> point cloud (point_cloud_extract, cuda_point_cloud.cu:22-70) */
 /* async.  Pixel (x,y) with depth d>0: p = ((x-cx)*d/fx, (y-cy)*d/fy, d)
  * computed in float64 and rounded to float32, then p_world = T*p in float32

@@ -80,7 +80,9 @@ private:
 
 struct Mesh {
     std::vector<float3> vertices;
+    std::vector<float3> normals;  // empty, or one per vertex (welded meshes)
     std::vector<int3> polygons;
+    // with normals when there are any (mesh.cpp:59-64 switches the same way)
     bool store_to_ply(const std::string& ply_filepath) const;
 };
 
@@ -112,6 +114,7 @@ public:
 private:
     bool reconstruct(const PtrStepSz<float>& depth, const unsigned char* rgb, size_t rgb_step);
     TSDFVolumeConfig cfg_;
+    bool welded_ = false;  // parameter tsdf.mesh.welded (this library's addition)
     std::array<double, 7> global_pose{1, 0, 0, 0, 0, 0, 0};
     Mesh surface_mesh;
 };

@@ -110,6 +110,14 @@ private:
 void generate_triangular_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
                                  std::vector<int3>& triangles);
 
+// No counterpart in the reference (whose soup never shares a vertex): appends
+// the welded mesh of the same surface -- one vertex per sign-changing grid
+// edge, its unit normal from the TSDF gradient, and the soup's triangles as
+// indices into `verts` (offset by the vertices already there).  `normals`
+// grows with `verts`.
+void generate_welded_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
+                             std::vector<float3>& normals, std::vector<int3>& triangles);
+
 // Device point buffer filled by extract_points (cudaPointCloud's role on the
 // TSDF path): xyz float3, a validity byte and an rgb triple per point.
 class PointCloud {
@@ -183,5 +191,8 @@ private:
 // utils.cpp:96-118: ASCII PLY of vertices and triangles.
 bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
                        const std::vector<int3>& triangles);
+// The same with per-vertex normals (property float nx, ny, nz after x, y, z).
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<float3>& normals, const std::vector<int3>& triangles);
 
 }  // namespace cuphoto

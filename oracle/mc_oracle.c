@@ -8,6 +8,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 
 #include "recon/mc_tables.h"
@@ -156,4 +157,119 @@ int64_t oracle_mc_extract(const float* field, const int32_t grid[3],
     }
     free(base);
     return total;
+}
+
+/* ---- welded mesh ---------------------------------------------------------
+ * No reference counterpart: the reference emits a triangle soup (its
+ * per-triangle vert_remap, marching_cubes.cu:171-179, never shares a
+ * vertex).  One vertex per grid edge whose end values differ in sign
+ * (v < 0 against not), owned by its lower end voxel, numbered in the order
+ * (voxel x + gx*(y + gy*z), axis x, y, z).  Triangles are the soup's, in the
+ * soup's order, as indices into these vertices.  The vertex runs from the
+ * negative end N to the other end M: r = (-dN) / (dM - dN), p = sN + r*(sM -
+ * sN) per component -- the soup's own formula whenever it orients the edge
+ * from N, i.e. except when dM is exactly +-0 and the cube lists M first.
+ * The normal is the TSDF gradient, central differences divided by
+ * (vs + vs) (one-sided, divided by vs, on the grid border), lerped from N to
+ * M with the same r and normalised ((0,0,0) for a zero gradient). */
+
+static inline float grad_axis(const float* f, const int32_t grid[3], int32_t x, int32_t y,
+                              int32_t z, int a, const float vs[3]) {
+    const int64_t sx = 1, sy = grid[0], sz = (int64_t)grid[0] * grid[1];
+    const int64_t st = a == 0 ? sx : (a == 1 ? sy : sz);
+    const int32_t c = a == 0 ? x : (a == 1 ? y : z);
+    const int64_t i = (int64_t)x + sy * y + sz * z;
+    if (c > 0 && c < grid[a] - 1) return (f[i + st] - f[i - st]) / (vs[a] + vs[a]);
+    if (c == 0) return grid[a] > 1 ? (f[i + st] - f[i]) / vs[a] : 0.0f;
+    return (f[i] - f[i - st]) / vs[a];
+}
+
+int64_t oracle_mc_weld(const float* field, const int32_t grid[3], const float voxel_size[3],
+                       const float offset[3], float* verts, float* normals, int64_t vertex_capacity,
+                       int32_t* tris, int64_t triangle_capacity, int64_t* num_vertices) {
+    const int32_t gx = grid[0], gy = grid[1], gz = grid[2];
+    *num_vertices = 0;
+    if (gx < 2 || gy < 2 || gz < 2) return 0;
+    oracle_volume vol = {{gx, gy, gz}, {voxel_size[0], voxel_size[1], voxel_size[2]},
+                         {offset[0], offset[1], offset[2]}, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t n = (int64_t)gx * gy * gz;
+    int64_t* id = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)n);
+    int64_t nv = 0;
+    for (int32_t z = 0; z < gz; ++z)
+        for (int32_t y = 0; y < gy; ++y)
+            for (int32_t x = 0; x < gx; ++x) {
+                const int64_t i = (int64_t)x + (int64_t)gx * (y + (int64_t)gy * z);
+                for (int a = 0; a < 3; ++a) {
+                    id[3 * i + a] = -1;
+                    const int32_t q[3] = {x + (a == 0), y + (a == 1), z + (a == 2)};
+                    if (q[a] >= grid[a]) continue;
+                    const int64_t j = (int64_t)q[0] + (int64_t)gx * (q[1] + (int64_t)gy * q[2]);
+                    if ((field[i] < 0.0f) == (field[j] < 0.0f)) continue;
+                    const int64_t v = nv++;
+                    id[3 * i + a] = v;
+                    if (v >= vertex_capacity) continue;
+                    /* N = negative end */
+                    const int neg_first = field[i] < 0.0f;
+                    const int32_t pn[3] = {neg_first ? x : q[0], neg_first ? y : q[1], neg_first ? z : q[2]};
+                    const int32_t pm[3] = {neg_first ? q[0] : x, neg_first ? q[1] : y, neg_first ? q[2] : z};
+                    const float dn = neg_first ? field[i] : field[j];
+                    const float dm = neg_first ? field[j] : field[i];
+                    const float r = (-dn) / (dm - dn);
+                    float sn[3], sm[3];
+                    oracle_voxel_centre_f32(&vol, pn[0], pn[1], pn[2], sn);
+                    oracle_voxel_centre_f32(&vol, pm[0], pm[1], pm[2], sm);
+                    if (verts)
+                        for (int k = 0; k < 3; ++k) {
+                            const float e = sm[k] - sn[k];
+                            const float o = r * e;
+                            verts[3 * v + k] = sn[k] + o;
+                        }
+                    if (normals) {
+                        float g[3];
+                        for (int k = 0; k < 3; ++k) {
+                            const float gn = grad_axis(field, grid, pn[0], pn[1], pn[2], k, voxel_size);
+                            const float gm = grad_axis(field, grid, pm[0], pm[1], pm[2], k, voxel_size);
+                            const float e = gm - gn;
+                            const float o = r * e;
+                            g[k] = gn + o;
+                        }
+                        const float len2 = (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+                        if (len2 > 0.0f) {
+                            const float len = sqrtf(len2);
+                            for (int k = 0; k < 3; ++k) normals[3 * v + k] = g[k] / len;
+                        } else {
+                            for (int k = 0; k < 3; ++k) normals[3 * v + k] = 0.0f;
+                        }
+                    }
+                }
+            }
+    *num_vertices = nv;
+    /* triangles in the soup's order (z, y, x, slot) */
+    int64_t t = 0;
+    for (int32_t z = 0; z < gz - 1; ++z)
+        for (int32_t y = 0; y < gy - 1; ++y)
+            for (int32_t x = 0; x < gx - 1; ++x) {
+                float v[8];
+                corner_values(field, gx, gy, x, y, z, v);
+                const int c = cube_class(v);
+                const int nt = class_ntri(c);
+                const uint64_t word = k_tri[c];
+                for (int s = 0; s < nt; ++s, ++t) {
+                    if (t >= triangle_capacity || !tris) continue;
+                    for (int k = 0; k < 3; ++k) {
+                        const int e = RECON_MC_EDGE(word, 3 * s + k);
+                        const int* ca = k_corner[k_edge[e][0]];
+                        const int* cb = k_corner[k_edge[e][1]];
+                        int a = 0;
+                        while (ca[a] == cb[a]) ++a;
+                        const int32_t ox = x + (ca[0] < cb[0] ? ca[0] : cb[0]);
+                        const int32_t oy = y + (ca[1] < cb[1] ? ca[1] : cb[1]);
+                        const int32_t oz = z + (ca[2] < cb[2] ? ca[2] : cb[2]);
+                        const int64_t i = (int64_t)ox + (int64_t)gx * (oy + (int64_t)gy * oz);
+                        tris[3 * t + k] = (int32_t)id[3 * i + a];
+                    }
+                }
+            }
+    free(id);
+    return t;
 }

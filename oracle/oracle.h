@@ -104,6 +104,17 @@ void oracle_depth_to_points(const float* depth, int32_t rows, int32_t cols, int6
                             const uint8_t* rgb, int64_t rgb_pitch,
                             float* xyz, uint8_t* valid, uint8_t* colors_out);
 
+/* Welded marching cubes (no reference counterpart; the reference emits a
+ * soup).  One vertex per grid edge whose end values differ in sign, numbered
+ * by (owning lower-end voxel, axis x, y, z); the vertex runs from the negative
+ * end (equal to the soup's vertex unless the other end is exactly +-0);
+ * normals are the lerped central-difference gradient, normalised.  Triangles
+ * are the soup's, in its order, as int32 vertex indices.  Writes up to the
+ * capacities; returns the triangle count and *num_vertices. */
+int64_t oracle_mc_weld(const float* field, const int32_t grid[3], const float voxel_size[3],
+                       const float offset[3], float* verts, float* normals, int64_t vertex_capacity,
+                       int32_t* tris, int64_t triangle_capacity, int64_t* num_vertices);
+
 /* project_to_depth / project_to_color_depth + filter_image_if
  * (cuda_point_cloud.cu:229-353, 89-99): points in index order, world to
  * camera with the float32 pose matrix, pixel = trunc((X*fx)/Z + cx) in

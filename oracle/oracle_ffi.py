@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
                                             C.c_double, vp]),
             "oracle_render_points": (None, [vp, vp, vp, i64, vp, C.c_float, C.c_float, C.c_float,
                                             C.c_float, i32, i32, i64, vp, vp, i64]),
+            "oracle_mc_weld": (i64, [vp, vp, vp, vp, vp, vp, i64, vp, i64, vp]),
         })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -178,6 +179,23 @@ def depth_to_points(depth, intrinsics, transform, rgb=None):
                                  _p(pose_arr(transform)), _p(rgbp), cols * 3, _p(xyz), _p(valid),
                                  _p(colors))
     return xyz, valid, colors
+
+
+def mc_weld(field, grid, voxel_size, offset):
+    """Welded mesh: vertices (V, 3), normals (V, 3) float32, triangles (T, 3)
+    int32."""
+    field = np.ascontiguousarray(field, np.float32)
+    g = np.array(grid, np.int32)
+    vs = np.array(voxel_size, np.float32)
+    off = np.array(offset, np.float32)
+    nv = C.c_int64()
+    nt = lib().oracle_mc_weld(_p(field), _p(g), _p(vs), _p(off), None, None, 0, None, 0, C.byref(nv))
+    verts = np.empty((max(nv.value, 1), 3), np.float32)
+    normals = np.empty((max(nv.value, 1), 3), np.float32)
+    tris = np.empty((max(nt, 1), 3), np.int32)
+    lib().oracle_mc_weld(_p(field), _p(g), _p(vs), _p(off), _p(verts), _p(normals), nv.value,
+                         _p(tris), nt, C.byref(nv))
+    return verts[:nv.value], normals[:nv.value], tris[:nt]
 
 
 def render_points(xyz, pose, intrinsics, image, valid=None, colors=None, want_color=True):

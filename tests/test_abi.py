@@ -208,3 +208,21 @@ def test_render_arguments_rejected_before_any_launch():
         assert call(fx=fx) == ffi.ERR_CONFIG, fx
     assert call(cx=float("nan")) == ffi.ERR_CONFIG
     assert call(pose=ffi.pose_array([1, 0, 0, float("inf"), 0, 0, 0])) == ffi.ERR_CONFIG
+
+
+def test_welded_arguments_rejected_before_any_launch():
+    lib = ffi.load()
+    nv, nt = C.c_int64(7), C.c_int64(7)
+    g, vs, off = ffi.i32x3((4, 4, 4)), ffi.f32x3((1, 1, 1)), ffi.f32x3((0, 0, 0))
+    f = C.c_void_p(4096)
+    assert lib.recon_mc_extract_welded(f, g, vs, off, None, None, 0, None, 0, None, C.byref(nt), None) == ffi.ERR_NULL
+    assert lib.recon_mc_extract_welded(None, g, vs, off, None, None, 0, None, 0, C.byref(nv), C.byref(nt), None) == ffi.ERR_NULL
+    assert nv.value == 0 and nt.value == 0
+    assert lib.recon_mc_extract_welded(f, ffi.i32x3((0, 4, 4)), vs, off, None, None, 0, None, 0,
+                                       C.byref(nv), C.byref(nt), None) == ffi.ERR_SIZE
+    assert lib.recon_mc_extract_welded(f, ffi.i32x3((2048, 1024, 1024)), vs, off, None, None, 0, None,
+                                       0, C.byref(nv), C.byref(nt), None) == ffi.ERR_RANGE
+    assert lib.recon_mc_extract_welded(f, g, vs, off, None, None, -1, None, 0, C.byref(nv),
+                                       C.byref(nt), None) == ffi.ERR_SIZE
+    assert lib.recon_tsdf_extract_welded_mesh(None, None, None, 0, None, 0, C.byref(nv), C.byref(nt),
+                                              None) == ffi.ERR_NULL

@@ -1,8 +1,9 @@
 """The end-to-end driver (cu-reconstruction_amd/lib/tsdf_surface, the
 reference's examples/tsdf_surface_reconstruction/tsdf_surface.cpp):
 parameter file -> first RGB-D pair of a directory -> integrate -> marching
-cubes -> ASCII PLY.  The PLY must equal, character for character, the one
-the CPU oracle's integrate and marching cubes give for the same decoded
+cubes -> ASCII PLY (a triangle soup, or with tsdf.mesh.welded: 1 shared
+vertices with normals).  The PLY must equal, character for character, the
+one the CPU oracle's integrate and marching cubes give for the same decoded
 depth and the parameter file's pose."""
 import ctypes as C
 import subprocess
@@ -75,7 +76,7 @@ def _euler_pose():
     return np.array(out[:])
 
 
-def _expected_ply(raw, pose):
+def _expected_ply(raw, pose, welded=False):
     scale = np.float64(np.float32(SCALE))      # read as float, widened
     depth = (raw.astype(np.float64) * scale).astype(np.float32)
     cfg = recon.make_config(GRID, physical_size=PHYS, offset=OFFSET, max_weight=15.0,
@@ -83,18 +84,25 @@ def _expected_ply(raw, pose):
     vol = O.volume_from_config(cfg)
     D, W = O.init(vol)
     O.integrate_f32(vol, D, W, depth, pose)
-    verts, _, _, _ = O.mc_extract(D, GRID, list(vol.voxel_size), list(vol.offset))
-    v = verts.reshape(-1, 3)
-    n = v.shape[0] // 3
-    lines = ["ply", "format ascii 1.0", f"element vertex {v.shape[0]}", "property float x",
-             "property float y", "property float z", f"element face {n}",
-             "property list uchar int vertex_indices", "end_header"]
-    lines += ["%g %g %g" % (float(a), float(b), float(c)) for a, b, c in v]
-    lines += [f"3 {3 * t} {3 * t + 1} {3 * t + 2}" for t in range(n)]
-    return "\n".join(lines) + "\n", n
+    if welded:
+        v, nrm, tri = O.mc_weld(D, GRID, list(vol.voxel_size), list(vol.offset))
+        props = ["property float x", "property float y", "property float z", "property float nx",
+                 "property float ny", "property float nz"]
+        rows = ["%g %g %g %g %g %g" % tuple(float(q) for q in (*a, *b)) for a, b in zip(v, nrm)]
+        faces = ["3 %d %d %d" % tuple(int(q) for q in f) for f in tri]
+    else:
+        verts, _, _, _ = O.mc_extract(D, GRID, list(vol.voxel_size), list(vol.offset))
+        v = verts.reshape(-1, 3)
+        props = ["property float x", "property float y", "property float z"]
+        rows = ["%g %g %g" % (float(a), float(b), float(c)) for a, b, c in v]
+        faces = [f"3 {3 * t} {3 * t + 1} {3 * t + 2}" for t in range(v.shape[0] // 3)]
+    lines = ["ply", "format ascii 1.0", f"element vertex {v.shape[0]}", *props,
+             f"element face {len(faces)}", "property list uchar int vertex_indices", "end_header"]
+    return "\n".join(lines + rows + faces) + "\n", len(faces)
 
 
-def test_driver_ply_equals_oracle(cuda, tmp_path):
+@pytest.mark.parametrize("welded", [False, True])
+def test_driver_ply_equals_oracle(cuda, tmp_path, welded):
     assert EXE.exists(), "build the libraries first"
     pose = _euler_pose()
     R, t = scenes.pose_matrix(pose)
@@ -114,13 +122,13 @@ def test_driver_ply_equals_oracle(cuda, tmp_path):
     Image.fromarray(raw).save(data / "000_depth.png")
     mesh = tmp_path / "mesh.ply"
     cfg = tmp_path / "params.yaml"
-    cfg.write_text(PARAMS.format(data=data, mesh=mesh))
+    cfg.write_text(PARAMS.format(data=data, mesh=mesh) + ("tsdf.mesh.welded: 1\n" if welded else ""))
     print("tsdf_surface driver", flush=True)
     r = subprocess.run([str(EXE), f"--config_path={cfg}"], capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
-    expected, ntri = _expected_ply(raw, pose)
+    expected, ntri = _expected_ply(raw, pose, welded)
     assert ntri > 20000
     got = mesh.read_text()
     if got != expected:
