@@ -553,6 +553,32 @@ def main() -> int:
         "fused_frames_ms_per_step": round(fused_ms, 3),
         "fused_frames_gvu_s": round(swept / (fused_ms / 1000.0) / 1e9, 3),
     }
+    # meshing of the final volume: the triangle soup (count + emit, the
+    # drop-in generate_triangular_surface path) against the welded mesh with
+    # normals (count + vertices/normals + indexed triangles); both are
+    # synchronous calls, timed on the host clock
+    def timed(fn, reps=5):
+        out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return 1000.0 * (time.perf_counter() - t0) / reps, out
+
+    soup_ms, _ = timed(lambda: vol.extract_mesh(verts, tris, stream))
+    nv, nt = vol.count_welded(stream)
+    wbuf = (torch.empty((nv, 3), dtype=torch.float32, device="cuda"),
+            torch.empty((nv, 3), dtype=torch.float32, device="cuda"),
+            torch.empty((nt, 3), dtype=torch.int32, device="cuda"))
+    weld_ms, (wv, _, wt) = timed(lambda: vol.extract_welded_mesh(out=wbuf, stream=stream))
+    result["meshing"] = {
+        "grid": list(w.grid), "triangles": int(wt.shape[0]),
+        "soup_ms": round(soup_ms, 3), "soup_vertices": 3 * int(wt.shape[0]),
+        "welded_ms": round(weld_ms, 3), "welded_vertices": int(wv.shape[0]),
+        "welded_bytes_vs_soup": round((24.0 * wv.shape[0] + 12.0 * wt.shape[0]) /
+                                      (36.0 * wt.shape[0] + 12.0 * wt.shape[0]), 3)}
+    del wv, wt, wbuf
     vol.close()
     del dev
     torch.cuda.empty_cache()

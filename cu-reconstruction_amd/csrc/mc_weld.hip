@@ -14,8 +14,10 @@
 // gives each chunk's first vertex.  The index of edge (voxel, axis) is then
 //   base[chunk] + popc of the three masks below the voxel's lane
 //               + the voxel's own set bits of the axes before `axis`,
-// which the vertex writer and the triangle writer both evaluate: 28 B of
-// masks and counts and 8 B of base per 64 voxels.
+// which the vertex writer and the triangle writer both evaluate: 48 B of
+// masks, counts and bases per 64 voxels.  Two passes over the field, like
+// the soup: weld_count_kernel (edge masks and the chunk's triangle count),
+// two exclusive scans, weld_write_kernel (vertices, normals, triangles).
 //
 // Vertex: from the negative end N to the other end M, r = (-dN)/(dM - dN),
 // p = sN + r*(sM - sN) per component.  This is the soup's own formula
@@ -33,7 +35,7 @@
 
 namespace recon {
 
-constexpr int kWeldWaves = 4;  // one wavefront per chunk or cube row, 256 threads
+constexpr int kWeldWaves = 4;  // one wavefront per voxel row, 256 threads
 
 struct WeldParams {
     McParams mc;
@@ -41,30 +43,68 @@ struct WeldParams {
     int32_t nchunks;  // cpr * gy * gz
 };
 
-__global__ __launch_bounds__(256) void weld_mask_kernel(const WeldParams w,
-                                                        unsigned long long* __restrict__ masks,
-                                                        int32_t* __restrict__ counts) {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    if (c >= w.nchunks) return;
+// Chunk c covers voxels x in [64k, 64k + 64) of voxel row (y, z), and the
+// cubes with the same lower corners.  One pass marks the chunk's
+// sign-changing edges and counts its cubes' triangles.
+struct ChunkCorners {
+    int x, y, z;
+    float v[8];    // corner values of cube (x, y, z) (valid when `cube`)
+    bool cube;     // (x, y, z) is the lower corner of a cube
+    bool in;       // (x, y, z) is a voxel
+};
+
+__device__ __forceinline__ ChunkCorners chunk_corners(const WeldParams& w, int c, int lane) {
     const McParams& p = w.mc;
+    ChunkCorners k;
     const int row = c / w.cpr;  // y + gy*z
-    const int x = (c % w.cpr) * 64 + lane;
-    const int y = row % p.gy, z = row / p.gy;
-    bool bx = false, by = false, bz = false;
-    if (x < p.gx) {
-        const int64_t i = (int64_t)x + (int64_t)p.gx * row;
-        const bool n0 = p.f[i] < 0.0f;
-        bx = x + 1 < p.gx && (p.f[i + 1] < 0.0f) != n0;
-        by = y + 1 < p.gy && (p.f[i + p.gx] < 0.0f) != n0;
-        bz = z + 1 < p.gz && (p.f[i + (int64_t)p.gx * p.gy] < 0.0f) != n0;
+    k.x = (c % w.cpr) * 64 + lane;
+    k.y = row % p.gy;
+    k.z = row / p.gy;
+    k.in = k.x < p.gx;
+    k.cube = k.x < p.gx - 1 && k.y < p.gy - 1 && k.z < p.gz - 1;
+    if (k.cube) {
+        load_corners(p, k.x, k.y, k.z, k.v);
+    } else if (k.in) {
+        // the voxel and whichever +x, +y, +z neighbours exist, in the
+        // corner slots 0, 1, 4, 3 (all edge tests read only these)
+        const int64_t i = (int64_t)k.x + (int64_t)p.gx * row;
+        k.v[0] = p.f[i];
+        k.v[1] = k.x + 1 < p.gx ? p.f[i + 1] : k.v[0];
+        k.v[4] = k.y + 1 < p.gy ? p.f[i + p.gx] : k.v[0];
+        k.v[3] = k.z + 1 < p.gz ? p.f[i + (int64_t)p.gx * p.gy] : k.v[0];
     }
-    const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
-    if (lane == 0) {
-        masks[3 * (int64_t)c + 0] = mx;
-        masks[3 * (int64_t)c + 1] = my;
-        masks[3 * (int64_t)c + 2] = mz;
-        counts[c] = __popcll(mx) + __popcll(my) + __popcll(mz);
+    return k;
+}
+
+__global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
+                                                         unsigned long long* __restrict__ masks,
+                                                         int32_t* __restrict__ vcount,
+                                                         int32_t* __restrict__ tcount) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
+    if (row >= w.mc.gy * w.mc.gz) return;
+    const McParams& p = w.mc;
+    for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
+        const ChunkCorners k = chunk_corners(w, c, lane);
+        bool bx = false, by = false, bz = false;
+        int nt = 0;
+        if (k.in) {
+            // corners 1, 4, 3 are the +x, +y, +z neighbours (marching_cubes.cu:118-127)
+            const bool n0 = k.v[0] < 0.0f;
+            bx = k.x + 1 < p.gx && (k.v[1] < 0.0f) != n0;
+            by = k.y + 1 < p.gy && (k.v[4] < 0.0f) != n0;
+            bz = k.z + 1 < p.gz && (k.v[3] < 0.0f) != n0;
+            if (k.cube) nt = class_ntri(cube_class(k.v));
+        }
+        const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
+        nt = wave_sum(nt);
+        if (lane == 0) {
+            masks[3 * (int64_t)c + 0] = mx;
+            masks[3 * (int64_t)c + 1] = my;
+            masks[3 * (int64_t)c + 2] = mz;
+            vcount[c] = __popcll(mx) + __popcll(my) + __popcll(mz);
+            tcount[c] = nt;
+        }
     }
 }
 
@@ -79,19 +119,14 @@ __device__ __forceinline__ float weld_grad(const McParams& p, int x, int y, int 
     return (p.f[i] - p.f[i - st]) / p.vs[a];
 }
 
-__global__ __launch_bounds__(256) void weld_vertex_kernel(const WeldParams w,
-                                                          const unsigned long long* __restrict__ masks,
-                                                          const long long* __restrict__ base,
-                                                          float* __restrict__ verts,
-                                                          float* __restrict__ normals) {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    if (c >= w.nchunks) return;
+// Vertices (and normals) of the edges lane `lane` of chunk c owns.
+__device__ __forceinline__ void weld_lane_vertices(const WeldParams& w, int c, int lane,
+                                                   const unsigned long long m[3], long long base,
+                                                   float* __restrict__ verts,
+                                                   float* __restrict__ normals) {
     const McParams& p = w.mc;
-    const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
-                                     masks[3 * (int64_t)c + 2]};
     const unsigned long long below = (1ull << lane) - 1ull;
-    long long idx = base[c] + __popcll(m[0] & below) + __popcll(m[1] & below) + __popcll(m[2] & below);
+    long long idx = base + __popcll(m[0] & below) + __popcll(m[1] & below) + __popcll(m[2] & below);
     const int row = c / w.cpr;
     const int x = (c % w.cpr) * 64 + lane;
     const int y = row % p.gy, z = row / p.gy;
@@ -164,49 +199,44 @@ __device__ __forceinline__ int32_t weld_index(const WeldParams& w,
     return (int32_t)idx;
 }
 
-__global__ __launch_bounds__(256) void weld_row_count_kernel(const McParams p,
-                                                             int32_t* __restrict__ row_count) {
+// One wavefront per voxel row, chunk by chunk: the chunk's vertices, then
+// its cubes' triangles in the soup's order (chunks ascend in z, y, x like
+// the soup's cubes).
+__global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
+                                                         const unsigned long long* __restrict__ masks,
+                                                         const long long* __restrict__ vbase,
+                                                         const int32_t* __restrict__ tcount,
+                                                         const long long* __restrict__ tbase,
+                                                         float* __restrict__ verts,
+                                                         float* __restrict__ normals,
+                                                         int32_t* __restrict__ tris) {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    if (r >= p.rows) return;
-    const int n = mc_row_triangles(p, r, lane);
-    if (lane == 0) row_count[r] = n;
-}
-
-// Triangles of cube row r in the soup's order (one wavefront per row).
-__global__ __launch_bounds__(256) void weld_emit_kernel(const WeldParams w,
-                                                        const int32_t* __restrict__ row_count,
-                                                        const long long* __restrict__ row_offset,
-                                                        const unsigned long long* __restrict__ masks,
-                                                        const long long* __restrict__ base,
-                                                        int32_t* __restrict__ tris) {
-    const int lane = threadIdx.x & 63;
-    const McParams& p = w.mc;
-    const int r = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    if (r >= p.rows || row_count[r] == 0) return;
-    const int y = r % p.cyr, z = r / p.cyr;
-    long long t0 = row_offset[r];
-    for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        const int x = x0 + lane;
-        int c = 0, nt = 0;
-        if (x < p.cx) {
-            float v[8];
-            load_corners(p, x, y, z, v);
-            c = cube_class(v);
-            nt = class_ntri(c);
+    const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
+    if (row >= w.mc.gy * w.mc.gz) return;
+    for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
+        if (verts || normals) {
+            const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
+                                             masks[3 * (int64_t)c + 2]};
+            if (m[0] | m[1] | m[2]) weld_lane_vertices(w, c, lane, m, vbase[c], verts, normals);
+        }
+        if (!tris || tcount[c] == 0) continue;
+        const ChunkCorners k = chunk_corners(w, c, lane);
+        int cls = 0, nt = 0;
+        if (k.cube) {
+            cls = cube_class(k.v);
+            nt = class_ntri(cls);
         }
         const int incl = wave_inclusive_scan(nt, lane);
-        long long t = t0 + (incl - nt);
+        long long t = tbase[c] + (incl - nt);
         if (nt) {
-            const uint64_t word = c_tri[c];
+            const uint64_t word = c_tri[cls];
             for (int s = 0; s < nt; ++s, ++t)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int e = (int)((word >> (4 + 4 * (3 * s + k))) & 0xFull);
-                    tris[3 * t + k] = weld_index(w, masks, base, x, y, z, e);
+                for (int q = 0; q < 3; ++q) {
+                    const int e = (int)((word >> (4 + 4 * (3 * s + q))) & 0xFull);
+                    tris[3 * t + q] = weld_index(w, masks, vbase, k.x, k.y, k.z, e);
                 }
         }
-        t0 += __shfl(incl, 63, 64);
     }
 }
 
@@ -235,16 +265,15 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     }
     w.cpr = (grid[0] + 63) / 64;
     w.nchunks = w.cpr * grid[1] * grid[2];
-    const int nbc = (w.nchunks + kScanBlock - 1) / kScanBlock;
-    const int nbr = (p.rows + kScanBlock - 1) / kScanBlock;
+    const int nb = (w.nchunks + kScanBlock - 1) / kScanBlock;
     const size_t o_masks = 0;
-    const size_t o_counts = o_masks + align256(24 * (size_t)w.nchunks);
-    const size_t o_base = o_counts + align256(4 * (size_t)w.nchunks);
-    const size_t o_csums = o_base + align256(8 * (size_t)w.nchunks);
-    const size_t o_rcount = o_csums + align256(8 * (size_t)(nbc + 1));
-    const size_t o_roff = o_rcount + align256(4 * (size_t)p.rows);
-    const size_t o_rsums = o_roff + align256(8 * (size_t)p.rows);
-    const size_t need = o_rsums + align256(8 * (size_t)(nbr + 1));
+    const size_t o_vcount = o_masks + align256(24 * (size_t)w.nchunks);
+    const size_t o_tcount = o_vcount + align256(4 * (size_t)w.nchunks);
+    const size_t o_vbase = o_tcount + align256(4 * (size_t)w.nchunks);
+    const size_t o_tbase = o_vbase + align256(8 * (size_t)w.nchunks);
+    const size_t o_vsums = o_tbase + align256(8 * (size_t)w.nchunks);
+    const size_t o_tsums = o_vsums + align256(8 * (size_t)(nb + 1));
+    const size_t need = o_tsums + align256(8 * (size_t)(nb + 1));
     if (*scratch_bytes < need) {
         if (*scratch) (void)hipFree(*scratch);
         *scratch = nullptr;
@@ -254,25 +283,22 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     }
     char* sb = static_cast<char*>(*scratch);
     auto* masks = reinterpret_cast<unsigned long long*>(sb + o_masks);
-    auto* counts = reinterpret_cast<int32_t*>(sb + o_counts);
-    auto* base = reinterpret_cast<long long*>(sb + o_base);
-    auto* csums = reinterpret_cast<long long*>(sb + o_csums);
-    auto* rcount = reinterpret_cast<int32_t*>(sb + o_rcount);
-    auto* roff = reinterpret_cast<long long*>(sb + o_roff);
-    auto* rsums = reinterpret_cast<long long*>(sb + o_rsums);
-    const int cblocks = (w.nchunks + kWeldWaves - 1) / kWeldWaves;
-    const int rblocks = (p.rows + kWeldWaves - 1) / kWeldWaves;
-    hipLaunchKernelGGL(weld_mask_kernel, dim3(cblocks), dim3(256), 0, s, w, masks, counts);
-    RECON_HIP_TRY(hipGetLastError(), "weld_mask_kernel launch");
-    int rc = launch_exclusive_scan(counts, w.nchunks, base, csums, s);
+    auto* vcount = reinterpret_cast<int32_t*>(sb + o_vcount);
+    auto* tcount = reinterpret_cast<int32_t*>(sb + o_tcount);
+    auto* vbase = reinterpret_cast<long long*>(sb + o_vbase);
+    auto* tbase = reinterpret_cast<long long*>(sb + o_tbase);
+    auto* vsums = reinterpret_cast<long long*>(sb + o_vsums);
+    auto* tsums = reinterpret_cast<long long*>(sb + o_tsums);
+    const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
+    hipLaunchKernelGGL(weld_count_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vcount, tcount);
+    RECON_HIP_TRY(hipGetLastError(), "weld_count_kernel launch");
+    int rc = launch_exclusive_scan(vcount, w.nchunks, vbase, vsums, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(weld_row_count_kernel, dim3(rblocks), dim3(256), 0, s, p, rcount);
-    RECON_HIP_TRY(hipGetLastError(), "weld_row_count_kernel launch");
-    rc = launch_exclusive_scan(rcount, p.rows, roff, rsums, s);
+    rc = launch_exclusive_scan(tcount, w.nchunks, tbase, tsums, s);
     if (rc) return rc;
     long long totals[2] = {0, 0};
-    RECON_HIP_TRY(hipMemcpyAsync(&totals[0], csums + nbc, 8, hipMemcpyDeviceToHost, s), "vertex total");
-    RECON_HIP_TRY(hipMemcpyAsync(&totals[1], rsums + nbr, 8, hipMemcpyDeviceToHost, s), "triangle total");
+    RECON_HIP_TRY(hipMemcpyAsync(&totals[0], vsums + nb, 8, hipMemcpyDeviceToHost, s), "vertex total");
+    RECON_HIP_TRY(hipMemcpyAsync(&totals[1], tsums + nb, 8, hipMemcpyDeviceToHost, s), "triangle total");
     RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes counts");
     *nv_out = totals[0];
     *nt_out = totals[1];
@@ -285,15 +311,10 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     if (tris && totals[1] > tcap)
         return fail(RECON_ERR_CAPACITY, "%s: %lld triangles do not fit capacity %lld", fn, totals[1],
                     (long long)tcap);
-    if ((verts || normals) && totals[0] > 0) {
-        hipLaunchKernelGGL(weld_vertex_kernel, dim3(cblocks), dim3(256), 0, s, w, masks, base, verts,
-                           normals);
-        RECON_HIP_TRY(hipGetLastError(), "weld_vertex_kernel launch");
-    }
-    if (tris && totals[1] > 0) {
-        hipLaunchKernelGGL(weld_emit_kernel, dim3(rblocks), dim3(256), 0, s, w, rcount, roff, masks,
-                           base, tris);
-        RECON_HIP_TRY(hipGetLastError(), "weld_emit_kernel launch");
+    if (totals[0] > 0 || totals[1] > 0) {
+        hipLaunchKernelGGL(weld_write_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vbase, tcount,
+                           tbase, verts, normals, tris);
+        RECON_HIP_TRY(hipGetLastError(), "weld_write_kernel launch");
     }
     RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes");
     return RECON_OK;

@@ -240,31 +240,46 @@ class TSDFVolume:
         t = int(n.value)
         return verts[:t], (tris[:t] if tris is not None else None)
 
-    def extract_welded_mesh(self, with_normals=True, stream=None):
+    def count_welded(self, stream=None):
+        """(vertices, triangles) of the welded mesh."""
+        nv, nt = C.c_int64(), C.c_int64()
+        check(self._lib.recon_tsdf_extract_welded_mesh(self._h, None, None, 0, None, 0, C.byref(nv),
+                                                       C.byref(nt), _stream(stream)))
+        return int(nv.value), int(nt.value)
+
+    def extract_welded_mesh(self, with_normals=True, out=None, stream=None):
         """Welded mesh of the zero level set (recon_tsdf_extract_welded_mesh):
-        vertices (V, 3), normals (V, 3) or None, triangles (T, 3) int32."""
+        vertices (V, 3), normals (V, 3) or None, triangles (T, 3) int32.
+        `out` = (vertices, normals or None, triangles) buffers large enough
+        for the mesh make it one call (no counting call first)."""
         return _welded(lambda *a: self._lib.recon_tsdf_extract_welded_mesh(self._h, *a),
-                       with_normals, stream)
+                       with_normals, stream, out)
 
 
-def _welded(call, with_normals, stream):
+def _welded(call, with_normals, stream, out=None):
     torch = _torch()
     nv, nt = C.c_int64(), C.c_int64()
-    check(call(None, None, 0, None, 0, C.byref(nv), C.byref(nt), _stream(stream)))
+    if out is None:
+        check(call(None, None, 0, None, 0, C.byref(nv), C.byref(nt), _stream(stream)))
+        V, T = int(nv.value), int(nt.value)
+        verts = torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda")
+        normals = (torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda")
+                   if with_normals else None)
+        tris = torch.empty((max(T, 1), 3), dtype=torch.int32, device="cuda")
+    else:
+        verts, normals, tris = out
+    check(call(verts.data_ptr(), normals.data_ptr() if normals is not None else None,
+               verts.shape[0], tris.data_ptr(), tris.shape[0], C.byref(nv), C.byref(nt),
+               _stream(stream)))
     V, T = int(nv.value), int(nt.value)
-    verts = torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda")
-    normals = torch.empty((max(V, 1), 3), dtype=torch.float32, device="cuda") if with_normals else None
-    tris = torch.empty((max(T, 1), 3), dtype=torch.int32, device="cuda")
-    check(call(verts.data_ptr(), normals.data_ptr() if normals is not None else None, V,
-               tris.data_ptr(), T, C.byref(nv), C.byref(nt), _stream(stream)))
     return verts[:V], (normals[:V] if normals is not None else None), tris[:T]
 
 
-def mc_extract_welded(field, grid, voxel_size, offset, with_normals=True, stream=None):
+def mc_extract_welded(field, grid, voxel_size, offset, with_normals=True, out=None, stream=None):
     """Welded marching cubes of a dense field (recon_mc_extract_welded)."""
     g, vs, off = _grid_args(grid, voxel_size, offset)
     return _welded(lambda *a: ffi.load().recon_mc_extract_welded(field.data_ptr(), g, vs, off, *a),
-                   with_normals, stream)
+                   with_normals, stream, out)
 
 
 def _grid_args(grid, voxel_size, offset):

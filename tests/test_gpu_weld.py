@@ -45,6 +45,14 @@ def test_integrated_volume_and_handle(cuda):
     hv, hn, ht = tv.extract_welded_mesh()
     cuda.cuda.synchronize()
     assert cuda.equal(hv, v) and cuda.equal(ht, t)
+    # one call into preallocated, larger buffers
+    nv, nt = tv.count_welded()
+    out = (cuda.full((nv + 5, 3), 9.0, device="cuda"), cuda.full((nv + 5, 3), 9.0, device="cuda"),
+           cuda.full((nt + 7, 3), -5, dtype=cuda.int32, device="cuda"))
+    ov, on, ot = tv.extract_welded_mesh(out=out)
+    cuda.cuda.synchronize()
+    assert cuda.equal(ov, v) and cuda.equal(on, hn) and cuda.equal(ot, t)
+    assert (out[0][nv:] == 9.0).all() and (out[2][nt:] == -5).all()
     tv.close()
 
 
