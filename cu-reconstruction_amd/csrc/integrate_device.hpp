@@ -225,7 +225,14 @@ __device__ __forceinline__ void append_tile(uint64_t live, uint32_t tile, uint32
 // to back; D and W of a row are read only if one of its voxels passes, and
 // written once after all frames (frames are applied in order, so the result
 // equals one sweep per frame).
-template <class S, class Frames, bool VEC, bool COUNT>
+//
+// COLOR (single-frame calls with a colour image): a voxel updated inside the
+// truncation band (-trunc <= sdf <= trunc) also averages the rgb pixel of its
+// depth sample into its colour, c' = ((c * wc) + rgb) / (wc + 1) per channel,
+// wc' = min(wc + 1, max_weight) -- the running average of the octree's
+// add_observation (octree.cu:118-137) with the dense update's capped weight.
+// Free-space updates (sdf > trunc) leave the colour alone.
+template <class S, class Frames, bool VEC, bool COUNT, bool COLOR = false>
 __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int nframes,
                                                const Frames& frames, const uint64_t live,
                                                const int x0, const int y0, const int z0,
@@ -267,6 +274,7 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
         bool ok[NV];
         float pz[NV], dep[NV];
         int addr[NV];
+        int uu[COLOR ? NV : 1], vv[COLOR ? NV : 1];
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             const int r = j >> 2, i = j & 3;
@@ -279,10 +287,14 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
             const bool in_v = pixel_index(py, vp.fy, pz[j], vp.cy, vp.frows, v);
             ok[j] = in_z && in_u && in_v && row_ok[r] && (VEC || i < nvox);
             addr[j] = ok[j] ? v * F.pitch + u : 0;
+            if (COLOR) {
+                uu[j] = u;
+                vv[j] = v;
+            }
         }
 #pragma unroll
         for (int j = 0; j < NV; ++j) dep[j] = F.depth[addr[j]];
-        bool pass[NV], any[NR];
+        bool pass[NV], any[NR], band[COLOR ? NV : 1];
         float tsdf[NV];
 #pragma unroll
         for (int r = 0; r < NR; ++r) any[r] = false;
@@ -292,6 +304,7 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
             pass[j] = ok[j] && !(dep[j] <= 0.0f || dep[j] > vp.maxd) && (sdf >= -vp.trunc);
             tsdf[j] = sdf > 0.0f ? fminf(sdf, vp.trunc) : sdf;
             any[j >> 2] = any[j >> 2] || pass[j];
+            if (COLOR) band[j] = sdf <= vp.trunc;
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -319,6 +332,17 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
             w[j] = nw < vp.maxw ? nw : vp.maxw;
             dirty[j >> 2] = true;
             if (COUNT) ++cnt;
+            if (COLOR && band[j]) {
+                float4* cp = reinterpret_cast<float4*>(vp.C) + base[j >> 2] + (j & 3);
+                float4 c = *cp;
+                const uint8_t* px = F.rgb + (int64_t)vv[j] * F.rgb_pitch + 3 * uu[j];
+                const float nwc = c.w + 1.0f;
+                c.x = ((c.x * c.w) + (float)px[0]) / nwc;
+                c.y = ((c.y * c.w) + (float)px[1]) / nwc;
+                c.z = ((c.z * c.w) + (float)px[2]) / nwc;
+                c.w = nwc < vp.maxw ? nwc : vp.maxw;
+                *cp = c;
+            }
         }
     }
 #pragma unroll

@@ -67,6 +67,8 @@ struct FrameParams {
     const float* bound1;
     int32_t pitch;   // floats per depth row
     float zmargin;   // bound on float32 error of a camera z: 1e-6 * (|t| + max|c|)
+    const uint8_t* rgb = nullptr;  // colour image (rgb triples) of the colour integrate
+    int32_t rgb_pitch = 0;         // bytes per colour row
 };
 
 constexpr int kBound0 = 8, kBound1 = 32;  // pixel block edges of the two bound levels
@@ -83,6 +85,7 @@ struct VolumeParams {
     int32_t cols, rows;
     float fcols, frows;
     int32_t b0x, b0y, b1x, b1y;  // bound-level block counts
+    float* C = nullptr;          // colour volume, 4 floats per voxel (r, g, b, weight)
 };
 
 constexpr int kMaxFusedFrames = 16;
@@ -153,6 +156,7 @@ struct recon_tsdf {
     size_t diff_scratch_bytes = 0;
     uint32_t* owner = nullptr;  // per-voxel table of the adjoint's pair merge
     bool owner_dirty = false;   // set while a merge is being queued
+    float* C = nullptr;         // colour volume (recon_tsdf_enable_color), 16 B per voxel
 };
 
 namespace recon {

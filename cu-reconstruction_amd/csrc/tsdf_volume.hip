@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void cull_kernel(const VolumeParams vp, const 
 // Each wavefront takes list entries i = wave, wave + waves, ...; the grid is
 // sized to the resident wave slots, so every wave-tile costs about the same
 // and the device stays full until the list is done.
-template <int MAXF, bool VEC, bool COUNT>
+template <int MAXF, bool VEC, bool COUNT, bool COLOR = false>
 __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
     const VolumeParams vp, const int nframes, const FrameBlock<MAXF> fb,
     const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count,
@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
                                 __builtin_amdgcn_readfirstlane((uint32_t)list[i].frames);
         int x0, y0, z0;
         tile_origin<Shape>(vp, tile, x0, y0, z0);
-        integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT>(vp, nframes, fb, frames, x0, y0, z0, cnt);
+        integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR>(vp, nframes, fb, frames, x0, y0,
+                                                                   z0, cnt);
     }
     if (COUNT && cnt) atomicAdd(counter, (unsigned long long)cnt);
 }
@@ -90,12 +91,22 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ D, float*
     fill4(D, W, 4ll * (blockIdx.x * (long long)blockDim.x + threadIdx.x), n, value_d);
 }
 
+__global__ __launch_bounds__(256) void fill_color_kernel(float4* __restrict__ C, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) C[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 static int launch_fill(recon_tsdf* h, hipStream_t s) {
     const long long n = h->info.num_voxels;
     const long long blocks = (n + 4 * 256 - 1) / (4 * 256);
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, h->D, h->W, n,
                        h->info.truncation);
     RECON_HIP_TRY(hipGetLastError(), "fill_kernel launch");
+    if (h->C) {
+        hipLaunchKernelGGL(fill_color_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<float4*>(h->C), n);
+        RECON_HIP_TRY(hipGetLastError(), "fill_color_kernel launch");
+    }
     return RECON_OK;
 }
 
@@ -174,6 +185,18 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     const bool vec = (vp.gx % 4) == 0;
     unsigned long long* ctr = h->update_counter;
     const dim3 grid(h->num_cus * h->integrate_blocks_per_cu);
+    if constexpr (MAXF == 1) {
+        if (fb.f[0].rgb && vp.C) {  // colour integrate (no update counting)
+            if (vec)
+                hipLaunchKernelGGL((integrate_list_kernel<1, true, false, true>), grid, dim3(kThreads),
+                                   0, s, vp, nframes, fb, list, h->list_count, ctr);
+            else
+                hipLaunchKernelGGL((integrate_list_kernel<1, false, false, true>), grid,
+                                   dim3(kThreads), 0, s, vp, nframes, fb, list, h->list_count, ctr);
+            RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
+            return RECON_OK;
+        }
+    }
     if (ctr) {
         if (vec)
             hipLaunchKernelGGL((integrate_list_kernel<MAXF, true, true>), grid, dim3(kThreads), 0,
@@ -296,6 +319,7 @@ extern "C" void recon_tsdf_destroy(recon_tsdf* h) {
     if (h->list) (void)hipFree(h->list);
     if (h->diff_scratch) (void)hipFree(h->diff_scratch);
     if (h->owner) (void)hipFree(h->owner);
+    if (h->C) (void)hipFree(h->C);
     delete h;
 }
 
@@ -351,8 +375,6 @@ extern "C" int recon_tsdf_set_update_counter(recon_tsdf* h, unsigned long long* 
 extern "C" int recon_tsdf_integrate(recon_tsdf* h, const recon_depth_view* depth,
                                     const uint8_t* rgb, size_t rgb_step,
                                     const double camera_pose[7], recon_stream_t stream) {
-    (void)rgb;
-    (void)rgb_step;
     clear_error();
     if (!h) return fail(RECON_ERR_NULL, "recon_tsdf_integrate: handle is NULL");
     int rc = check_depth(depth, "recon_tsdf_integrate");
@@ -360,11 +382,56 @@ extern "C" int recon_tsdf_integrate(recon_tsdf* h, const recon_depth_view* depth
     if (!camera_pose) return fail(RECON_ERR_NULL, "recon_tsdf_integrate: pose is NULL");
     if (!pose_is_finite(camera_pose))
         return fail(RECON_ERR_CONFIG, "recon_tsdf_integrate: pose has a non-finite entry");
+    const bool color = h->C && rgb;  // the image is ignored unless colour is enabled
+    if (color && (rgb_step < 3 * (size_t)depth->cols || rgb_step * (size_t)depth->rows >= (size_t(1) << 31)))
+        return fail(RECON_ERR_SIZE, "recon_tsdf_integrate: rgb step %zu invalid", rgb_step);
     const VolumeParams vp = volume_params(h, depth->cols, depth->rows);
     FrameBlock<1> fb;
     const int pitch = (int32_t)(depth->step / 4);
     make_frame_params(vp, camera_pose, depth->data, pitch, &fb.f[0]);
+    if (color) {
+        fb.f[0].rgb = rgb;
+        fb.f[0].rgb_pitch = (int32_t)rgb_step;
+    }
     return launch_integrate<1>(h, vp, depth->data, 0, pitch, 1, fb, (hipStream_t)stream);
+}
+
+extern "C" int recon_tsdf_enable_color(recon_tsdf* h, int enable, recon_stream_t stream) {
+    clear_error();
+    if (!h) return fail(RECON_ERR_NULL, "recon_tsdf_enable_color: handle is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    if (!enable) {
+        if (h->C) {
+            RECON_HIP_TRY(hipStreamSynchronize(s), "colour release synchronize");
+            (void)hipFree(h->C);
+        }
+        h->C = nullptr;
+        h->vp.C = nullptr;
+        return RECON_OK;
+    }
+    if (h->C) return RECON_OK;
+    int rc = require_gfx950();
+    if (rc) return rc;
+    const long long n = h->info.num_voxels;
+    RECON_HIP_TRY(hipMalloc((void**)&h->C, 16 * (size_t)n), "colour volume");
+    h->vp.C = h->C;
+    hipLaunchKernelGGL(fill_color_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<float4*>(h->C), n);
+    RECON_HIP_TRY(hipGetLastError(), "fill_color_kernel launch");
+    return RECON_OK;
+}
+
+extern "C" float* recon_tsdf_colors(recon_tsdf* h) { return h ? h->C : nullptr; }
+
+extern "C" int recon_tsdf_download_colors(recon_tsdf* h, float* host, recon_stream_t stream) {
+    clear_error();
+    if (!h || !host) return fail(RECON_ERR_NULL, "recon_tsdf_download_colors: NULL argument");
+    if (!h->C) return fail(RECON_ERR_CONFIG, "recon_tsdf_download_colors: colour is not enabled");
+    hipStream_t s = (hipStream_t)stream;
+    RECON_HIP_TRY(hipMemcpyAsync(host, h->C, 16 * (size_t)h->info.num_voxels, hipMemcpyDeviceToHost, s),
+                  "download colours");
+    RECON_HIP_TRY(hipStreamSynchronize(s), "recon_tsdf_download_colors: synchronize");
+    return RECON_OK;
 }
 
 extern "C" int recon_tsdf_integrate_frames(recon_tsdf* h, const recon_depth_view* depth0,

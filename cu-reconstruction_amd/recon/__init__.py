@@ -209,6 +209,25 @@ class TSDFVolume:
         check(self._lib.recon_tsdf_download(self._h, D.ctypes.data, W.ctypes.data, _stream(stream)))
         return D, W
 
+    def enable_color(self, enable=True, stream=None):
+        """Allocates (or frees) the colour volume (recon_tsdf_enable_color)."""
+        check(self._lib.recon_tsdf_enable_color(self._h, 1 if enable else 0, _stream(stream)))
+
+    def download_colors(self, stream=None):
+        """(gz, gy, gx, 4) float32: r, g, b, colour weight."""
+        out = np.empty(self.shape + (4,), np.float32)
+        check(self._lib.recon_tsdf_download_colors(self._h, out.ctypes.data, _stream(stream)))
+        return out
+
+    def sample_colors(self, points, stream=None):
+        """Colours (N, 3) uint8 at (N, 3) float32 device points."""
+        torch = _torch()
+        pts = points.reshape(-1, 3).contiguous()
+        out = torch.empty((pts.shape[0], 3), dtype=torch.uint8, device=pts.device)
+        check(self._lib.recon_tsdf_sample_colors(self._h, pts.data_ptr(), pts.shape[0],
+                                                 out.data_ptr(), _stream(stream)))
+        return out
+
     def upload(self, D=None, W=None, stream=None):
         Dp = np.ascontiguousarray(D, np.float32) if D is not None else None
         Wp = np.ascontiguousarray(W, np.float32) if W is not None else None

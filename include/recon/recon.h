@@ -125,8 +125,12 @@ RECON_API int recon_tsdf_upload(recon_tsdf* h, const float* host_d, const float*
 /* ---- integrate (TSDFVolume::integrate, tsdf_volume.cu:144-190,257-290) -- */
 /* async.  One depth frame (metres, float32, device memory) seen from
  * camera_pose = {qw,qx,qy,qz,tx,ty,tz}, world->camera, narrowed to float32
- * as the reference does (tsdf_volume.cu:271-272).  `rgb` (pitched uchar3) is
- * accepted and ignored, as in the reference; it may be NULL. */
+ * as the reference does (tsdf_volume.cu:271-272).  `rgb` (pitched uchar3, the
+ * depth's size) may be NULL; it is ignored, as in the reference, unless
+ * colour is enabled on the volume (recon_tsdf_enable_color): then a voxel
+ * updated inside the truncation band (-trunc <= sdf <= trunc) averages the
+ * rgb pixel of its depth sample into its colour, c' = ((c*wc) + rgb)/(wc+1)
+ * per channel, wc' = min(wc+1, max_weight). */
 RECON_API int recon_tsdf_integrate(recon_tsdf* h, const recon_depth_view* depth,
                          const uint8_t* rgb, size_t rgb_step,
                          const double camera_pose[7], recon_stream_t stream);
@@ -169,6 +173,25 @@ RECON_API int recon_mc_extract(const float* field, const int32_t grid[3], const 
  * extraction is built on. */
 RECON_API int recon_mc_class_histogram(const float* field, const int32_t grid[3],
                              unsigned long long* hist_device, recon_stream_t stream);
+
+/* ---- colour ----------------------------------------------------------------- */
+/* No counterpart on the reference's dense path (its integrate ignores colour;
+ * its octree keeps a colour average per node, octree.cu:118-137).  enable != 0
+ * allocates (once) a colour volume of 4 floats per voxel (r, g, b, weight),
+ * zeroed, which later recon_tsdf_integrate calls with an rgb image update and
+ * recon_tsdf_reset clears; enable == 0 frees it.  async (enable), sync
+ * (disable). */
+RECON_API int recon_tsdf_enable_color(recon_tsdf* h, int enable, recon_stream_t stream);
+RECON_API float* recon_tsdf_colors(recon_tsdf* h);  /* device pointer, 4N floats, or NULL */
+/* sync.  Copies the colour volume (4N floats) to host memory. */
+RECON_API int recon_tsdf_download_colors(recon_tsdf* h, float* host, recon_stream_t stream);
+/* async.  Colour at `num_points` world points (3 floats each, device): the
+ * trilinear support of the TSDF sampler, each neighbour weighted by its
+ * coefficient times its colour weight, c = sum(k*w*c) / sum(k*w) in
+ * neighbour order, rounded to 0..255 (0,0,0 without colour or support);
+ * 3 bytes per point into colors_out (device). */
+RECON_API int recon_tsdf_sample_colors(recon_tsdf* h, const float* points, int64_t num_points,
+                                       uint8_t* colors_out, recon_stream_t stream);
 
 /* ---- welded mesh ---------------------------------------------------------- */
 /* sync.  No counterpart in the reference, whose extraction emits a triangle

@@ -346,3 +346,70 @@ void oracle_stable_points(const oracle_volume* vol, const float* depth, int32_t 
         weights[k] = ok ? 1.0 : 0.0;
     }
 }
+
+/* Colour at points (float32, the device sampler's operation order): the
+ * support of tri_support -- q = p - off with the BOUNDARY_EPS clamps,
+ * vox = floor(q / vs), lower = vox - 1 when q lies below the voxel's centre,
+ * at least 0, t = (q - centre(lower)) / vs -- and neighbours weighted by
+ * coef * wc, c = sum(coef*wc*c) / sum(coef*wc) accumulated in neighbour
+ * order, rounded to nearest (rintf) and clamped to 0..255. */
+void oracle_sample_colors_f32(const oracle_volume* vol, const float* C, const float* points,
+                              int64_t npoints, uint8_t* out) {
+    const int gx = vol->grid[0], gy = vol->grid[1];
+    for (int64_t k = 0; k < npoints; ++k) {
+        const float* p = points + 3 * k;
+        int lo[3];
+        float t[3];
+        int ok = 1;
+        for (int a = 0; a < 3 && ok; ++a) {
+            const float vs = vol->voxel_size[a];
+            float q = p[a] - vol->offset[a];
+            const float bound = (float)vol->grid[a] * vs;
+            if (q < -1e-4f) q = 0.0f;
+            if ((q > bound) && (q - bound < 1e-4f)) q = bound - 1e-4f;
+            if (!(q <= bound)) { ok = 0; break; }
+            const int vox = (int)floorf(q / vs);
+            const float ct = ((float)vox + 0.5f) * vs;
+            int l = q < ct ? vox - 1 : vox;
+            if (l < 0) l = 0;
+            if (!(l + 1 <= vol->grid[a] - 1)) { ok = 0; break; }
+            lo[a] = l;
+            const float lct = ((float)l + 0.5f) * vs;
+            t[a] = (q - lct) / vs;
+        }
+        float r = 0.0f, g = 0.0f, b = 0.0f, wsum = 0.0f;
+        if (ok) {
+            const int64_t dx = 1, dy = gx, dz = (int64_t)gx * gy;
+            int64_t ids[8];
+            ids[0] = lo[0] + (int64_t)gx * (lo[1] + (int64_t)gy * lo[2]);
+            ids[1] = ids[0] + dx;
+            ids[2] = ids[1] + dz;
+            ids[3] = ids[0] + dz;
+            ids[4] = ids[0] + dy;
+            ids[5] = ids[1] + dy;
+            ids[6] = ids[2] + dy;
+            ids[7] = ids[3] + dy;
+            const float x = t[0], y = t[1], z = t[2];
+            const float coef[8] = {(1 - x) * (1 - y) * (1 - z), x * (1 - y) * (1 - z),
+                                   x * (1 - y) * z,             (1 - x) * (1 - y) * z,
+                                   (1 - x) * y * (1 - z),       x * y * (1 - z),
+                                   x * y * z,                   (1 - x) * y * z};
+            for (int i = 0; i < 8; ++i) {
+                const float* c = C + 4 * ids[i];
+                const float a = coef[i] * c[3];
+                r = r + a * c[0];
+                g = g + a * c[1];
+                b = b + a * c[2];
+                wsum = wsum + a;
+            }
+        }
+        uint8_t q[3] = {0, 0, 0};
+        if (wsum > 0.0f) {
+            const float v[3] = {r / wsum, g / wsum, b / wsum};
+            for (int i = 0; i < 3; ++i) q[i] = (uint8_t)fminf(fmaxf(rintf(v[i]), 0.0f), 255.0f);
+        }
+        out[3 * k] = q[0];
+        out[3 * k + 1] = q[1];
+        out[3 * k + 2] = q[2];
+    }
+}

@@ -104,6 +104,19 @@ void oracle_depth_to_points(const float* depth, int32_t rows, int32_t cols, int6
                             const uint8_t* rgb, int64_t rgb_pitch,
                             float* xyz, uint8_t* valid, uint8_t* colors_out);
 
+/* The colour integrate: oracle_tsdf_integrate_f32 plus, for voxels updated
+ * with -trunc <= sdf <= trunc, c = ((c*wc) + rgb)/(wc+1) per channel and
+ * wc = min(wc+1, max_weight) in C (4 floats per voxel).  No reference
+ * counterpart on the dense path (octree.cu:118-137 averages colour per node). */
+int64_t oracle_tsdf_integrate_color_f32(const oracle_volume* vol, float* D, float* W, float* C,
+                                        const float* depth, int32_t rows, int32_t cols,
+                                        int64_t pitch, const uint8_t* rgb, int64_t rgb_pitch,
+                                        const double pose[7], int nthreads);
+/* Colour at float32 points from C: the device sampler's support and
+ * coefficients, neighbours weighted by coef * wc, rounded to 0..255. */
+void oracle_sample_colors_f32(const oracle_volume* vol, const float* C, const float* points,
+                              int64_t npoints, uint8_t* out);
+
 /* Welded marching cubes (no reference counterpart; the reference emits a
  * soup).  One vertex per grid edge whose end values differ in sign, numbered
  * by (owning lower-end voxel, axis x, y, z); the vertex runs from the negative

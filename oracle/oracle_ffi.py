@@ -56,6 +56,9 @@ def lib() -> C.CDLL:
             "oracle_render_points": (None, [vp, vp, vp, i64, vp, C.c_float, C.c_float, C.c_float,
                                             C.c_float, i32, i32, i64, vp, vp, i64]),
             "oracle_mc_weld": (i64, [vp, vp, vp, vp, vp, vp, i64, vp, i64, vp]),
+            "oracle_tsdf_integrate_color_f32": (i64, [vp, vp, vp, vp, vp, i32, i32, i64, vp, i64,
+                                                      vp, C.c_int]),
+            "oracle_sample_colors_f32": (None, [vp, vp, vp, i64, vp]),
         })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -179,6 +182,25 @@ def depth_to_points(depth, intrinsics, transform, rgb=None):
                                  _p(pose_arr(transform)), _p(rgbp), cols * 3, _p(xyz), _p(valid),
                                  _p(colors))
     return xyz, valid, colors
+
+
+def integrate_color_f32(vol, D, W, Cv, depth, rgb, pose, nthreads=0):
+    """Colour integrate: D, W (gz, gy, gx) float32 and Cv (gz, gy, gx, 4)
+    float32 updated in place; rgb (rows, cols, 3) uint8."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    rows, cols = depth.shape
+    return lib().oracle_tsdf_integrate_color_f32(C.byref(vol), _p(D), _p(W), _p(Cv), _p(depth),
+                                                 rows, cols, cols, _p(rgb), 3 * cols,
+                                                 _p(pose_arr(pose)), nthreads)
+
+
+def sample_colors_f32(vol, Cv, points):
+    points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.empty((points.shape[0], 3), np.uint8)
+    lib().oracle_sample_colors_f32(C.byref(vol), _p(np.ascontiguousarray(Cv, np.float32)),
+                                   _p(points), points.shape[0], _p(out))
+    return out
 
 
 def mc_weld(field, grid, voxel_size, offset):

@@ -58,10 +58,36 @@ static inline int pixel_index(float pa, float f, float pz, float c, int32_t size
     return 1;
 }
 
+static int64_t integrate_f32_impl(const oracle_volume* vol, float* D, float* W, float* C,
+                                  const float* depth, int32_t rows, int32_t cols,
+                                  int64_t pitch, const uint8_t* rgb, int64_t rgb_pitch,
+                                  const double pose[7], uint8_t* update_mask, int nthreads);
+
 int64_t oracle_tsdf_integrate_f32(const oracle_volume* vol, float* D, float* W,
                                   const float* depth, int32_t rows, int32_t cols,
                                   int64_t pitch, const double pose[7],
                                   uint8_t* update_mask, int nthreads) {
+    return integrate_f32_impl(vol, D, W, NULL, depth, rows, cols, pitch, NULL, 0, pose, update_mask,
+                              nthreads);
+}
+
+/* The colour integrate (no reference counterpart on the dense path, whose
+ * integrate ignores colour; the octree's add_observation, octree.cu:118-137,
+ * keeps a weighted colour average): C holds (r, g, b, wc) per voxel; a voxel
+ * updated with -trunc <= sdf <= trunc also sets
+ * c = ((c * wc) + rgb[v][u]) / (wc + 1) per channel, wc = min(wc + 1, maxw). */
+int64_t oracle_tsdf_integrate_color_f32(const oracle_volume* vol, float* D, float* W, float* C,
+                                        const float* depth, int32_t rows, int32_t cols,
+                                        int64_t pitch, const uint8_t* rgb, int64_t rgb_pitch,
+                                        const double pose[7], int nthreads) {
+    return integrate_f32_impl(vol, D, W, C, depth, rows, cols, pitch, rgb, rgb_pitch, pose, NULL,
+                              nthreads);
+}
+
+static int64_t integrate_f32_impl(const oracle_volume* vol, float* D, float* W, float* C,
+                                  const float* depth, int32_t rows, int32_t cols,
+                                  int64_t pitch, const uint8_t* rgb, int64_t rgb_pitch,
+                                  const double pose[7], uint8_t* update_mask, int nthreads) {
     float R[9], t[3];
     oracle_pose_matrix_f32(pose, R, t);
     const int32_t gx = vol->grid[0], gy = vol->grid[1], gz = vol->grid[2];
@@ -100,6 +126,13 @@ int64_t oracle_tsdf_integrate_f32(const oracle_volume* vol, float* D, float* W,
             const float new_d = ((dv * w) + tsdf) / new_w; /* tsdf_volume.cu:183 */
             W[idx] = new_w < maxw ? new_w : maxw;
             D[idx] = new_d;
+            if (C && rgb && sdf <= trunc) {
+                float* c4 = C + 4 * idx;
+                const uint8_t* px = rgb + (int64_t)v * rgb_pitch + 3 * (int64_t)u;
+                const float wc = c4[3], nwc = wc + 1.0f;
+                for (int k = 0; k < 3; ++k) c4[k] = ((c4[k] * wc) + (float)px[k]) / nwc;
+                c4[3] = nwc < maxw ? nwc : maxw;
+            }
             if (update_mask) update_mask[idx] = 1;
             total += 1;
         }

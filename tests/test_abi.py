@@ -226,3 +226,11 @@ def test_welded_arguments_rejected_before_any_launch():
                                        C.byref(nt), None) == ffi.ERR_SIZE
     assert lib.recon_tsdf_extract_welded_mesh(None, None, None, 0, None, 0, C.byref(nv), C.byref(nt),
                                               None) == ffi.ERR_NULL
+
+
+def test_colour_arguments_rejected_before_any_launch():
+    lib = ffi.load()
+    assert lib.recon_tsdf_enable_color(None, 1, None) == ffi.ERR_NULL
+    assert lib.recon_tsdf_colors(None) is None
+    assert lib.recon_tsdf_download_colors(None, C.c_void_p(4096), None) == ffi.ERR_NULL
+    assert lib.recon_tsdf_sample_colors(None, C.c_void_p(4096), 1, C.c_void_p(4096), None) == ffi.ERR_NULL
