@@ -22,6 +22,21 @@ RECON_API int recon_png_read_gray(const char* path, int32_t* width, int32_t* hei
     return RECON_OK;
 }
 
+RECON_API int recon_png_read_rgb(const char* path, int32_t* width, int32_t* height,
+                                 uint8_t* pixels, int64_t capacity) {
+    if (!path || !width || !height) return RECON_ERR_NULL;
+    int w = 0, h = 0;
+    std::vector<uint8_t> rgb;
+    std::string err;
+    if (!cuphoto::read_rgb_png(path, &w, &h, &rgb, &err)) return RECON_ERR_CONFIG;
+    *width = w;
+    *height = h;
+    if (!pixels) return RECON_OK;
+    if ((int64_t)rgb.size() > capacity) return RECON_ERR_CAPACITY;
+    std::memcpy(pixels, rgb.data(), rgb.size());
+    return RECON_OK;
+}
+
 // Value of `key` in parameter-file text (into buf, NUL-terminated);
 // RECON_ERR_RANGE when absent.
 RECON_API int recon_config_lookup(const char* text, const char* key, char* buf, int64_t capacity) {

@@ -1,8 +1,9 @@
 // png_reader.cpp -- decoding of the depth images of an RGB-D directory
 // (the reference reads them with cv::imread(..., IMREAD_ANYDEPTH),
-// examples/common/rgbd_dataset.cpp:26).  Grey PNGs of 8 or 16 bits, not
-// interlaced: chunks, zlib inflate of the IDAT stream (zlib), and the five
-// scanline filters of the PNG specification.
+// examples/common/rgbd_dataset.cpp:26) and, in colour mode, of the colour
+// images.  Grey or RGB PNGs of 8 or 16 bits, not interlaced: chunks, zlib
+// inflate of the IDAT stream (zlib), and the five scanline filters of the
+// PNG specification.
 #include "recon/tsdf_surface.hpp"
 
 #include <zlib.h>
@@ -34,7 +35,27 @@ bool fail(std::string* error, const std::string& what) {
 
 }  // namespace
 
+static bool read_png(const std::string& path, GrayImage* out, std::string* error);
+
 bool read_gray_png(const std::string& path, GrayImage* out, std::string* error) {
+    if (!read_png(path, out, error)) return false;
+    if (out->channels != 1) return fail(error, path + ": not a grey PNG");
+    return true;
+}
+
+bool read_rgb_png(const std::string& path, int* width, int* height, std::vector<uint8_t>* rgb,
+                  std::string* error) {
+    GrayImage img;
+    if (!read_png(path, &img, error)) return false;
+    if (img.channels != 3 || img.bit_depth != 8) return fail(error, path + ": not an 8-bit RGB PNG");
+    *width = img.width;
+    *height = img.height;
+    rgb->resize(img.pixels.size());
+    for (size_t i = 0; i < img.pixels.size(); ++i) (*rgb)[i] = (uint8_t)img.pixels[i];
+    return true;
+}
+
+static bool read_png(const std::string& path, GrayImage* out, std::string* error) {
     std::ifstream f(path, std::ios::binary);
     if (!f.is_open()) return fail(error, "cannot open " + path);
     const std::vector<unsigned char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
@@ -65,11 +86,12 @@ bool read_gray_png(const std::string& path, GrayImage* out, std::string* error) 
         pos += 12 + len;
     }
     if (!have_header) return fail(error, path + ": no IHDR");
-    if (ctype != 0 || (depth != 8 && depth != 16) || interlace != 0)
-        return fail(error, path + ": only non-interlaced grey PNGs of 8 or 16 bits are read");
+    if ((ctype != 0 && ctype != 2) || (depth != 8 && depth != 16) || interlace != 0)
+        return fail(error, path + ": only non-interlaced grey or RGB PNGs of 8 or 16 bits are read");
     if (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t(1) << 28))
         return fail(error, path + ": bad size");
-    const size_t bpp = depth / 8;  // bytes per pixel
+    const int channels = ctype == 2 ? 3 : 1;
+    const size_t bpp = (size_t)channels * (depth / 8);  // bytes per pixel
     const size_t stride = (size_t)width * bpp;
     std::vector<unsigned char> raw((stride + 1) * (size_t)height);
     uLongf raw_len = (uLongf)raw.size();
@@ -80,7 +102,9 @@ bool read_gray_png(const std::string& path, GrayImage* out, std::string* error) 
     out->width = width;
     out->height = height;
     out->bit_depth = depth;
-    out->pixels.assign((size_t)width * height, 0);
+    out->channels = channels;
+    const size_t samples = (size_t)width * channels;  // per row
+    out->pixels.assign(samples * height, 0);
     for (int y = 0; y < height; ++y) {
         const unsigned char* line = &raw[(size_t)y * (stride + 1)];
         const int filter = line[0];
@@ -98,8 +122,8 @@ bool read_gray_png(const std::string& path, GrayImage* out, std::string* error) 
             }
             cur[i] = (unsigned char)v;
         }
-        uint16_t* row = &out->pixels[(size_t)y * width];
-        for (int x = 0; x < width; ++x)
+        uint16_t* row = &out->pixels[(size_t)y * samples];
+        for (size_t x = 0; x < samples; ++x)
             row[x] = depth == 16 ? (uint16_t)((cur[2 * x] << 8) | cur[2 * x + 1]) : cur[x];
         prev.swap(cur);
     }

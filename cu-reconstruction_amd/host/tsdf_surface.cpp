@@ -58,6 +58,7 @@ TSDFSurface::TSDFSurface(const Config& cfg) {
     cfg_.truncation = cfg.get<float>("tsdf.volume.truncation", 0.f);
     cfg_.max_depth = cfg.get<float>("tsdf.volume.max_depth", 0.f);
     welded_ = cfg.get<int>("tsdf.mesh.welded", 0) != 0;
+    color_ = cfg.get<int>("dataset.color.mode", 0) != 0;
     global_pose = euler_pose(cfg.get<float>("tsdf.volume.camera.pose.euler.roll", 0.f),
                              cfg.get<float>("tsdf.volume.camera.pose.euler.yaw", 0.f),
                              cfg.get<float>("tsdf.volume.camera.pose.euler.pitch", 0.f),
@@ -72,6 +73,8 @@ bool TSDFSurface::reconstruct(const PtrStepSz<float>& depth, const unsigned char
                               size_t rgb_step) {
     TSDFVolume volume(cfg_);
     if (!volume.ok()) return false;
+    const bool colored = color_ && rgb;
+    if (colored && !volume.enable_color()) return false;
     PtrStepSz<unsigned char> color{const_cast<unsigned char*>(rgb), rgb_step, depth.rows, depth.cols};
     if (!volume.integrate(color, depth, global_pose)) return false;
     std::vector<float3> verts, normals;
@@ -80,8 +83,30 @@ bool TSDFSurface::reconstruct(const PtrStepSz<float>& depth, const unsigned char
         generate_welded_surface(volume, verts, normals, triangles);
     else
         generate_triangular_surface(volume, verts, triangles);
+    std::vector<uchar3> colors;
+    if (colored && !verts.empty()) {
+        // vertex colours from the colour volume (recon_tsdf_sample_colors)
+        float* dp = nullptr;
+        uint8_t* dc = nullptr;
+        bool ok = hipMalloc((void**)&dp, 12 * verts.size()) == hipSuccess &&
+                  hipMalloc((void**)&dc, 3 * verts.size()) == hipSuccess &&
+                  hipMemcpy(dp, verts.data(), 12 * verts.size(), hipMemcpyHostToDevice) == hipSuccess &&
+                  volume.sample_colors(dp, verts.size(), dc);
+        std::vector<uint8_t> host(3 * verts.size());
+        ok = ok && hipMemcpy(host.data(), dc, host.size(), hipMemcpyDeviceToHost) == hipSuccess;
+        if (dp) (void)hipFree(dp);
+        if (dc) (void)hipFree(dc);
+        if (!ok) {
+            std::fprintf(stderr, "[cuphoto] TSDFSurface::reconstruct_surface() -- vertex colours failed\n");
+            return false;
+        }
+        colors.resize(verts.size());
+        for (size_t i = 0; i < verts.size(); ++i)
+            colors[i] = make_uchar3(host[3 * i], host[3 * i + 1], host[3 * i + 2]);
+    }
     surface_mesh.vertices = std::move(verts);
     surface_mesh.normals = std::move(normals);
+    surface_mesh.colors = std::move(colors);
     surface_mesh.polygons = std::move(triangles);
     return true;
 }
@@ -107,7 +132,8 @@ bool TSDFSurface::reconstruct_surface(PointCloud& cloud) {
 }
 
 bool Mesh::store_to_ply(const std::string& ply_filepath) const {
-    if (!normals.empty()) return write_mesh_to_ply(ply_filepath, vertices, normals, polygons);
+    if (!normals.empty() || !colors.empty())
+        return write_mesh_to_ply(ply_filepath, vertices, normals, colors, polygons);
     return write_mesh_to_ply(ply_filepath, vertices, polygons);
 }
 
@@ -133,15 +159,41 @@ RGBDDataset::RGBDDataset(const std::string& data_path, double scale) : depth_sca
 
 RGBDDataset::~RGBDDataset() {
     if (device_depth) (void)hipFree(device_depth);
+    if (device_color) (void)hipFree(device_color);
 }
 
-bool RGBDDataset::get_next(PtrStepSz<float>* depth) {
+bool RGBDDataset::get_next(PtrStepSz<float>* depth, PtrStepSz<unsigned char>* color) {
     if (next + 1 >= files.size()) {
         std::fprintf(stderr, "[cuphoto] all files in the directory were traversed\n");
         return false;
     }
+    const std::string& color_file = files[next];
     const std::string& depth_file = files[next + 1];  // the colour file comes first
     next += 2;
+    if (color) {
+        int cw = 0, ch = 0;
+        std::vector<uint8_t> rgb;
+        std::string cerr;
+        if (!read_rgb_png(color_file, &cw, &ch, &rgb, &cerr)) {
+            std::fprintf(stderr, "[cuphoto] %s\n", cerr.c_str());
+            return false;
+        }
+        if (rgb.size() > device_color_bytes) {
+            if (device_color) (void)hipFree(device_color);
+            device_color = nullptr;
+            device_color_bytes = 0;
+            if (hipMalloc((void**)&device_color, rgb.size()) != hipSuccess) {
+                std::fprintf(stderr, "[cuphoto] out of device memory for a colour frame\n");
+                return false;
+            }
+            device_color_bytes = rgb.size();
+        }
+        if (hipMemcpy(device_color, rgb.data(), rgb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            std::fprintf(stderr, "[cuphoto] colour upload failed\n");
+            return false;
+        }
+        *color = PtrStepSz<unsigned char>{device_color, 3 * (size_t)cw, ch, cw};
+    }
     GrayImage img;
     std::string err;
     if (!read_gray_png(depth_file, &img, &err)) {

@@ -28,11 +28,20 @@ int main(int argc, char* argv[]) {
     if (!cfg.set_parameter_file(config_path)) return EXIT_FAILURE;
     cuphoto::RGBDDataset dataset(cfg.get<std::string>("rgbd.dataset.path"),
                                  (double)cfg.get<float>("depth_scale"));
+    // dataset.color.mode: 1 reads the colour image too (an 8-bit RGB PNG of
+    // the depth's size), integrates colour and colours the mesh vertices
+    const bool color_mode = cfg.get<int>("dataset.color.mode", 0) != 0;
     cuphoto::PtrStepSz<float> depth;
-    if (!dataset.get_next(&depth)) return EXIT_FAILURE;
+    cuphoto::PtrStepSz<unsigned char> color;
+    if (!dataset.get_next(&depth, color_mode ? &color : nullptr)) return EXIT_FAILURE;
+    if (color_mode && (color.rows != depth.rows || color.cols != depth.cols)) {
+        std::fprintf(stderr, "[cuphoto] colour and depth images differ in size\n");
+        return EXIT_FAILURE;
+    }
     const std::string store_path = cfg.get<std::string>("mesh_store.path");
     cuphoto::TSDFSurface tsdf(cfg);
-    if (!tsdf.reconstruct_surface(depth)) {
+    const bool done = color_mode ? tsdf.reconstruct_surface(color, depth) : tsdf.reconstruct_surface(depth);
+    if (!done) {
         std::fprintf(stderr, "[cuphoto] reconstruction failed\n");
         return EXIT_FAILURE;
     }

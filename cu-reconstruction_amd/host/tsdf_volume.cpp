@@ -75,6 +75,22 @@ bool TSDFVolume::integrate_raw(const uint8_t* rgb, size_t rgb_step, const float*
     return sync(stream_, "TSDFVolume::integrate()");
 }
 
+bool TSDFVolume::enable_color(bool enable) {
+    if (!h_ || recon_tsdf_enable_color(h_, enable ? 1 : 0, stream_) != RECON_OK) {
+        log_error("TSDFVolume::enable_color()");
+        return false;
+    }
+    return sync(stream_, "TSDFVolume::enable_color()");
+}
+
+bool TSDFVolume::sample_colors(const float* points, size_t n, uint8_t* colors) const {
+    if (!h_ || recon_tsdf_sample_colors(h_, points, (int64_t)n, colors, stream_) != RECON_OK) {
+        log_error("TSDFVolume::sample_colors()");
+        return false;
+    }
+    return sync(stream_, "TSDFVolume::sample_colors()");
+}
+
 bool TSDFVolume::integrate_frames(const float* depth0, size_t step, int rows, int cols,
                                   size_t frame_stride_bytes,
                                   const std::vector<std::array<r64, 7>>& poses) {
@@ -317,16 +333,29 @@ namespace cuphoto {
 bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
                        const std::vector<float3>& normals, const std::vector<int3>& triangles) {
     if (normals.size() != vertices.size()) return false;
+    return write_mesh_to_ply(file_name, vertices, normals, std::vector<uchar3>(), triangles);
+}
+
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<float3>& normals, const std::vector<uchar3>& colors,
+                       const std::vector<int3>& triangles) {
+    const bool hn = !normals.empty(), hc = !colors.empty();
+    if ((hn && normals.size() != vertices.size()) || (hc && colors.size() != vertices.size()))
+        return false;
     std::ofstream of(file_name);
     if (!of.is_open()) return false;
     of << "ply\nformat ascii 1.0\nelement vertex " << vertices.size()
-       << "\nproperty float x\nproperty float y\nproperty float z"
-          "\nproperty float nx\nproperty float ny\nproperty float nz"
-          "\nelement face " << triangles.size()
+       << "\nproperty float x\nproperty float y\nproperty float z";
+    if (hn) of << "\nproperty float nx\nproperty float ny\nproperty float nz";
+    if (hc) of << "\nproperty uchar red\nproperty uchar green\nproperty uchar blue";
+    of << "\nelement face " << triangles.size()
        << "\nproperty list uchar int vertex_indices\nend_header\n";
-    for (size_t i = 0; i < vertices.size(); ++i)
-        of << vertices[i].x << " " << vertices[i].y << " " << vertices[i].z << " " << normals[i].x
-           << " " << normals[i].y << " " << normals[i].z << "\n";
+    for (size_t i = 0; i < vertices.size(); ++i) {
+        of << vertices[i].x << " " << vertices[i].y << " " << vertices[i].z;
+        if (hn) of << " " << normals[i].x << " " << normals[i].y << " " << normals[i].z;
+        if (hc) of << " " << (int)colors[i].x << " " << (int)colors[i].y << " " << (int)colors[i].z;
+        of << "\n";
+    }
     for (const int3& t : triangles) of << "3 " << t.x << " " << t.y << " " << t.z << "\n";
     return (bool)of;
 }

@@ -47,14 +47,18 @@ template <> float Config::get<float>(const std::string& key, float def) const;
 template <> int Config::get<int>(const std::string& key, int def) const;
 template <> std::string Config::get<std::string>(const std::string& key, std::string def) const;
 
-// Decoded grey image (8- or 16-bit samples widened to uint16).
+// Decoded image (8- or 16-bit samples widened to uint16, channels
+// interleaved).
 struct GrayImage {
-    int width = 0, height = 0, bit_depth = 0;
+    int width = 0, height = 0, bit_depth = 0, channels = 1;
     std::vector<uint16_t> pixels;  // row-major
 };
 // Decodes a non-interlaced grey PNG (colour type 0, 8 or 16 bit).  Returns
 // false (with a message in *error) for anything else.
 bool read_gray_png(const std::string& path, GrayImage* out, std::string* error);
+// Decodes a non-interlaced 8-bit RGB PNG (colour type 2) into rgb bytes.
+bool read_rgb_png(const std::string& path, int* width, int* height, std::vector<uint8_t>* rgb,
+                  std::string* error);
 
 // Image files of a directory (extensions .jpg .jpeg .png .tiff), sorted by
 // canonical path (utils.cpp:122-145).
@@ -66,8 +70,10 @@ class RGBDDataset {
 public:
     RGBDDataset(const std::string& data_path, double depth_scale);
     ~RGBDDataset();
-    // false when the files are exhausted or the depth file cannot be read
-    bool get_next(PtrStepSz<float>* depth);
+    // false when the files are exhausted or the depth file cannot be read;
+    // with `color`, the colour file (an 8-bit RGB PNG of the depth's size)
+    // is decoded too, into device memory (3 bytes per pixel)
+    bool get_next(PtrStepSz<float>* depth, PtrStepSz<unsigned char>* color = nullptr);
     int num_files() const { return (int)files.size() / 2; }
 
 private:
@@ -76,11 +82,14 @@ private:
     double depth_scale;
     float* device_depth = nullptr;
     size_t device_bytes = 0;
+    unsigned char* device_color = nullptr;
+    size_t device_color_bytes = 0;
 };
 
 struct Mesh {
     std::vector<float3> vertices;
     std::vector<float3> normals;  // empty, or one per vertex (welded meshes)
+    std::vector<uchar3> colors;   // empty, or one per vertex (colour mode)
     std::vector<int3> polygons;
     // with normals when there are any (mesh.cpp:59-64 switches the same way)
     bool store_to_ply(const std::string& ply_filepath) const;
@@ -115,6 +124,7 @@ private:
     bool reconstruct(const PtrStepSz<float>& depth, const unsigned char* rgb, size_t rgb_step);
     TSDFVolumeConfig cfg_;
     bool welded_ = false;  // parameter tsdf.mesh.welded (this library's addition)
+    bool color_ = false;   // parameter dataset.color.mode: integrate colour, colour the mesh
     std::array<double, 7> global_pose{1, 0, 0, 0, 0, 0, 0};
     Mesh surface_mesh;
 };
@@ -133,6 +143,10 @@ extern "C" {
  * NULL to query the size). */
 RECON_API int recon_png_read_gray(const char* path, int32_t* width, int32_t* height,
                                   int32_t* bit_depth, uint16_t* pixels, int64_t capacity);
+/* Decodes an 8-bit RGB PNG into pixels (3 bytes each; capacity bytes; may be
+ * NULL to query the size). */
+RECON_API int recon_png_read_rgb(const char* path, int32_t* width, int32_t* height,
+                                 uint8_t* pixels, int64_t capacity);
 /* The value of `key` in parameter-file text. */
 RECON_API int recon_config_lookup(const char* text, const char* key, char* buf, int64_t capacity);
 /* euler_pose() into pose[7]. */

@@ -72,7 +72,8 @@ public:
     TSDFVolume(const TSDFVolume&) = delete;
     TSDFVolume& operator=(const TSDFVolume&) = delete;
 
-    // tsdf_volume.cu:257-290; `color` is accepted and ignored.
+    // tsdf_volume.cu:257-290; `color` is ignored, as in the reference, unless
+    // enable_color() was called (then band voxels average it).
     template <typename Color, typename Depth>
     bool integrate(const Color& color, const Depth& depth, const std::array<r64, 7>& camera_pose) {
         return integrate_raw(reinterpret_cast<const uint8_t*>(color.data), color.step,
@@ -93,6 +94,11 @@ public:
 
     bool ok() const { return h_ != nullptr; }
     void reset();
+    // No counterpart in the reference's dense volume: a colour average per
+    // voxel (r, g, b, weight) fed by integrate() calls with a colour image.
+    bool enable_color(bool enable = true);
+    // Colour (rgb bytes) at n device points (3 floats each) into device memory.
+    bool sample_colors(const float* points, size_t n, uint8_t* colors) const;
     recon_tsdf* handle() const { return h_; }
     hipStream_t stream() const { return stream_; }
 
@@ -194,5 +200,10 @@ bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& 
 // The same with per-vertex normals (property float nx, ny, nz after x, y, z).
 bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
                        const std::vector<float3>& normals, const std::vector<int3>& triangles);
+// The same with optional per-vertex normals and colours (either vector may be
+// empty; property uchar red, green, blue after the normals).
+bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,
+                       const std::vector<float3>& normals, const std::vector<uchar3>& colors,
+                       const std::vector<int3>& triangles);
 
 }  // namespace cuphoto

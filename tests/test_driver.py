@@ -24,6 +24,9 @@ def host():
     lib.recon_png_read_gray.restype = C.c_int
     lib.recon_png_read_gray.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32), C.c_void_p, C.c_int64]
+    lib.recon_png_read_rgb.restype = C.c_int
+    lib.recon_png_read_rgb.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.c_void_p, C.c_int64]
     lib.recon_config_lookup.restype = C.c_int
     lib.recon_config_lookup.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]
     lib.recon_euler_pose.restype = None
@@ -115,6 +118,25 @@ def test_png_written_by_pil(host, tmp_path):
     assert st == ffi.ERR_CONFIG
     (tmp_path / "x.png").write_bytes(b"not a png")
     assert read_png(host, tmp_path / "x.png")[0] == ffi.ERR_CONFIG
+
+
+def test_rgb_png_written_by_pil(host, tmp_path):
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 256, (29, 41, 3)).astype(np.uint8)
+    img[5:20, 5:30] = 77
+    Image.fromarray(img, "RGB").save(tmp_path / "c.png")
+    w, h = C.c_int32(), C.c_int32()
+    assert host.recon_png_read_rgb(str(tmp_path / "c.png").encode(), C.byref(w), C.byref(h), None, 0) == 0
+    out = np.empty((h.value, w.value, 3), np.uint8)
+    assert host.recon_png_read_rgb(str(tmp_path / "c.png").encode(), C.byref(w), C.byref(h),
+                                   out.ctypes.data, out.size) == 0
+    assert (out == img).all()
+    assert host.recon_png_read_rgb(str(tmp_path / "c.png").encode(), C.byref(w), C.byref(h),
+                                   out.ctypes.data, out.size - 1) == ffi.ERR_CAPACITY
+    # a grey image is not a colour image
+    Image.fromarray(img[..., 0]).save(tmp_path / "g.png")
+    assert host.recon_png_read_rgb(str(tmp_path / "g.png").encode(), C.byref(w), C.byref(h), None,
+                                   0) == ffi.ERR_CONFIG
 
 
 PARAMS = """%YAML:1.0

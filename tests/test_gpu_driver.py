@@ -2,7 +2,8 @@
 reference's examples/tsdf_surface_reconstruction/tsdf_surface.cpp):
 parameter file -> first RGB-D pair of a directory -> integrate -> marching
 cubes -> ASCII PLY (a triangle soup, or with tsdf.mesh.welded: 1 shared
-vertices with normals).  The PLY must equal, character for character, the
+vertices with normals; with dataset.color.mode: 1 the colour PNG is
+integrated too and the vertices carry its colour).  The PLY must equal, character for character, the
 one the CPU oracle's integrate and marching cubes give for the same decoded
 depth and the parameter file's pose."""
 import ctypes as C
@@ -76,33 +77,41 @@ def _euler_pose():
     return np.array(out[:])
 
 
-def _expected_ply(raw, pose, welded=False):
+def _expected_ply(raw, pose, welded=False, rgb=None):
     scale = np.float64(np.float32(SCALE))      # read as float, widened
     depth = (raw.astype(np.float64) * scale).astype(np.float32)
     cfg = recon.make_config(GRID, physical_size=PHYS, offset=OFFSET, max_weight=15.0,
                             intrinsics=INTR, image=(640, 480))
     vol = O.volume_from_config(cfg)
     D, W = O.init(vol)
-    O.integrate_f32(vol, D, W, depth, pose)
+    if rgb is None:
+        O.integrate_f32(vol, D, W, depth, pose)
+    else:
+        Cv = np.zeros(D.shape + (4,), np.float32)
+        O.integrate_color_f32(vol, D, W, Cv, depth, rgb, pose)
+    props = ["property float x", "property float y", "property float z"]
     if welded:
         v, nrm, tri = O.mc_weld(D, GRID, list(vol.voxel_size), list(vol.offset))
-        props = ["property float x", "property float y", "property float z", "property float nx",
-                 "property float ny", "property float nz"]
+        props += ["property float nx", "property float ny", "property float nz"]
         rows = ["%g %g %g %g %g %g" % tuple(float(q) for q in (*a, *b)) for a, b in zip(v, nrm)]
         faces = ["3 %d %d %d" % tuple(int(q) for q in f) for f in tri]
     else:
         verts, _, _, _ = O.mc_extract(D, GRID, list(vol.voxel_size), list(vol.offset))
         v = verts.reshape(-1, 3)
-        props = ["property float x", "property float y", "property float z"]
         rows = ["%g %g %g" % (float(a), float(b), float(c)) for a, b, c in v]
         faces = [f"3 {3 * t} {3 * t + 1} {3 * t + 2}" for t in range(v.shape[0] // 3)]
+    if rgb is not None:
+        props += ["property uchar red", "property uchar green", "property uchar blue"]
+        col = O.sample_colors_f32(vol, Cv, v)
+        rows = [r + " %d %d %d" % tuple(int(q) for q in c) for r, c in zip(rows, col)]
     lines = ["ply", "format ascii 1.0", f"element vertex {v.shape[0]}", *props,
              f"element face {len(faces)}", "property list uchar int vertex_indices", "end_header"]
     return "\n".join(lines + rows + faces) + "\n", len(faces)
 
 
-@pytest.mark.parametrize("welded", [False, True])
-def test_driver_ply_equals_oracle(cuda, tmp_path, welded):
+@pytest.mark.parametrize("welded,colour", [(False, False), (True, False), (False, True),
+                                           (True, True)])
+def test_driver_ply_equals_oracle(cuda, tmp_path, welded, colour):
     assert EXE.exists(), "build the libraries first"
     pose = _euler_pose()
     R, t = scenes.pose_matrix(pose)
@@ -118,17 +127,21 @@ def test_driver_ply_equals_oracle(cuda, tmp_path, welded):
     raw = np.clip(np.rint(depth / SCALE), 0, 65535).astype(np.uint16)
     data = tmp_path / "scene"
     data.mkdir()
-    Image.fromarray(np.zeros((480, 640, 3), np.uint8)).save(data / "000_color.png")
+    rgb = np.random.default_rng(17).integers(0, 256, (480, 640, 3)).astype(np.uint8)
+    Image.fromarray(rgb, "RGB").save(data / "000_color.png")
     Image.fromarray(raw).save(data / "000_depth.png")
     mesh = tmp_path / "mesh.ply"
     cfg = tmp_path / "params.yaml"
-    cfg.write_text(PARAMS.format(data=data, mesh=mesh) + ("tsdf.mesh.welded: 1\n" if welded else ""))
+    text = PARAMS.format(data=data, mesh=mesh) + ("tsdf.mesh.welded: 1\n" if welded else "")
+    if colour:
+        text = text.replace("dataset.color.mode: 0", "dataset.color.mode: 1")
+    cfg.write_text(text)
     print("tsdf_surface driver", flush=True)
     r = subprocess.run([str(EXE), f"--config_path={cfg}"], capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
-    expected, ntri = _expected_ply(raw, pose, welded)
+    expected, ntri = _expected_ply(raw, pose, welded, rgb if colour else None)
     assert ntri > 20000
     got = mesh.read_text()
     if got != expected:
