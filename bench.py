@@ -579,6 +579,33 @@ def main() -> int:
         "welded_bytes_vs_soup": round((24.0 * wv.shape[0] + 12.0 * wt.shape[0]) /
                                       (36.0 * wt.shape[0] + 12.0 * wt.shape[0]), 3)}
     del wv, wt, wbuf
+    # colour integrate: the first 30 frames, per-frame calls, with random
+    # colour images, against the same calls without colour (HIP events)
+    ncol = min(30, F)
+    rng = np.random.default_rng(11)
+    rgbs = torch.from_numpy(rng.integers(0, 256, (ncol, w.image[1], w.image[0], 3),
+                                         dtype=np.uint8)).cuda()
+
+    def frames_ms(colour):
+        vol.reset(stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for f in range(ncol):
+            vol.integrate(dev[f], w.poses[f], stream, rgb=rgbs[f] if colour else None)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / ncol
+
+    vol.enable_color(True, stream)
+    frames_ms(True)
+    colour_ms = frames_ms(True)
+    plain_ms = frames_ms(False)
+    vol.enable_color(False, stream)
+    del rgbs
+    result["colour"] = {"frames": ncol, "ms_per_call_colour": round(colour_ms, 4),
+                        "ms_per_call_plain": round(plain_ms, 4),
+                        "colour_over_plain": round(colour_ms / plain_ms, 3),
+                        "colour_volume_bytes": 16 * w.num_voxels}
     vol.close()
     del dev
     torch.cuda.empty_cache()
