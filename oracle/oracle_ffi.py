@@ -59,6 +59,13 @@ def lib() -> C.CDLL:
             "oracle_tsdf_integrate_color_f32": (i64, [vp, vp, vp, vp, vp, i32, i32, i64, vp, i64,
                                                       vp, C.c_int]),
             "oracle_sample_colors_f32": (None, [vp, vp, vp, i64, vp]),
+            "oracle_sparse_create": (vp, [vp]),
+            "oracle_sparse_destroy": (None, [vp]),
+            "oracle_sparse_allocate_box": (None, [vp, vp, vp]),
+            "oracle_sparse_integrate": (i64, [vp, vp, i32, i32, i64, vp]),
+            "oracle_sparse_num_bricks": (i64, [vp]),
+            "oracle_sparse_export": (None, [vp, vp, vp, vp, vp]),
+            "oracle_sparse_bricks": (i64, [vp, vp, vp, vp, i64]),
         })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -69,7 +76,10 @@ def lib() -> C.CDLL:
 
 
 def _p(a: np.ndarray | None):
-    return None if a is None else a.ctypes.data
+    # a ctypes pointer that keeps `a` alive until the call has read it (a bare
+    # address would let a temporary array be freed, and its memory reused,
+    # before the C function runs)
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def volume(grid, voxel_size, offset, truncation, max_weight, intrinsics, max_depth=24.0):
@@ -201,6 +211,49 @@ def sample_colors_f32(vol, Cv, points):
     lib().oracle_sample_colors_f32(C.byref(vol), _p(np.ascontiguousarray(Cv, np.float32)),
                                    _p(points), points.shape[0], _p(out))
     return out
+
+
+class Sparse:
+    """The sparse (brick) volume oracle over `vol`'s lattice."""
+
+    def __init__(self, vol):
+        self.vol = vol
+        self._h = lib().oracle_sparse_create(C.byref(vol))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_sparse_destroy(self._h)
+            self._h = None
+
+    def allocate_box(self, lo, hi):
+        lib().oracle_sparse_allocate_box(self._h, _p(np.array(lo, np.int32)),
+                                         _p(np.array(hi, np.int32)))
+
+    def integrate(self, depth, pose):
+        depth = np.ascontiguousarray(depth, np.float32)
+        rows, cols = depth.shape
+        return lib().oracle_sparse_integrate(self._h, _p(depth), rows, cols, cols,
+                                             _p(pose_arr(pose)))
+
+    def num_bricks(self):
+        return lib().oracle_sparse_num_bricks(self._h)
+
+    def export(self, origin, size):
+        """Dense (sz, sy, sx) D and W of the box [origin, origin + size)."""
+        D = np.empty(tuple(size[::-1]), np.float32)
+        W = np.empty(tuple(size[::-1]), np.float32)
+        lib().oracle_sparse_export(self._h, _p(np.array(origin, np.int32)),
+                                   _p(np.array(size, np.int32)), _p(D), _p(W))
+        return D, W
+
+    def bricks(self):
+        """{(bx, by, bz): (D, W)} with D, W (8, 8, 8) in [z][y][x] order."""
+        n = self.num_bricks()
+        c = np.empty((max(n, 1), 3), np.int32)
+        D = np.empty((max(n, 1), 8, 8, 8), np.float32)
+        W = np.empty((max(n, 1), 8, 8, 8), np.float32)
+        lib().oracle_sparse_bricks(self._h, _p(c), _p(D), _p(W), n)
+        return {tuple(int(q) for q in c[i]): (D[i], W[i]) for i in range(n)}
 
 
 def mc_weld(field, grid, voxel_size, offset):
