@@ -275,6 +275,66 @@ class TSDFVolume:
                        with_normals, stream, out)
 
 
+class SparseVolume:
+    """Sparse brick volume over the lattice of a dense config
+    (recon_sparse_*): unbounded, 8x8x8-voxel bricks allocated along each
+    frame's truncation band, the dense update rule voxel for voxel."""
+
+    def __init__(self, cfg, brick_capacity):
+        _torch()
+        self._lib = ffi.load()
+        h = C.c_void_p()
+        check(self._lib.recon_sparse_create(C.byref(cfg), int(brick_capacity), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.recon_sparse_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def reset(self, stream=None):
+        check(self._lib.recon_sparse_reset(self._h, _stream(stream)))
+
+    def integrate(self, depth, pose, stream=None):
+        dv = depth_view(depth)
+        check(self._lib.recon_sparse_integrate(self._h, C.byref(dv), ffi.pose_array(pose),
+                                               _stream(stream)))
+
+    def allocate_box(self, lo, hi, stream=None):
+        check(self._lib.recon_sparse_allocate_box(self._h, ffi.i32x3(lo), ffi.i32x3(hi),
+                                                  _stream(stream)))
+
+    def stats(self, stream=None):
+        """(allocated bricks, overflow flags)."""
+        n, f = C.c_int64(), C.c_uint32()
+        check(self._lib.recon_sparse_stats(self._h, C.byref(n), C.byref(f), _stream(stream)))
+        return int(n.value), int(f.value)
+
+    def export(self, origin, size, stream=None):
+        """Dense (sz, sy, sx) D and W device tensors of the voxel box."""
+        torch = _torch()
+        D = torch.empty(tuple(size[::-1]), dtype=torch.float32, device="cuda")
+        W = torch.empty(tuple(size[::-1]), dtype=torch.float32, device="cuda")
+        check(self._lib.recon_sparse_export(self._h, ffi.i32x3(origin), ffi.i32x3(size),
+                                            D.data_ptr(), W.data_ptr(), _stream(stream)))
+        return D, W
+
+    def bricks(self, stream=None):
+        """{(bx, by, bz): (D, W)} host arrays (8, 8, 8) in [z][y][x] order."""
+        n, _ = self.stats(stream)
+        c = np.empty((max(n, 1), 3), np.int32)
+        D = np.empty((max(n, 1), 8, 8, 8), np.float32)
+        W = np.empty((max(n, 1), 8, 8, 8), np.float32)
+        got = C.c_int64()
+        check(self._lib.recon_sparse_download_bricks(self._h, c.ctypes.data, D.ctypes.data,
+                                                     W.ctypes.data, n, C.byref(got),
+                                                     _stream(stream)))
+        return {tuple(int(q) for q in c[i]): (D[i], W[i]) for i in range(int(got.value))}
+
+
 def _welded(call, with_normals, stream, out=None):
     torch = _torch()
     nv, nt = C.c_int64(), C.c_int64()

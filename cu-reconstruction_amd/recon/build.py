@@ -42,7 +42,7 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{IN
 # host-side pieces of the drop-in beside tsdf_volume.cpp (librecon_host.so)
 HOST_SOURCES = ["config.cpp", "png_reader.cpp", "tsdf_surface.cpp", "driver_capi.cpp"]
 KERNEL_SOURCES = ["recon_common.hip", "tsdf_volume.hip", "marching_cubes.hip", "depth_points.hip",
-                  "tsdf_batch.hip", "tsdf_diff.hip", "diff_batch.hip", "render_points.hip", "mc_weld.hip", "color.hip"]
+                  "tsdf_batch.hip", "tsdf_diff.hip", "diff_batch.hip", "render_points.hip", "mc_weld.hip", "color.hip", "sparse.hip"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

@@ -115,6 +115,14 @@ SIGNATURES = {
     "recon_tsdf_colors": (C.c_void_p, [_vp]),
     "recon_tsdf_download_colors": (C.c_int, [_vp, _vp, _vp]),
     "recon_tsdf_sample_colors": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "recon_sparse_create": (C.c_int, [C.POINTER(TsdfConfig), _i64, C.POINTER(_vp)]),
+    "recon_sparse_destroy": (None, [_vp]),
+    "recon_sparse_reset": (C.c_int, [_vp, _vp]),
+    "recon_sparse_integrate": (C.c_int, [_vp, C.POINTER(DepthView), _d7, _vp]),
+    "recon_sparse_allocate_box": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32), _vp]),
+    "recon_sparse_stats": (C.c_int, [_vp, _pi64, C.POINTER(C.c_uint32), _vp]),
+    "recon_sparse_export": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32), _vp, _vp, _vp]),
+    "recon_sparse_download_bricks": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _pi64, _vp]),
     "recon_mc_extract_welded": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
                                           C.POINTER(C.c_float), _vp, _vp, _i64, _vp, _i64, _pi64,
                                           _pi64, _vp]),

@@ -193,6 +193,48 @@ RECON_API int recon_tsdf_download_colors(recon_tsdf* h, float* host, recon_strea
 RECON_API int recon_tsdf_sample_colors(recon_tsdf* h, const float* points, int64_t num_points,
                                        uint8_t* colors_out, recon_stream_t stream);
 
+/* ---- sparse volume --------------------------------------------------------- */
+/* No counterpart on the reference's path (a dense grid, tsdf_volume.cu:194-254;
+ * octree.cu is an unfinished sparse prototype).  The dense volume's voxel
+ * lattice -- voxel v (int32 triple, |v| < 2^23) centred at ((v+0.5)*vs)+off,
+ * vs = physical_size / voxel_grid_size of `cfg` -- stored in 8x8x8-voxel
+ * bricks, brick b = floor(v/8), found through a hash table; at most
+ * `brick_capacity` (1..2^24) bricks, 4 KiB each.  Unallocated voxels read as
+ * D = truncation, W = 0.  The grid size of `cfg` sets the voxel size only:
+ * the sparse volume is unbounded. */
+typedef struct recon_sparse recon_sparse;
+RECON_API int recon_sparse_create(const recon_tsdf_config* cfg, int64_t brick_capacity,
+                                  recon_sparse** out);               /* sync */
+RECON_API void recon_sparse_destroy(recon_sparse* h);
+/* async.  Frees every brick. */
+RECON_API int recon_sparse_reset(recon_sparse* h, recon_stream_t stream);
+/* async.  One depth frame: allocates every brick that meets, one voxel wide,
+ * the world box of some valid pixel's frustum slab between d - trunc and
+ * d + trunc, then applies the dense update rule of recon_tsdf_integrate to
+ * every voxel of every allocated brick.  Voxels the dense integrate updates
+ * inside the band always lie in allocated bricks; with every brick already
+ * allocated the result equals the dense integrate bit for bit. */
+RECON_API int recon_sparse_integrate(recon_sparse* h, const recon_depth_view* depth,
+                                     const double camera_pose[7], recon_stream_t stream);
+/* async.  Allocates every brick that meets the voxel box [lo, hi). */
+RECON_API int recon_sparse_allocate_box(recon_sparse* h, const int32_t lo[3], const int32_t hi[3],
+                                        recon_stream_t stream);
+/* sync.  Allocated bricks and overflow flags (bit 0: the pool filled and
+ * bricks were dropped, bit 1: a pixel slab spanned more than 8 bricks on an
+ * axis and was cut, bit 2: a slab left the |v| < 2^23 range). */
+RECON_API int recon_sparse_stats(recon_sparse* h, int64_t* num_bricks, uint32_t* overflow,
+                                 recon_stream_t stream);
+/* async.  Dense box of voxels [origin, origin + size) into device arrays
+ * (x fastest; D and/or W may be NULL) -- e.g. for marching cubes. */
+RECON_API int recon_sparse_export(recon_sparse* h, const int32_t origin[3], const int32_t size[3],
+                                  float* D, float* W, recon_stream_t stream);
+/* sync.  Brick coordinates (3 int32 each) and contents (512 floats each,
+ * voxel (x, y, z) of a brick at 64z + 8y + x) to host memory, in pool order
+ * (the allocation order, which may vary between runs). */
+RECON_API int recon_sparse_download_bricks(recon_sparse* h, int32_t* coords, float* D, float* W,
+                                           int64_t capacity, int64_t* num_bricks,
+                                           recon_stream_t stream);
+
 /* ---- welded mesh ---------------------------------------------------------- */
 /* sync.  No counterpart in the reference, whose extraction emits a triangle
  * soup (marching_cubes.cu:171-179 never shares a vertex): the same surface

@@ -117,6 +117,20 @@ int64_t oracle_tsdf_integrate_color_f32(const oracle_volume* vol, float* D, floa
 void oracle_sample_colors_f32(const oracle_volume* vol, const float* C, const float* points,
                               int64_t npoints, uint8_t* out);
 
+/* Sparse (brick) volume: the dense lattice and update over 8x8x8-voxel
+ * bricks allocated along each frame's truncation band (sparse_oracle.c). */
+typedef struct oracle_sparse oracle_sparse;
+oracle_sparse* oracle_sparse_create(const oracle_volume* vol);
+void oracle_sparse_destroy(oracle_sparse* s);
+void oracle_sparse_allocate_box(oracle_sparse* s, const int32_t lo[3], const int32_t hi[3]);
+int64_t oracle_sparse_integrate(oracle_sparse* s, const float* depth, int32_t rows, int32_t cols,
+                                int64_t pitch, const double pose[7]);
+int64_t oracle_sparse_num_bricks(const oracle_sparse* s);
+void oracle_sparse_export(const oracle_sparse* s, const int32_t origin[3], const int32_t size[3],
+                          float* D, float* W);
+int64_t oracle_sparse_bricks(const oracle_sparse* s, int32_t* coords, float* D, float* W,
+                             int64_t capacity);
+
 /* Welded marching cubes (no reference counterpart; the reference emits a
  * soup).  One vertex per grid edge whose end values differ in sign, numbered
  * by (owning lower-end voxel, axis x, y, z); the vertex runs from the negative
