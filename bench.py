@@ -566,6 +566,7 @@ def main() -> int:
         torch.cuda.synchronize()
         return 1000.0 * (time.perf_counter() - t0) / reps, out
 
+    log("timing the meshing of the final volume")
     soup_ms, _ = timed(lambda: vol.extract_mesh(verts, tris, stream))
     nv, nt = vol.count_welded(stream)
     wbuf = (torch.empty((nv, 3), dtype=torch.float32, device="cuda"),
@@ -596,12 +597,41 @@ def main() -> int:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / ncol
 
+    log("timing the colour integrate")
     vol.enable_color(True, stream)
     frames_ms(True)
     colour_ms = frames_ms(True)
     plain_ms = frames_ms(False)
     vol.enable_color(False, stream)
     del rgbs
+    # sparse brick volume on the same lattice and frames: per-frame calls
+    # (allocation + integrate), bricks and memory against the dense volume;
+    # the mesh of the exported grid box against the dense mesh
+    log("timing the sparse volume")
+    sv = recon.SparseVolume(w.config(), 1 << 20)
+    sv.set_bounds((0, 0, 0), w.grid)        # the dense grid's region
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for f in range(F):
+        sv.integrate(dev[f], w.poses[f], stream)
+        if f % 50 == 49:
+            torch.cuda.synchronize()
+            log(f"sparse: {f + 1} frames, {sv.stats(stream)[0]} bricks")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sparse_ms = e0.elapsed_time(e1) / F
+    nb, flags = sv.stats(stream)
+    Ds, _ = sv.export((0, 0, 0), w.grid, stream)
+    sverts, _ = recon.mc_extract(Ds, w.grid, list(vol.info.voxel_size), list(vol.info.global_offset),
+                                 stream=stream)
+    torch.cuda.synchronize()
+    result["sparse"] = {"frames": F, "ms_per_call": round(sparse_ms, 4),
+                        "dense_ms_per_call": round(avg_launch_ms, 4), "bricks": nb,
+                        "overflow_flags": flags, "brick_bytes": nb * 4096,
+                        "dense_bytes": 8 * w.num_voxels,
+                        "mesh_triangles_grid_box": int(sverts.shape[0])}
+    del Ds, sverts
+    sv.close()
     result["colour"] = {"frames": ncol, "ms_per_call_colour": round(colour_ms, 4),
                         "ms_per_call_plain": round(plain_ms, 4),
                         "colour_over_plain": round(colour_ms / plain_ms, 3),

@@ -41,6 +41,12 @@ constexpr int kBrickVoxels = kBrick * kBrick * kBrick;
 constexpr unsigned long long kEmptySlot = ~0ull;
 constexpr int kMaxBricksPerAxis = 8;             // per pixel slab; more sets the overflow flag
 constexpr int kCoordBits = 21;                   // brick coordinate range [-2^20, 2^20)
+// Linear probing stops after kMaxProbes slots, on insertion and lookup
+// alike, so a key is always within kMaxProbes of its hash.  The table has
+// twice the pool's slots and stops taking keys when the pool is full, so it
+// stays at most about half full and the bound is almost never reached; when
+// it is, the brick is dropped and flagged.
+constexpr int kMaxProbes = 128;
 
 struct SparseParams {
     float vs[3], off[3];
@@ -55,7 +61,9 @@ struct SparseParams {
     float* W;
     unsigned* counter;                // bricks handed out (may exceed capacity)
     unsigned* first_new;              // counter at the start of the call
-    unsigned* overflow;               // bit 0: pool full, bit 1: slab too wide, bit 2: coordinate range
+    unsigned* overflow;               // bit 0: pool full, bit 1: slab too wide, bit 2: coordinate
+                                      // range, bit 3: probe limit
+    int32_t blo[3], bhi[3];           // allocation bounds in bricks (inclusive)
 };
 
 __host__ __device__ inline unsigned long long brick_key(int bx, int by, int bz) {
@@ -77,11 +85,11 @@ __host__ __device__ inline uint64_t hash_key(unsigned long long k) {
 __device__ __forceinline__ int floor_div8(int v) { return v >= 0 ? v >> 3 : -((-v + 7) >> 3); }
 
 
-// Brick index of (bx, by, bz), or -1.  Probes at most the whole table.
+// Brick index of (bx, by, bz), or -1.
 __device__ __forceinline__ int32_t find_brick(const SparseParams& p, int bx, int by, int bz) {
     const unsigned long long key = brick_key(bx, by, bz);
     uint64_t h = hash_key(key) & p.mask;
-    for (uint64_t n = 0; n <= p.mask; ++n) {
+    for (uint64_t n = 0; n < (uint64_t)kMaxProbes && n <= p.mask; ++n) {
         const unsigned long long k = p.keys[h];
         if (k == key) return p.vals[h];
         if (k == kEmptySlot) return -1;
@@ -93,10 +101,15 @@ __device__ __forceinline__ int32_t find_brick(const SparseParams& p, int bx, int
 __device__ __forceinline__ void insert_brick(const SparseParams& p, int bx, int by, int bz) {
     const unsigned long long key = brick_key(bx, by, bz);
     uint64_t h = hash_key(key) & p.mask;
-    for (uint64_t n = 0; n <= p.mask; ++n) {
+    for (uint64_t n = 0; n < (uint64_t)kMaxProbes && n <= p.mask; ++n) {
         unsigned long long k = p.keys[h];
         if (k == key) return;
         if (k == kEmptySlot) {
+            // a full pool takes no more keys (a racing few may still pass)
+            if ((int64_t)*p.counter >= p.capacity) {
+                atomicOr(p.overflow, 1u);
+                return;
+            }
             k = atomicCAS(p.keys + h, kEmptySlot, key);
             if (k == kEmptySlot) {
                 const unsigned idx = atomicAdd(p.counter, 1u);
@@ -113,7 +126,7 @@ __device__ __forceinline__ void insert_brick(const SparseParams& p, int bx, int 
         }
         h = (h + 1) & p.mask;
     }
-    atomicOr(p.overflow, 1u);
+    atomicOr(p.overflow, 8u);
 }
 
 __global__ void sparse_mark_kernel(const SparseParams p) {
@@ -160,6 +173,9 @@ __global__ __launch_bounds__(256) void sparse_alloc_kernel(const SparseParams p,
             atomicOr(p.overflow, 2u);
             b1[k] = b0[k] + kMaxBricksPerAxis - 1;
         }
+        // allocation bounds (recon_sparse_set_bounds)
+        b0[k] = max(b0[k], p.blo[k]);
+        b1[k] = min(b1[k], p.bhi[k]);
     }
     for (int bz = b0[2]; bz <= b1[2]; ++bz)
         for (int by = b0[1]; by <= b1[1]; ++by)
@@ -177,6 +193,47 @@ __global__ __launch_bounds__(256) void sparse_init_kernel(const SparseParams p) 
     }
 }
 
+// Conservative visibility of a brick for one frame (wave-uniform result).
+// Lanes 0..7 take the brick's corner voxel centres to the camera; a voxel
+// can only pass the per-voxel tests when its camera point p satisfies
+//   z > 0,  fx*x + (cx+1)*z > 0,  (cols-cx)*z - fx*x > 0,
+//   fy*y + (cy+1)*z > 0,  (rows-cy)*z - fy*y > 0,  z <= maxd + trunc,
+// all linear in p, so when every corner violates one of them by more than a
+// relative 1e-3 every voxel centre of the brick (inside the corners' hull)
+// does too, and the brick is skipped.
+__device__ __forceinline__ bool brick_may_update(const SparseParams& p, const FrameParams& F,
+                                                 const int4 bc, float fcols, float frows, int lane) {
+    unsigned out = 0;  // bit k: this corner violates constraint k
+    if (lane < 8) {
+        const int v[3] = {bc.x * kBrick + ((lane & 1) ? kBrick - 1 : 0),
+                          bc.y * kBrick + ((lane & 2) ? kBrick - 1 : 0),
+                          bc.z * kBrick + ((lane & 4) ? kBrick - 1 : 0)};
+        float c[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[k] = voxel_centre(v[k], p.vs[k], p.off[k]);
+        float q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            q[k] = ((F.R[3 * k] * c[0] + F.R[3 * k + 1] * c[1]) + F.R[3 * k + 2] * c[2]) + F.t[k];
+        const float x = q[0], y = q[1], z = q[2];
+        const float ax = fabsf(p.fx * x), ay = fabsf(p.fy * y), az = fabsf(z);
+        const float e = 1e-3f;
+        if (z < -e * (az + 1e-6f)) out |= 1u;
+        if (p.fx * x + (p.cx + 1.0f) * z < -e * (ax + fabsf(p.cx + 1.0f) * az + 1e-6f)) out |= 2u;
+        if ((fcols - p.cx) * z - p.fx * x < -e * (ax + fabsf(fcols - p.cx) * az + 1e-6f)) out |= 4u;
+        if (p.fy * y + (p.cy + 1.0f) * z < -e * (ay + fabsf(p.cy + 1.0f) * az + 1e-6f)) out |= 8u;
+        if ((frows - p.cy) * z - p.fy * y < -e * (ay + fabsf(frows - p.cy) * az + 1e-6f)) out |= 16u;
+        if (z > (p.maxd + p.trunc) * (1.0f + e)) out |= 32u;
+    } else {
+        out = 63u;
+    }
+    // a constraint every corner violates
+    unsigned all = out;
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) all &= __shfl_xor(all, d, 64);
+    return __builtin_amdgcn_readfirstlane(all) == 0u;
+}
+
 // One wavefront per brick (grid-stride over the allocated bricks); lane l
 // holds the x-row (y, z) = (l % 8, l / 8) of the brick.
 __global__ __launch_bounds__(256) void sparse_integrate_kernel(const SparseParams p,
@@ -187,6 +244,7 @@ __global__ __launch_bounds__(256) void sparse_integrate_kernel(const SparseParam
     const unsigned waves = gridDim.x * 4;
     for (unsigned b = blockIdx.x * 4 + (threadIdx.x >> 6); b < n; b += waves) {
         const int4 bc = p.coords[b];
+        if (!brick_may_update(p, F, bc, fcols, frows, lane)) continue;
         const int y = bc.y * kBrick + (lane & 7), z = bc.z * kBrick + (lane >> 3);
         const float cyw = voxel_centre(y, p.vs[1], p.off[1]);
         const float czw = voxel_centre(z, p.vs[2], p.off[2]);
@@ -329,6 +387,10 @@ extern "C" int recon_sparse_create(const recon_tsdf_config* cfg, int64_t brick_c
     p.coords = reinterpret_cast<int4*>(m + o_coords);
     p.D = reinterpret_cast<float*>(m + o_D);
     p.W = reinterpret_cast<float*>(m + o_W);
+    for (int k = 0; k < 3; ++k) {
+        p.blo[k] = -(1 << (kCoordBits - 1));
+        p.bhi[k] = (1 << (kCoordBits - 1)) - 1;
+    }
     p.counter = reinterpret_cast<unsigned*>(m + o_cnt);
     p.first_new = p.counter + 1;
     p.overflow = p.counter + 2;
@@ -430,6 +492,28 @@ extern "C" int recon_sparse_allocate_box(recon_sparse* h, const int32_t lo[3], c
                        h->p, make_int4(b0[0], b0[1], b0[2], 0), make_int4(nb[0], nb[1], nb[2], 0));
     RECON_HIP_TRY(hipGetLastError(), "sparse_alloc_box_kernel launch");
     return sparse_new_bricks(h, s);
+}
+
+extern "C" int recon_sparse_set_bounds(recon_sparse* h, const int32_t lo[3], const int32_t hi[3]) {
+    clear_error();
+    const char* fn = "recon_sparse_set_bounds";
+    if (!h) return fail(RECON_ERR_NULL, "%s: handle is NULL", fn);
+    for (int k = 0; k < 3; ++k) {
+        if (!lo || !hi) {
+            h->p.blo[k] = -(1 << (kCoordBits - 1));
+            h->p.bhi[k] = (1 << (kCoordBits - 1)) - 1;
+            continue;
+        }
+        if (hi[k] <= lo[k]) return fail(RECON_ERR_SIZE, "%s: empty box", fn);
+        if (lo[k] < -(1 << 23) || hi[k] > (1 << 23)) return fail(RECON_ERR_RANGE, "%s: box beyond 2^23", fn);
+    }
+    if (lo && hi)
+        for (int k = 0; k < 3; ++k) {
+            h->p.blo[k] = lo[k] >= 0 ? lo[k] / 8 : -((-lo[k] + 7) / 8);
+            const int hm = hi[k] - 1;
+            h->p.bhi[k] = hm >= 0 ? hm / 8 : -((-hm + 7) / 8);
+        }
+    return RECON_OK;
 }
 
 extern "C" int recon_sparse_stats(recon_sparse* h, int64_t* num_bricks, uint32_t* overflow,

@@ -303,6 +303,11 @@ class SparseVolume:
         check(self._lib.recon_sparse_integrate(self._h, C.byref(dv), ffi.pose_array(pose),
                                                _stream(stream)))
 
+    def set_bounds(self, lo=None, hi=None):
+        """Limit allocation to bricks meeting the voxel box [lo, hi) (None: unbounded)."""
+        check(self._lib.recon_sparse_set_bounds(self._h, ffi.i32x3(lo) if lo is not None else None,
+                                                ffi.i32x3(hi) if hi is not None else None))
+
     def allocate_box(self, lo, hi, stream=None):
         check(self._lib.recon_sparse_allocate_box(self._h, ffi.i32x3(lo), ffi.i32x3(hi),
                                                   _stream(stream)))

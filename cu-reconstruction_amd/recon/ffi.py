@@ -120,6 +120,7 @@ SIGNATURES = {
     "recon_sparse_reset": (C.c_int, [_vp, _vp]),
     "recon_sparse_integrate": (C.c_int, [_vp, C.POINTER(DepthView), _d7, _vp]),
     "recon_sparse_allocate_box": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32), _vp]),
+    "recon_sparse_set_bounds": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32)]),
     "recon_sparse_stats": (C.c_int, [_vp, _pi64, C.POINTER(C.c_uint32), _vp]),
     "recon_sparse_export": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(_i32), _vp, _vp, _vp]),
     "recon_sparse_download_bricks": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _pi64, _vp]),

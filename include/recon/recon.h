@@ -216,12 +216,16 @@ RECON_API int recon_sparse_reset(recon_sparse* h, recon_stream_t stream);
  * allocated the result equals the dense integrate bit for bit. */
 RECON_API int recon_sparse_integrate(recon_sparse* h, const recon_depth_view* depth,
                                      const double camera_pose[7], recon_stream_t stream);
+/* Host only.  Later integrate calls allocate only bricks that meet the
+ * voxel box [lo, hi) (NULL, NULL: unbounded, the default). */
+RECON_API int recon_sparse_set_bounds(recon_sparse* h, const int32_t lo[3], const int32_t hi[3]);
 /* async.  Allocates every brick that meets the voxel box [lo, hi). */
 RECON_API int recon_sparse_allocate_box(recon_sparse* h, const int32_t lo[3], const int32_t hi[3],
                                         recon_stream_t stream);
 /* sync.  Allocated bricks and overflow flags (bit 0: the pool filled and
  * bricks were dropped, bit 1: a pixel slab spanned more than 8 bricks on an
- * axis and was cut, bit 2: a slab left the |v| < 2^23 range). */
+ * axis and was cut, bit 2: a slab left the |v| < 2^23 range, bit 3: a brick
+ * was dropped after 128 hash probes). */
 RECON_API int recon_sparse_stats(recon_sparse* h, int64_t* num_bricks, uint32_t* overflow,
                                  recon_stream_t stream);
 /* async.  Dense box of voxels [origin, origin + size) into device arrays

@@ -122,6 +122,7 @@ void oracle_sample_colors_f32(const oracle_volume* vol, const float* C, const fl
 typedef struct oracle_sparse oracle_sparse;
 oracle_sparse* oracle_sparse_create(const oracle_volume* vol);
 void oracle_sparse_destroy(oracle_sparse* s);
+void oracle_sparse_set_bounds(oracle_sparse* s, const int32_t lo[3], const int32_t hi[3]);
 void oracle_sparse_allocate_box(oracle_sparse* s, const int32_t lo[3], const int32_t hi[3]);
 int64_t oracle_sparse_integrate(oracle_sparse* s, const float* depth, int32_t rows, int32_t cols,
                                 int64_t pitch, const double pose[7]);

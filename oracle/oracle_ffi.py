@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
             "oracle_sparse_create": (vp, [vp]),
             "oracle_sparse_destroy": (None, [vp]),
             "oracle_sparse_allocate_box": (None, [vp, vp, vp]),
+            "oracle_sparse_set_bounds": (None, [vp, vp, vp]),
             "oracle_sparse_integrate": (i64, [vp, vp, i32, i32, i64, vp]),
             "oracle_sparse_num_bricks": (i64, [vp]),
             "oracle_sparse_export": (None, [vp, vp, vp, vp, vp]),
@@ -228,6 +229,10 @@ class Sparse:
     def allocate_box(self, lo, hi):
         lib().oracle_sparse_allocate_box(self._h, _p(np.array(lo, np.int32)),
                                          _p(np.array(hi, np.int32)))
+
+    def set_bounds(self, lo, hi):
+        lib().oracle_sparse_set_bounds(self._h, _p(np.array(lo, np.int32)),
+                                       _p(np.array(hi, np.int32)))
 
     def integrate(self, depth, pose):
         depth = np.ascontiguousarray(depth, np.float32)

@@ -11,8 +11,9 @@
  * moved to the world with R^T (p - t) as ((R[k]*qx + R[3+k]*qy) + R[6+k]*qz);
  * their bounding box gives the voxel range floor((lo - off) / vs) - 1 ..
  * floor((hi - off) / vs) + 1 on each axis, and every brick of that range is
- * allocated (at most 8 per axis).  Then every voxel of every allocated brick
- * takes the dense update.  Bricks live in a hash map of this file's own.
+ * allocated (at most 8 per axis; within the allocation bounds when set).
+ * Then every voxel of every allocated brick takes the dense update.  Bricks
+ * live in a hash map of this file's own.
  */
 #include "oracle.h"
 
@@ -25,6 +26,8 @@
 
 struct oracle_sparse {
     oracle_volume vol;
+    int bounded;
+    int32_t blo[3], bhi[3];  /* allocation bounds in bricks, inclusive */
     int64_t n, cap;          /* bricks, capacity of the arrays */
     int32_t* coords;         /* 3 per brick */
     float* D;                /* 512 per brick */
@@ -104,6 +107,15 @@ void oracle_sparse_destroy(oracle_sparse* s) {
     free(s);
 }
 
+void oracle_sparse_set_bounds(oracle_sparse* s, const int32_t lo[3], const int32_t hi[3]) {
+    s->bounded = lo && hi;
+    if (!s->bounded) return;
+    for (int k = 0; k < 3; ++k) {
+        s->blo[k] = floor_div8(lo[k]);
+        s->bhi[k] = floor_div8(hi[k] - 1);
+    }
+}
+
 void oracle_sparse_allocate_box(oracle_sparse* s, const int32_t lo[3], const int32_t hi[3]) {
     for (int32_t z = floor_div8(lo[2]); z <= floor_div8(hi[2] - 1); ++z)
         for (int32_t y = floor_div8(lo[1]); y <= floor_div8(hi[1] - 1); ++y)
@@ -151,6 +163,10 @@ int64_t oracle_sparse_integrate(oracle_sparse* s, const float* depth, int32_t ro
                 b0[k] = floor_div8((int32_t)a);
                 b1[k] = floor_div8((int32_t)b);
                 if (b1[k] - b0[k] >= 8) b1[k] = b0[k] + 7;
+                if (s->bounded) {
+                    if (b0[k] < s->blo[k]) b0[k] = s->blo[k];
+                    if (b1[k] > s->bhi[k]) b1[k] = s->bhi[k];
+                }
             }
             if (!ok) continue;
             for (int32_t bz = b0[2]; bz <= b1[2]; ++bz)

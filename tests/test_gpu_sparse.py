@@ -93,6 +93,26 @@ def test_full_allocation_equals_dense_and_export_meshes(cuda):
     sv2.close()
 
 
+def test_allocation_bounds(cuda):
+    w, depth = _scene(2)
+    cfg = w.config()
+    lo, hi = (5, -3, 10), (50, 40, 70)
+    sv = recon.SparseVolume(cfg, 1 << 16)
+    sv.set_bounds(lo, hi)
+    so = O.Sparse(O.volume_from_config(cfg))
+    so.set_bounds(lo, hi)
+    for f in range(2):
+        sv.integrate(cuda.from_numpy(depth[f]).cuda(), w.poses[f])
+        so.integrate(depth[f], w.poses[f])
+    cuda.cuda.synchronize()
+    n = _compare_bricks(sv, so, "bounded")
+    assert n > 20
+    for k in sv.bricks():
+        for a in range(3):
+            assert lo[a] // 8 <= k[a] <= (hi[a] - 1) // 8
+    sv.close()
+
+
 def test_full_pool_is_flagged(cuda):
     w, depth = _scene(1)
     sv = recon.SparseVolume(w.config(), 16)
