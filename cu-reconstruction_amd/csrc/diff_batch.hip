@@ -541,6 +541,9 @@ int recon_diff_batch::update(float* poses, const AdamState& opt, double* history
 static int check_dvolume(const recon_diff_volume& v, int i, recon_tsdf_info* info) {
     int rc = describe(&v.config, info);
     if (rc) return rc;
+    if (info->weight_type != RECON_WEIGHT_BY_DEPTH)
+        return fail(RECON_ERR_CONFIG, "recon_diff_batch_create: volume %d: the differentiable "
+                    "integrate weighs every observation 1 (WEIGHT_BY_DEPTH)", i);
     if (v.num_frames < 1 || v.num_frames > 64)
         return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d has %d frames (1..64)", i,
                     v.num_frames);

@@ -232,7 +232,11 @@ __device__ __forceinline__ void append_tile(uint64_t live, uint32_t tile, uint32
 // wc' = min(wc + 1, max_weight) -- the running average of the octree's
 // add_observation (octree.cu:118-137) with the dense update's capped weight.
 // Free-space updates (sdf > trunc) leave the colour alone.
-template <class S, class Frames, bool VEC, bool COUNT, bool COLOR = false>
+//
+// VARW (WEIGHT_BY_VARIANCE): an observation weighs
+// variance_weight(d) = sigma(1 m)^2 / sigma(d)^2 instead of 1, so
+// d' = ((d * w) + (wo * tsdf)) / (w + wo), w' = min(w + wo, max_weight).
+template <class S, class Frames, bool VEC, bool COUNT, bool COLOR = false, bool VARW = false>
 __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int nframes,
                                                const Frames& frames, const uint64_t live,
                                                const int x0, const int y0, const int z0,
@@ -327,9 +331,16 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
         for (int j = 0; j < NV; ++j) {
             if (!pass[j]) continue;
             // tsdf_volume.cu:178-186 with the weight cap
-            const float nw = w[j] + 1.0f;
-            d[j] = ((d[j] * w[j]) + tsdf[j]) / nw;
-            w[j] = nw < vp.maxw ? nw : vp.maxw;
+            if (VARW) {
+                const float wo = variance_weight(vp.noise, vp.wref, dep[j]);
+                const float nw = w[j] + wo;
+                d[j] = ((d[j] * w[j]) + (wo * tsdf[j])) / nw;
+                w[j] = nw < vp.maxw ? nw : vp.maxw;
+            } else {
+                const float nw = w[j] + 1.0f;
+                d[j] = ((d[j] * w[j]) + tsdf[j]) / nw;
+                w[j] = nw < vp.maxw ? nw : vp.maxw;
+            }
             dirty[j >> 2] = true;
             if (COUNT) ++cnt;
             if (COLOR && band[j]) {

@@ -124,6 +124,20 @@ int describe(const recon_tsdf_config* cfg, recon_tsdf_info* info) {
     info->intrinsics[3] = cfg->cy;
     info->num_voxels = n;
     if (!finite_pos(info->truncation)) return fail(RECON_ERR_CONFIG, "truncation must be > 0");
+    if (cfg->weight_type != RECON_WEIGHT_BY_DEPTH && cfg->weight_type != RECON_WEIGHT_BY_VARIANCE)
+        return fail(RECON_ERR_CONFIG, "weight_type %d is not a WEIGHT_TYPE", cfg->weight_type);
+    info->weight_type = cfg->weight_type;
+    const float* nm = cfg->noise_model;
+    if (nm[0] == 0.0f && nm[1] == 0.0f && nm[2] == 0.0f) {
+        // axial noise of a structured-light depth camera (Nguyen, Izadi, Lovell 2012)
+        info->noise_model[0] = 0.0012f;
+        info->noise_model[1] = 0.0019f;
+        info->noise_model[2] = 0.4f;
+    } else {
+        if (!finite_pos(nm[0]) || !std::isfinite(nm[1]) || nm[1] < 0.0f || !std::isfinite(nm[2]))
+            return fail(RECON_ERR_CONFIG, "noise_model needs a > 0, b >= 0, finite c");
+        for (int i = 0; i < 3; ++i) info->noise_model[i] = nm[i];
+    }
     return RECON_OK;
 }
 

@@ -86,7 +86,17 @@ struct VolumeParams {
     float fcols, frows;
     int32_t b0x, b0y, b1x, b1y;  // bound-level block counts
     float* C = nullptr;          // colour volume, 4 floats per voxel (r, g, b, weight)
+    float noise[3] = {0.f, 0.f, 0.f};  // WEIGHT_BY_VARIANCE: sigma(d) = a + b (d - c)^2
+    float wref = 0.f;                  // sigma(1 m)^2
 };
+
+// Observation weight of depth sample d under WEIGHT_BY_VARIANCE:
+// sigma(1 m)^2 / sigma(d)^2 with sigma(d) = a + (b * ((d - c) * (d - c))).
+__host__ __device__ inline float variance_weight(const float noise[3], float wref, float d) {
+    const float e = d - noise[2];
+    const float sig = noise[0] + (noise[1] * (e * e));
+    return wref / (sig * sig);
+}
 
 constexpr int kMaxFusedFrames = 16;
 

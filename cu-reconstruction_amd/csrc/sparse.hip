@@ -346,6 +346,8 @@ extern "C" int recon_sparse_create(const recon_tsdf_config* cfg, int64_t brick_c
     recon_tsdf_info info;
     int rc = describe(cfg, &info);
     if (rc) return rc;
+    if (info.weight_type != RECON_WEIGHT_BY_DEPTH)
+        return fail(RECON_ERR_CONFIG, "%s: sparse volumes integrate with WEIGHT_BY_DEPTH only", fn);
     if (brick_capacity < 1 || brick_capacity > (int64_t(1) << 24))
         return fail(RECON_ERR_RANGE, "%s: brick capacity %lld outside 1..2^24", fn,
                     (long long)brick_capacity);

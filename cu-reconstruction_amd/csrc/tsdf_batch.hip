@@ -350,6 +350,8 @@ int recon_batch::record(hipStream_t s) {
 static int check_volume(const recon_batch_volume& v, int i, recon_tsdf_info* info) {
     int rc = describe(&v.config, info);
     if (rc) return fail(rc, "volume %d: %s", i, recon_last_error());
+    if (info->weight_type != RECON_WEIGHT_BY_DEPTH)
+        return fail(RECON_ERR_CONFIG, "volume %d: batches integrate with WEIGHT_BY_DEPTH only", i);
     if (v.num_frames < 0 || v.num_frames > 64)
         return fail(RECON_ERR_SIZE, "volume %d: num_frames %d outside 0..64", i, v.num_frames);
     if (v.num_frames > 0) {

@@ -476,6 +476,9 @@ static int check_frames(const recon_tsdf* h, const recon_depth_view* depth0,
                         size_t frame_stride_bytes, int32_t num_frames, const float* poses,
                         const char* fn) {
     if (!h) return fail(RECON_ERR_NULL, "%s: handle is NULL", fn);
+    if (h->info.weight_type != RECON_WEIGHT_BY_DEPTH)
+        return fail(RECON_ERR_CONFIG, "%s: the differentiable integrate weighs every observation 1 "
+                    "(WEIGHT_BY_DEPTH)", fn);
     if (!depth0 || !depth0->data) return fail(RECON_ERR_NULL, "%s: depth is NULL", fn);
     if (!poses && num_frames > 0) return fail(RECON_ERR_NULL, "%s: poses is NULL", fn);
     if (num_frames < 0) return fail(RECON_ERR_SIZE, "%s: num_frames < 0", fn);

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void cull_kernel(const VolumeParams vp, const 
 // Each wavefront takes list entries i = wave, wave + waves, ...; the grid is
 // sized to the resident wave slots, so every wave-tile costs about the same
 // and the device stays full until the list is done.
-template <int MAXF, bool VEC, bool COUNT, bool COLOR = false>
+template <int MAXF, bool VEC, bool COUNT, bool COLOR = false, bool VARW = false>
 __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
     const VolumeParams vp, const int nframes, const FrameBlock<MAXF> fb,
     const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count,
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
                                 __builtin_amdgcn_readfirstlane((uint32_t)list[i].frames);
         int x0, y0, z0;
         tile_origin<Shape>(vp, tile, x0, y0, z0);
-        integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR>(vp, nframes, fb, frames, x0, y0,
-                                                                   z0, cnt);
+        integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR, VARW>(vp, nframes, fb, frames, x0,
+                                                                         y0, z0, cnt);
     }
     if (COUNT && cnt) atomicAdd(counter, (unsigned long long)cnt);
 }
@@ -149,6 +149,18 @@ static int ensure_scratch(recon_tsdf* h, const VolumeParams& vp, hipStream_t s) 
     return RECON_OK;
 }
 
+template <int MAXF, bool VEC, bool COUNT, bool COLOR>
+static void launch_list(bool varw, dim3 grid, hipStream_t s, const VolumeParams& vp, int nframes,
+                        const FrameBlock<MAXF>& fb, const ListEntry* list, const unsigned* count,
+                        unsigned long long* ctr) {
+    if (varw)
+        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, true>), grid,
+                           dim3(kThreads), 0, s, vp, nframes, fb, list, count, ctr);
+    else
+        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, false>), grid,
+                           dim3(kThreads), 0, s, vp, nframes, fb, list, count, ctr);
+}
+
 // The three launches of one integrate sweep over `nframes` frames (frame f at
 // depth0 + f * stride_floats).
 template <int MAXF>
@@ -183,34 +195,28 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
                        nframes, fb, list, h->list_count);
     RECON_HIP_TRY(hipGetLastError(), "cull_kernel launch");
     const bool vec = (vp.gx % 4) == 0;
+    const bool varw = h->info.weight_type == RECON_WEIGHT_BY_VARIANCE;
     unsigned long long* ctr = h->update_counter;
     const dim3 grid(h->num_cus * h->integrate_blocks_per_cu);
-    if constexpr (MAXF == 1) {
-        if (fb.f[0].rgb && vp.C) {  // colour integrate (no update counting)
+    bool color = false;
+    if constexpr (MAXF == 1) color = fb.f[0].rgb && vp.C;  // colour integrate (no counting)
+    if (color) {
+        if constexpr (MAXF == 1) {
             if (vec)
-                hipLaunchKernelGGL((integrate_list_kernel<1, true, false, true>), grid, dim3(kThreads),
-                                   0, s, vp, nframes, fb, list, h->list_count, ctr);
+                launch_list<1, true, false, true>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
             else
-                hipLaunchKernelGGL((integrate_list_kernel<1, false, false, true>), grid,
-                                   dim3(kThreads), 0, s, vp, nframes, fb, list, h->list_count, ctr);
-            RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
-            return RECON_OK;
+                launch_list<1, false, false, true>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
         }
-    }
-    if (ctr) {
+    } else if (ctr) {
         if (vec)
-            hipLaunchKernelGGL((integrate_list_kernel<MAXF, true, true>), grid, dim3(kThreads), 0,
-                               s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, true, true, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
         else
-            hipLaunchKernelGGL((integrate_list_kernel<MAXF, false, true>), grid, dim3(kThreads), 0,
-                               s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, false, true, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
     } else {
         if (vec)
-            hipLaunchKernelGGL((integrate_list_kernel<MAXF, true, false>), grid, dim3(kThreads), 0,
-                               s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, true, false, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
         else
-            hipLaunchKernelGGL((integrate_list_kernel<MAXF, false, false>), grid, dim3(kThreads), 0,
-                               s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, false, false, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
     }
     RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
     return RECON_OK;
@@ -287,6 +293,12 @@ extern "C" int recon_tsdf_create(const recon_tsdf_config* cfg, recon_tsdf** out)
     vp.trunc = info.truncation;
     vp.maxw = info.max_weight;
     vp.maxd = info.max_depth;
+    if (info.weight_type == RECON_WEIGHT_BY_VARIANCE) {
+        for (int i = 0; i < 3; ++i) vp.noise[i] = info.noise_model[i];
+        const float e1 = 1.0f - vp.noise[2];
+        const float s1 = vp.noise[0] + (vp.noise[1] * (e1 * e1));
+        vp.wref = s1 * s1;
+    }
     const size_t bytes = sizeof(float) * (size_t)info.num_voxels;
     hipError_t e = hipMalloc((void**)&h->D, bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&h->W, bytes);

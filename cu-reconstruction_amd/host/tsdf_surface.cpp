@@ -57,6 +57,7 @@ TSDFSurface::TSDFSurface(const Config& cfg) {
     // additions of this library (0: the reference's constants)
     cfg_.truncation = cfg.get<float>("tsdf.volume.truncation", 0.f);
     cfg_.max_depth = cfg.get<float>("tsdf.volume.max_depth", 0.f);
+    cfg_.weight_type = cfg.get<int>("tsdf.volume.weight_type", 0) == 1 ? WEIGHT_BY_VARIANCE : WEIGHT_BY_DEPTH;
     welded_ = cfg.get<int>("tsdf.mesh.welded", 0) != 0;
     color_ = cfg.get<int>("dataset.color.mode", 0) != 0;
     global_pose = euler_pose(cfg.get<float>("tsdf.volume.camera.pose.euler.roll", 0.f),

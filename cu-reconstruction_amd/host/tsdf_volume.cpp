@@ -37,6 +37,8 @@ recon_tsdf_config to_c(const TSDFVolumeConfig& c) {
     r.max_cell_size = c.max_cell_size;
     r.truncation = c.truncation;
     r.max_depth = c.max_depth;
+    r.weight_type = (int32_t)c.weight_type;
+    for (int i = 0; i < 3; ++i) r.noise_model[i] = c.noise_model[i];
     return r;
 }
 

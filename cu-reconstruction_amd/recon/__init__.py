@@ -32,7 +32,7 @@ def _stream(stream) -> int:
 
 def make_config(grid, voxel_size=None, physical_size=None, offset=(0.0, 0.0, 0.0),
                 max_weight=128.0, intrinsics=(525.0, 525.0, 319.5, 239.5), image=(640, 480),
-                truncation=0.0, max_depth=0.0) -> TsdfConfig:
+                truncation=0.0, max_depth=0.0, weight_type=0, noise_model=(0.0, 0.0, 0.0)) -> TsdfConfig:
     """TSDFVolumeConfig (tsdf_volume.cuh:17-29); give voxel_size or physical_size."""
     cfg = TsdfConfig()
     grid = [int(g) for g in grid]
@@ -48,6 +48,8 @@ def make_config(grid, voxel_size=None, physical_size=None, offset=(0.0, 0.0, 0.0
     cfg.max_cell_size = 0.0
     cfg.truncation = float(truncation)
     cfg.max_depth = float(max_depth)
+    cfg.weight_type = int(weight_type)
+    cfg.noise_model = ffi.f32x3(noise_model)
     return cfg
 
 

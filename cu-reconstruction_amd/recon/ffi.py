@@ -41,6 +41,8 @@ class TsdfConfig(C.Structure):
         ("max_cell_size", C.c_float),
         ("truncation", C.c_float),
         ("max_depth", C.c_float),
+        ("weight_type", C.c_int32),
+        ("noise_model", C.c_float * 3),
     ]
 
 
@@ -54,6 +56,8 @@ class TsdfInfo(C.Structure):
         ("max_depth", C.c_float),
         ("intrinsics", C.c_float * 4),
         ("num_voxels", C.c_int64),
+        ("weight_type", C.c_int32),
+        ("noise_model", C.c_float * 3),
     ]
 
 

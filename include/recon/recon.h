@@ -72,7 +72,18 @@ typedef struct recon_tsdf_config {
     float max_cell_size;          /* unused by the reference; kept for layout   */
     float truncation;             /* metres; <= 0 -> 1.1*|voxel_size|           */
     float max_depth;              /* metres; <= 0 -> 24                         */
+    /* WEIGHT_TYPE (tsdf_volume.cuh:12-15, declared and unused there):
+     * RECON_WEIGHT_BY_DEPTH (0) weighs every observation 1, the reference's
+     * update; RECON_WEIGHT_BY_VARIANCE (1) weighs a depth sample d by
+     * sigma(1 m)^2 / sigma(d)^2 with the axial noise model
+     * sigma(d) = a + b*(d - c)^2 (Nguyen, Izadi, Lovell 2012), so
+     * D' = (D*W + w*tsdf) / (W + w), W' = min(W + w, max_weight). */
+    int32_t weight_type;
+    float noise_model[3];         /* a, b, c; all 0 -> 0.0012, 0.0019, 0.4     */
 } recon_tsdf_config;
+
+#define RECON_WEIGHT_BY_DEPTH 0
+#define RECON_WEIGHT_BY_VARIANCE 1
 
 /* Read-only description of a created volume. */
 typedef struct recon_tsdf_info {
@@ -84,6 +95,8 @@ typedef struct recon_tsdf_info {
     float max_depth;
     float intrinsics[4];          /* fx, fy, cx, cy */
     int64_t num_voxels;
+    int32_t weight_type;          /* as configured */
+    float noise_model[3];         /* a, b, c in use */
 } recon_tsdf_info;
 
 /* Pitched single-channel float32 image in device memory, the shape of

@@ -34,6 +34,15 @@ using r32 = float;
 using r64 = double;
 using ui32 = uint32_t;
 
+// tsdf_volume.cuh:12-15 (declared, unused there): the observation weight of
+// the integrate.  WEIGHT_BY_DEPTH weighs every observation 1 (the reference's
+// update); WEIGHT_BY_VARIANCE weighs a depth sample d by sigma(1 m)^2 /
+// sigma(d)^2, sigma(d) = a + b*(d - c)^2 (noise_model; 0 -> 0.0012, 0.0019, 0.4).
+enum WEIGHT_TYPE {
+    WEIGHT_BY_DEPTH = 0,
+    WEIGHT_BY_VARIANCE,
+};
+
 struct TSDFVolumeConfig {
     dim3 voxel_grid_size;
     float3 physical_size;   // in physical units
@@ -49,6 +58,8 @@ struct TSDFVolumeConfig {
     // additions (0 keeps the reference's constants 1.1*|voxel_size| and 24 m)
     r32 truncation = 0.0f;
     r32 max_depth = 0.0f;
+    WEIGHT_TYPE weight_type = WEIGHT_BY_DEPTH;
+    r32 noise_model[3] = {0.0f, 0.0f, 0.0f};
 };
 
 // Device image view with the members of cv::cuda::PtrStepSz<T>.

@@ -109,7 +109,7 @@ int64_t oracle_mc_extract(const float* field, const int32_t grid[3],
     const int32_t gx = grid[0], gy = grid[1], gz = grid[2];
     if (gx < 2 || gy < 2 || gz < 2) return 0;
     oracle_volume vol = {{gx, gy, gz}, {voxel_size[0], voxel_size[1], voxel_size[2]},
-                         {offset[0], offset[1], offset[2]}, 0, 0, 0, 0, 0, 0, 0};
+                         {offset[0], offset[1], offset[2]}, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0}};
     /* per-layer counts, then an exclusive scan gives each layer's first
      * triangle: the emission order is z (outer), y, x, slot
      * (marching_cubes.cu:163-188 inside the z loop of :213-225). */
@@ -191,7 +191,7 @@ int64_t oracle_mc_weld(const float* field, const int32_t grid[3], const float vo
     *num_vertices = 0;
     if (gx < 2 || gy < 2 || gz < 2) return 0;
     oracle_volume vol = {{gx, gy, gz}, {voxel_size[0], voxel_size[1], voxel_size[2]},
-                         {offset[0], offset[1], offset[2]}, 0, 0, 0, 0, 0, 0, 0};
+                         {offset[0], offset[1], offset[2]}, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0}};
     const int64_t n = (int64_t)gx * gy * gz;
     int64_t* id = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)n);
     int64_t nv = 0;

@@ -43,6 +43,8 @@ typedef struct oracle_volume {
     float max_weight;
     float max_depth;
     float fx, fy, cx, cy;
+    int32_t weight_type;    /* 0: weight 1; 1: sigma(1)^2 / sigma(d)^2 */
+    float noise[3];         /* sigma(d) = noise[0] + noise[1] * (d - noise[2])^2 */
 } oracle_volume;
 
 /* Pose {qw,qx,qy,qz,tx,ty,tz} narrowed to float and expanded to a row-major

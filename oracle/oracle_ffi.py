@@ -18,7 +18,8 @@ LIB = HERE / "lib" / "liboracle.so"
 class OracleVolume(C.Structure):
     _fields_ = [("grid", C.c_int32 * 3), ("voxel_size", C.c_float * 3), ("offset", C.c_float * 3),
                 ("truncation", C.c_float), ("max_weight", C.c_float), ("max_depth", C.c_float),
-                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float)]
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("weight_type", C.c_int32), ("noise", C.c_float * 3)]
 
 
 _lib = None
@@ -110,8 +111,14 @@ def volume_from_config(cfg) -> OracleVolume:
         dot = (vs[0] * vs[0] + vs[1] * vs[1]) + vs[2] * vs[2]
         trunc = np.float32(1.1) * np.sqrt(np.float32(dot))
     maxd = np.float32(cfg.max_depth) if cfg.max_depth > 0 else np.float32(24.0)
-    return volume(grid, vs, list(cfg.global_offset), trunc, cfg.max_weight,
-                  (cfg.focal_x, cfg.focal_y, cfg.cx, cfg.cy), maxd)
+    v = volume(grid, vs, list(cfg.global_offset), trunc, cfg.max_weight,
+               (cfg.focal_x, cfg.focal_y, cfg.cx, cfg.cy), maxd)
+    v.weight_type = int(getattr(cfg, "weight_type", 0))
+    nm = list(getattr(cfg, "noise_model", (0.0, 0.0, 0.0)))
+    if not any(nm):
+        nm = [0.0012, 0.0019, 0.4]       # the library's default noise model
+    v.noise = (C.c_float * 3)(*[float(np.float32(q)) for q in nm])
+    return v
 
 
 def shape(vol: OracleVolume):

@@ -121,9 +121,21 @@ static int64_t integrate_f32_impl(const oracle_volume* vol, float* D, float* W, 
             float tsdf = sdf;
             if (sdf > 0.0f) tsdf = fminf(sdf, trunc);      /* tsdf_volume.cu:176-177 */
             const float w = W[idx];
-            const float new_w = w + 1.0f;
             const float dv = D[idx];
-            const float new_d = ((dv * w) + tsdf) / new_w; /* tsdf_volume.cu:183 */
+            float new_w, new_d;
+            if (vol->weight_type == 1) {
+                /* WEIGHT_BY_VARIANCE: the sample weighs sigma(1)^2 / sigma(d)^2 */
+                const float e1 = 1.0f - vol->noise[2];
+                const float s1 = vol->noise[0] + (vol->noise[1] * (e1 * e1));
+                const float e = d - vol->noise[2];
+                const float sg = vol->noise[0] + (vol->noise[1] * (e * e));
+                const float wo = (s1 * s1) / (sg * sg);
+                new_w = w + wo;
+                new_d = ((dv * w) + (wo * tsdf)) / new_w;
+            } else {
+                new_w = w + 1.0f;
+                new_d = ((dv * w) + tsdf) / new_w; /* tsdf_volume.cu:183 */
+            }
             W[idx] = new_w < maxw ? new_w : maxw;
             D[idx] = new_d;
             if (C && rgb && sdf <= trunc) {
