@@ -16,6 +16,11 @@ exactly by the kernel's counting mode outside the timed region) divided by
 the call's average duration from HIP events on the launch stream; traffic
 is the HBM bytes per call from the committed rocprofv3 summary
 (profiles/integrate_plain_large_512.md).
+
+Side legs in the same JSON line: meshing (soup vs welded), colour, sparse
+bricks, the small 128^3 x 10-frame workload in full on the GPU and on the
+CPU (1 thread and all host cores), point-cloud render, batched volumes, the
+differentiable forward + adjoint, batched pose refinement.
 """
 from __future__ import annotations
 
@@ -72,6 +77,58 @@ def cpu_baseline(w, depth, seconds: float) -> dict:
             "cores": threads, "kind": "port",
             "sample": f"oracle float32 integrate of the first {frames} frames of the "
                       f"{w.name} workload ({dt:.1f} s, OpenMP over voxel rows)"}
+
+
+def bench_small(steps: int, cpu: bool = True) -> dict:
+    """The small CPU-runnable workload (128^3 volume, 8 mm voxels, 10 frames
+    of 640x480) in full: the GPU per-frame integrate (HIP events on the launch
+    stream, `steps` passes) beside the oracle's float32 integrate of all 10
+    frames on one host thread and on up to 16 threads."""
+    import torch
+
+    import recon
+    from recon import scenes
+
+    w = scenes.plain_small()
+    depth = w.render(nthreads=min(os.cpu_count() or 1, 16))
+    dev = torch.from_numpy(depth).cuda()
+    vol = recon.TSDFVolume(w.config())
+    stream = torch.cuda.current_stream()
+    F = w.num_frames
+
+    def one_pass():
+        vol.reset(stream)
+        for f in range(F):
+            vol.integrate(dev[f], w.poses[f], stream)
+
+    one_pass()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        one_pass()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    pass_ms = e0.elapsed_time(e1) / steps
+    vol.close()
+    out = {"workload": w.name, "grid": list(w.grid), "frames": F, "image": list(w.image),
+           "gpu_ms_per_pass": round(pass_ms, 4),
+           "gpu_gvu_s": round(w.num_voxels * F / (pass_ms / 1000.0) / 1e9, 3)}
+    del dev
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_ffi as O
+
+        ov = O.volume_from_config(w.config())
+        for threads in (1, min(os.cpu_count() or 1, 16)):
+            Dc, Wc = O.init(ov)
+            t0 = time.perf_counter()
+            for f in range(F):
+                O.integrate_f32(ov, Dc, Wc, depth[f], w.poses[f], nthreads=threads)
+            dt = time.perf_counter() - t0
+            out[f"cpu_{threads}t_ms_per_pass"] = round(1000.0 * dt, 1)
+            out[f"cpu_{threads}t_gvu_s"] = round(w.num_voxels * F / dt / 1e9, 4)
+    return out
 
 
 def bench_batched(steps: int, autotune_to: str | None = None) -> dict:
@@ -639,6 +696,8 @@ def main() -> int:
     vol.close()
     del dev
     torch.cuda.empty_cache()
+    log("timing the small workload (128^3 x 10 frames) on the GPU and the CPU")
+    result["small"] = bench_small(max(args.steps, 5), cpu=not args.no_cpu)
     if not args.no_render:
         log("timing the point-cloud render")
         result["render"] = bench_render(w, depth, args.steps, cpu=not args.no_cpu)

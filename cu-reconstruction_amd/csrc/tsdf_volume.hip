@@ -91,6 +91,24 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ D, float*
     fill4(D, W, 4ll * (blockIdx.x * (long long)blockDim.x + threadIdx.x), n, value_d);
 }
 
+// init_voxel_transofrmation (tsdf_volume.cu:106-127): packed 28-byte records
+// {float3 translation = voxel centre; float4 quat_rotation = (0, 0, 0, 1)}.
+__global__ __launch_bounds__(256) void transformation_voxels_kernel(const VolumeParams vp,
+                                                                    float* __restrict__ out,
+                                                                    long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % vp.gx), y = (int)((i / vp.gx) % vp.gy), z = (int)(i / ((long long)vp.gx * vp.gy));
+    float* r = out + 7 * i;
+    r[0] = voxel_centre(x, vp.vs[0], vp.off[0]);
+    r[1] = voxel_centre(y, vp.vs[1], vp.off[1]);
+    r[2] = voxel_centre(z, vp.vs[2], vp.off[2]);
+    r[3] = 0.0f;
+    r[4] = 0.0f;
+    r[5] = 0.0f;
+    r[6] = 1.0f;
+}
+
 __global__ __launch_bounds__(256) void fill_color_kernel(float4* __restrict__ C, long long n) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i < n) C[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -430,6 +448,16 @@ extern "C" int recon_tsdf_enable_color(recon_tsdf* h, int enable, recon_stream_t
     hipLaunchKernelGGL(fill_color_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<float4*>(h->C), n);
     RECON_HIP_TRY(hipGetLastError(), "fill_color_kernel launch");
+    return RECON_OK;
+}
+
+extern "C" int recon_tsdf_transformation_voxels(recon_tsdf* h, void* out, recon_stream_t stream) {
+    clear_error();
+    if (!h || !out) return fail(RECON_ERR_NULL, "recon_tsdf_transformation_voxels: NULL argument");
+    const long long n = h->info.num_voxels;
+    hipLaunchKernelGGL(transformation_voxels_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, h->vp, static_cast<float*>(out), n);
+    RECON_HIP_TRY(hipGetLastError(), "transformation_voxels_kernel launch");
     return RECON_OK;
 }
 

@@ -61,7 +61,10 @@ TSDFVolume::TSDFVolume(const TSDFVolumeConfig& cfg, hipStream_t stream) : cfg_(c
     }
 }
 
-TSDFVolume::~TSDFVolume() { recon_tsdf_destroy(h_); }
+TSDFVolume::~TSDFVolume() {
+    if (t_voxels_) (void)hipFree(t_voxels_);
+    recon_tsdf_destroy(h_);
+}
 
 bool TSDFVolume::integrate_raw(const uint8_t* rgb, size_t rgb_step, const float* depth, size_t step,
                                int rows, int cols, const std::array<r64, 7>& pose) {
@@ -75,6 +78,25 @@ bool TSDFVolume::integrate_raw(const uint8_t* rgb, size_t rgb_step, const float*
         return false;
     }
     return sync(stream_, "TSDFVolume::integrate()");
+}
+
+TSDFVolume::TransformationVoxel::Ptr TSDFVolume::transformation_voxels_data() const {
+    if (t_voxels_ || !h_) return t_voxels_;
+    recon_tsdf_info info;
+    if (recon_tsdf_get_info(h_, &info) != RECON_OK) return nullptr;
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(TransformationVoxel) * (size_t)info.num_voxels) != hipSuccess) {
+        std::fprintf(stderr, "[cuphoto] TSDFVolume::transformation_voxels_data() -- out of device memory\n");
+        return nullptr;
+    }
+    if (recon_tsdf_transformation_voxels(h_, p, stream_) != RECON_OK ||
+        !sync(stream_, "TSDFVolume::transformation_voxels_data()")) {
+        log_error("TSDFVolume::transformation_voxels_data()");
+        (void)hipFree(p);
+        return nullptr;
+    }
+    t_voxels_ = static_cast<TransformationVoxel*>(p);
+    return t_voxels_;
 }
 
 bool TSDFVolume::enable_color(bool enable) {

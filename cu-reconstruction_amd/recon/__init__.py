@@ -211,6 +211,14 @@ class TSDFVolume:
         check(self._lib.recon_tsdf_download(self._h, D.ctypes.data, W.ctypes.data, _stream(stream)))
         return D, W
 
+    def transformation_voxels(self, stream=None):
+        """(N, 7) float32 device tensor: voxel centre and quaternion (x, y, z, w)
+        per voxel, the reference's packed TransformationVoxel records."""
+        torch = _torch()
+        out = torch.empty((self.num_voxels, 7), dtype=torch.float32, device="cuda")
+        check(self._lib.recon_tsdf_transformation_voxels(self._h, out.data_ptr(), _stream(stream)))
+        return out
+
     def enable_color(self, enable=True, stream=None):
         """Allocates (or frees) the colour volume (recon_tsdf_enable_color)."""
         check(self._lib.recon_tsdf_enable_color(self._h, 1 if enable else 0, _stream(stream)))

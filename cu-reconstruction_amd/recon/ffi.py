@@ -117,6 +117,7 @@ SIGNATURES = {
     "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
     "recon_tsdf_enable_color": (C.c_int, [_vp, C.c_int, _vp]),
     "recon_tsdf_colors": (C.c_void_p, [_vp]),
+    "recon_tsdf_transformation_voxels": (C.c_int, [_vp, _vp, _vp]),
     "recon_tsdf_download_colors": (C.c_int, [_vp, _vp, _vp]),
     "recon_tsdf_sample_colors": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "recon_sparse_create": (C.c_int, [C.POINTER(TsdfConfig), _i64, C.POINTER(_vp)]),

@@ -128,6 +128,13 @@ RECON_API int recon_tsdf_reset(recon_tsdf* h, recon_stream_t stream);
 RECON_API int recon_tsdf_get_info(const recon_tsdf* h, recon_tsdf_info* info);
 
 /* ---- accessors (tsdf_volume.cuh:57-92) --------------------------------- */
+/* async.  The reference's per-voxel transformation records
+ * (TSDFVolume::transformation_voxels_data, tsdf_volume.cuh:33-38,84-90,
+ * filled by tsdf_volume.cu:106-127), which this library does not store:
+ * 28-byte packed {float3 translation = ((v+0.5)*vs)+off; float4
+ * quat_rotation = (x 0, y 0, z 0, w 1)} for every voxel into `out` (device,
+ * 28*N bytes). */
+RECON_API int recon_tsdf_transformation_voxels(recon_tsdf* h, void* out, recon_stream_t stream);
 RECON_API float* recon_tsdf_distances(recon_tsdf* h);   /* device pointer, N floats */
 RECON_API float* recon_tsdf_weights(recon_tsdf* h);     /* device pointer, N floats */
 /* sync.  Copies D and/or W (either may be NULL) to host memory of N floats. */

@@ -103,6 +103,20 @@ public:
     float3 voxel_phys_size() const { return cfg_.physical_size; }
     r32 truncation() const;
 
+    // tsdf_volume.cuh:33-38: packed voxel transformation record
+    struct TransformationVoxel {
+        using Ptr = TransformationVoxel*;
+        // plain members: HIP's float3/float4 are not POD, so a packed
+        // struct of them would not be 28 bytes; .x/.y/.z(/.w) access is the same
+        struct { float x, y, z; } translation;
+        struct { float x, y, z, w; } quat_rotation;
+    } __attribute__((packed));
+    static_assert(sizeof(TransformationVoxel) == 28, "packed 28-byte record");
+    // tsdf_volume.cuh:84-92.  Not stored by this library (centres are
+    // computed); synthesised on the first call (28 B per voxel of device
+    // memory, kept until the volume is destroyed), then returned as is.
+    TransformationVoxel::Ptr transformation_voxels_data() const;
+
     bool ok() const { return h_ != nullptr; }
     void reset();
     // No counterpart in the reference's dense volume: a colour average per
@@ -120,6 +134,7 @@ private:
     TSDFVolumeConfig cfg_;
     recon_tsdf* h_ = nullptr;
     hipStream_t stream_ = nullptr;
+    mutable TransformationVoxel* t_voxels_ = nullptr;
 };
 
 // marching_cubes.cu:191-236: appends the triangle soup of the volume's zero

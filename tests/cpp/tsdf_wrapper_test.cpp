@@ -131,6 +131,27 @@ int main() {
     CHECK(std::memcmp(D.data(), Do.data(), nvox * 4) == 0, "TSDF differs from the oracle");
     CHECK(std::memcmp(W.data(), Wo.data(), nvox * 4) == 0, "weights differ from the oracle");
 
+    // transformation_voxels_data: the voxel centre and the identity quaternion
+    // per voxel (tsdf_volume.cu:106-127); the same pointer on a second call
+    {
+        using TV = cuphoto::TSDFVolume::TransformationVoxel;
+        const TV* tv = vol.transformation_voxels_data();
+        CHECK(tv != nullptr && tv == vol.transformation_voxels_data(), "transformation_voxels_data");
+        std::vector<TV> T(nvox);
+        if (tv)
+            CHECK(hipMemcpy(T.data(), tv, nvox * sizeof(TV), hipMemcpyDeviceToHost) == hipSuccess, "T");
+        bool tv_ok = tv != nullptr;
+        for (size_t i = 0; tv_ok && i < nvox; ++i) {
+            float c[3];
+            oracle_voxel_centre_f32(&ov, (int32_t)(i % g.x), (int32_t)((i / g.x) % g.y),
+                                    (int32_t)(i / ((size_t)g.x * g.y)), c);
+            const float want[7] = {c[0], c[1], c[2], 0.f, 0.f, 0.f, 1.f};
+            tv_ok = std::memcmp(&T[i], want, sizeof(want)) == 0;
+            if (!tv_ok) std::fprintf(stderr, "transformation voxel %zu differs\n", i);
+        }
+        CHECK(tv_ok, "transformation voxels differ from the oracle's centres");
+    }
+
     std::vector<float3> verts;
     std::vector<int3> tris;
     verts.push_back(make_float3(7.f, 7.f, 7.f));  // generate_triangular_surface appends

@@ -234,3 +234,8 @@ def test_colour_arguments_rejected_before_any_launch():
     assert lib.recon_tsdf_colors(None) is None
     assert lib.recon_tsdf_download_colors(None, C.c_void_p(4096), None) == ffi.ERR_NULL
     assert lib.recon_tsdf_sample_colors(None, C.c_void_p(4096), 1, C.c_void_p(4096), None) == ffi.ERR_NULL
+
+
+def test_transformation_voxels_rejects_null_before_any_launch():
+    lib = ffi.load()
+    assert lib.recon_tsdf_transformation_voxels(None, C.c_void_p(4096), None) == ffi.ERR_NULL
