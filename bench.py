@@ -81,9 +81,10 @@ def cpu_baseline(w, depth, seconds: float) -> dict:
 
 def bench_small(steps: int, cpu: bool = True) -> dict:
     """The small CPU-runnable workload (128^3 volume, 8 mm voxels, 10 frames
-    of 640x480) in full: the GPU per-frame integrate (HIP events on the launch
-    stream, `steps` passes) beside the oracle's float32 integrate of all 10
-    frames on one host thread and on up to 16 threads."""
+    of 640x480) in full: the GPU per-frame integrate and the fused ten-frame
+    integrate (HIP events on the launch stream, `steps` passes each) beside
+    the oracle's float32 integrate of all 10 frames on one host thread and on
+    up to 16 threads."""
     import torch
 
     import recon
@@ -110,10 +111,24 @@ def bench_small(steps: int, cpu: bool = True) -> dict:
     e1.record(stream)
     torch.cuda.synchronize()
     pass_ms = e0.elapsed_time(e1) / steps
+
+    def fused_pass():
+        vol.reset(stream)
+        vol.integrate_frames(dev, w.poses, stream)
+
+    fused_pass()
+    e0.record(stream)
+    for _ in range(steps):
+        fused_pass()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fused_ms = e0.elapsed_time(e1) / steps
     vol.close()
     out = {"workload": w.name, "grid": list(w.grid), "frames": F, "image": list(w.image),
            "gpu_ms_per_pass": round(pass_ms, 4),
-           "gpu_gvu_s": round(w.num_voxels * F / (pass_ms / 1000.0) / 1e9, 3)}
+           "gpu_gvu_s": round(w.num_voxels * F / (pass_ms / 1000.0) / 1e9, 3),
+           "gpu_fused_ms_per_pass": round(fused_ms, 4),
+           "gpu_fused_gvu_s": round(w.num_voxels * F / (fused_ms / 1000.0) / 1e9, 3)}
     del dev
     if cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
