@@ -132,6 +132,11 @@ __device__ __forceinline__ bool bound_rect_reaches(const float* __restrict__ lvl
 // in front of the camera, the hull of their projections bounds the voxels'
 // pixel coordinates.  Two pixels of slack cover float32 rounding, the
 // hardware reciprocal used here (<= 1 ulp) and the truncation to an index.
+// A footprint of more than kMaxFine 8x8 blocks keeps the box after the
+// 32x32 test: the fine scan is a chain of dependent loads, and on a cull
+// launch that fits the device in one pass its longest chain is the kernel's
+// duration.
+template <int kMaxFine = 96>
 __device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const FrameParams& F,
                                               const float lo[3], const float hi[3]) {
     float zmin = 3.0e38f, umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
@@ -163,13 +168,13 @@ __device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const Fram
                             iv1 / kBound1, need))
         return false;
     const int bx0 = iu0 / kBound0, bx1 = iu1 / kBound0, by0 = iv0 / kBound0, by1 = iv1 / kBound0;
-    if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 96) return true;
+    if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > kMaxFine) return true;
     return bound_rect_reaches(F.bound0, vp.b0x, bx0, bx1, by0, by1, need);
 }
 
 // Frames (bit f of the result) whose frustum and depth bound reach wave-tile
 // `id` of a volume; only frames first, first + step, ... are tested.
-template <class S, class Frames>
+template <class S, class Frames, int kMaxFine = 96>
 __device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const int nframes,
                                                 const Frames& frames, uint32_t id,
                                                 const int first = 0, const int step = 1) {
@@ -188,7 +193,7 @@ __device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const in
         bool in = true;
 #pragma unroll
         for (int k = 0; k < 6; ++k) in = in && (plane_max(F.plane[k], lo, hi) >= 0.0f);
-        if (in && depth_reaches(vp, F, lo, hi)) live |= 1ull << f;
+        if (in && depth_reaches<kMaxFine>(vp, F, lo, hi)) live |= 1ull << f;
     }
     return live;
 }
@@ -218,6 +223,89 @@ __device__ __forceinline__ void append_tile(uint64_t live, uint32_t tile, uint32
     if (threadIdx.x == 0) *s_base = total ? atomicAdd(list_count, total) : 0u;
     __syncthreads();
     if (n) list[*s_base + wave_base + incl - n] = ListEntry{tile, volume, live};
+}
+
+// ---- segmented cull list (no global counter) -------------------------------------
+// A cull workgroup owns a segment of the list (the entries of its own tiles,
+// at most one per tile) and writes only its count: no returning atomic, whose
+// round trip would sit on every workgroup's critical path.  The consumer
+// turns the counts into segment offsets in LDS and maps a dense entry index
+// to its list slot with a binary search.
+// Appends the workgroup's surviving tiles (`live` != 0) to `out` (the next
+// free slot of its segment) in thread order; returns their number.  Every
+// thread of the workgroup must call; s_wave has kThreads/64 entries.
+__device__ __forceinline__ unsigned append_local(uint64_t live, uint32_t tile, uint32_t volume,
+                                                 ListEntry* __restrict__ out, unsigned* s_wave) {
+    const unsigned n = live != 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    unsigned wave_base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i) {
+        wave_base += i < wave ? s_wave[i] : 0u;
+        total += s_wave[i];
+    }
+    if (n) out[wave_base + incl - n] = ListEntry{tile, volume, live};
+    __syncthreads();  // s_wave is reused by the next call
+    return total;
+}
+
+// s_pre[s] = counts[0] + ... + counts[s-1] for s = 0..nseg (nseg <=
+// kMaxSegments); returns s_pre[nseg].  Every thread of the workgroup must
+// call; s_wave has kThreads/64 entries.
+__device__ __forceinline__ unsigned segment_offsets(const unsigned* __restrict__ counts, int nseg,
+                                                    unsigned* s_pre, unsigned* s_wave) {
+    for (int i = threadIdx.x; i < nseg; i += kThreads) s_pre[i] = counts[i];
+    __syncthreads();
+    const int per = (nseg + kThreads - 1) / kThreads;
+    const int b = threadIdx.x * per, e = min(b + per, nseg);
+    unsigned sum = 0;
+    for (int i = b; i < e; ++i) sum += s_pre[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    unsigned run = incl - sum, total = 0;
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i) {
+        run += i < wave ? s_wave[i] : 0u;
+        total += s_wave[i];
+    }
+    for (int i = b; i < e; ++i) {
+        const unsigned c = s_pre[i];
+        s_pre[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 0) s_pre[nseg] = total;
+    __syncthreads();
+    return total;
+}
+
+// List slot of dense entry i (< s_pre[nseg]): segment s with s_pre[s] <= i <
+// s_pre[s+1] (the last s with s_pre[s] <= i, which skips empty segments),
+// then i - s_pre[s] entries into it.  Every lane takes the same number of
+// steps.
+__device__ __forceinline__ unsigned segment_slot(const unsigned* s_pre, int nseg, unsigned seg_len,
+                                                 unsigned i) {
+    unsigned lo = 0, len = (unsigned)nseg;
+    while (len > 1) {
+        const unsigned half = len >> 1;
+        if (s_pre[lo + half] <= i) lo += half;
+        len -= half;
+    }
+    return lo * seg_len + (i - s_pre[lo]);
 }
 
 // ---- the per-voxel update -------------------------------------------------------

@@ -84,6 +84,14 @@ static bool finite_pos(float v) { return std::isfinite(v) && v > 0.0f; }
 
 // TSDFVolume::TSDFVolume (tsdf_volume.cu:194-199): voxel_size = physical /
 // grid (float3 / dim3, cudaMath.cuh:1225-1228), truncation 1.1*|voxel_size|.
+int ensure_cull_list(recon_tsdf* h, size_t ntiles, size_t entry_bytes) {
+    if (h->list) return RECON_OK;
+    RECON_HIP_TRY(hipMalloc(&h->list, ntiles * entry_bytes + sizeof(unsigned) * kMaxSegments),
+                  "cull list");
+    h->list_count = reinterpret_cast<unsigned*>(static_cast<char*>(h->list) + ntiles * entry_bytes);
+    return RECON_OK;
+}
+
 int describe(const recon_tsdf_config* cfg, recon_tsdf_info* info) {
     if (!cfg || !info) return fail(RECON_ERR_NULL, "recon_tsdf_describe: null argument");
     int64_t n = 1;

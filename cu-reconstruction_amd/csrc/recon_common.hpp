@@ -72,6 +72,9 @@ struct FrameParams {
 };
 
 constexpr int kBound0 = 8, kBound1 = 32;  // pixel block edges of the two bound levels
+// Segments of a segmented cull list (one count word each after the entries);
+// the consumer keeps their offsets in 16 KB of LDS.
+constexpr int kMaxSegments = 4096;
 
 struct VolumeParams {
     float* D;
@@ -156,8 +159,8 @@ struct recon_tsdf {
     // per-frame depth bounds, grown on demand
     float* bounds = nullptr;
     size_t bounds_floats = 0;
-    void* list = nullptr;            // cull list entries
-    unsigned* list_count = nullptr;  // inside `list`
+    void* list = nullptr;            // cull list: one entry per wave-tile, then
+    unsigned* list_count = nullptr;  // kMaxSegments count words (inside `list`)
     // marching-cubes scratch, grown on demand
     void* mc_scratch = nullptr;
     size_t mc_scratch_bytes = 0;
@@ -174,6 +177,10 @@ namespace recon {
 // Validates a configuration and derives voxel size / truncation / max depth
 // (tsdf_volume.cu:194-199).  Host only, no HIP calls.
 int describe(const recon_tsdf_config* cfg, recon_tsdf_info* info);
+
+// Allocates the volume's cull list (`ntiles` entries of `entry_bytes`, then
+// the count words) on first use.
+int ensure_cull_list(recon_tsdf* h, size_t ntiles, size_t entry_bytes);
 
 // Marching cubes over a dense field; `scratch` / `scratch_bytes` may point
 // at a reusable buffer owned by the caller (grown by the callee).

@@ -451,12 +451,7 @@ static int ensure_diff_scratch(recon_tsdf* h, size_t bytes, hipStream_t s) {
 }
 
 static int ensure_list(recon_tsdf* h, const VolumeParams& vp) {
-    if (h->list) return RECON_OK;
-    const size_t ntiles = (size_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    RECON_HIP_TRY(hipMalloc(&h->list, ntiles * sizeof(ListEntry) + 256), "cull list");
-    h->list_count = reinterpret_cast<unsigned*>(static_cast<char*>(h->list) +
-                                                ntiles * sizeof(ListEntry));
-    return RECON_OK;
+    return ensure_cull_list(h, (size_t)vp.tiles_x * vp.tiles_y * vp.tiles_z, sizeof(ListEntry));
 }
 
 static VolumeParams diff_volume_params(const recon_tsdf* h, int cols, int rows) {

@@ -7,13 +7,14 @@
 // coalesces) and re-reads a 28-byte voxel-centre table per voxel.  Here one
 // integrate call is three launches on the caller's stream:
 //   1. depth_bounds_kernel  per 8x8 and 32x32 pixel block, the largest valid
-//                           depth of each frame (and the list counter <- 0);
+//                           depth of each frame;
 //   2. cull_kernel          one thread per wave-tile (32 x 8 x 2 voxels):
 //                           the tile survives a frame if its voxel-centre box
 //                           meets the frame's frustum and the depth bound of
 //                           its image footprint is not closer than the box
-//                           minus the truncation; survivors are appended to a
-//                           compact list with their frame mask;
+//                           minus the truncation; each workgroup compacts its
+//                           survivors (with their frame masks) into its own
+//                           segment of the list and writes their count;
 //   3. integrate_list_kernel  a grid sized to the resident wave slots walks
 //                           the list, one wave-tile per wavefront: lane l owns
 //                           voxels x..x+3 of row y for two z, so a wave moves
@@ -38,11 +39,9 @@ namespace recon {
 using Shape = DefaultShape;
 
 __global__ __launch_bounds__(256) void depth_bounds_kernel(const BoundsJob job, size_t frame_stride,
-                                                           size_t bounds_stride,
-                                                           unsigned* list_count) {
+                                                           size_t bounds_stride) {
     __shared__ int s0[16];
     __shared__ int s1;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) *list_count = 0;
     BoundsJob j = job;
     j.depth += frame_stride * blockIdx.z;
     j.bound0 += bounds_stride * blockIdx.z;
@@ -50,38 +49,64 @@ __global__ __launch_bounds__(256) void depth_bounds_kernel(const BoundsJob job, 
     bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
 }
 
+// Footprints wider than this many 8x8 depth blocks skip the fine bound test
+// (the tile is kept after the 32x32 test).
+constexpr int kCullMaxFine = 24;
+
+// Workgroup b tests tiles [b*seg_len, (b+1)*seg_len) (seg_len a multiple of
+// 256, one tile per thread per pass), compacts the survivors into the same
+// slots of the list in tile order and writes their number to counts[b].
 template <int MAXF>
 __global__ __launch_bounds__(256) void cull_kernel(const VolumeParams vp, const int nframes,
                                                    const FrameBlock<MAXF> fb,
                                                    ListEntry* __restrict__ list,
-                                                   unsigned* __restrict__ list_count) {
+                                                   unsigned* __restrict__ counts,
+                                                   const unsigned seg_len) {
     __shared__ unsigned s_wave[kThreads / 64];
-    __shared__ unsigned s_base;
     const uint32_t ntiles = (uint32_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-    const uint64_t live = id < ntiles ? tile_frames<Shape>(vp, nframes, fb, id) : 0ull;
-    append_tile(live, id, 0u, list, list_count, s_wave, &s_base);
+    const uint32_t first = blockIdx.x * seg_len;
+    const uint32_t end = min(first + seg_len, ntiles);
+    unsigned n = 0;
+    for (uint32_t k = first; k < end; k += kThreads) {
+        const uint32_t id = k + threadIdx.x;
+        const uint64_t live =
+            id < end ? tile_frames<Shape, FrameBlock<MAXF>, kCullMaxFine>(vp, nframes, fb, id) : 0ull;
+        n += append_local(live, id, 0u, list + first + n, s_wave);
+    }
+    if (threadIdx.x == 0) counts[blockIdx.x] = n;
 }
 
-// Each wavefront takes list entries i = wave, wave + waves, ...; the grid is
-// sized to the resident wave slots, so every wave-tile costs about the same
-// and the device stays full until the list is done.
+// The list's entries in dense order i = 0 .. total-1 (segment by segment):
+// each wavefront takes i = wave, wave + waves, ... (the grid is sized to the
+// resident wave slots, so every wave-tile costs about the same and the device
+// stays full until the list is done); lane j of a wave locates the list slot
+// of the wave's j-th next entry, so one search serves 64 entries.
 template <int MAXF, bool VEC, bool COUNT, bool COLOR = false, bool VARW = false>
 __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
     const VolumeParams vp, const int nframes, const FrameBlock<MAXF> fb,
-    const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count,
-    unsigned long long* counter) {
-    const unsigned n = __builtin_amdgcn_readfirstlane(*list_count);
+    const ListEntry* __restrict__ list, const unsigned* __restrict__ counts, const int nseg,
+    const unsigned seg_len, unsigned long long* counter) {
+    __shared__ unsigned s_pre[kMaxSegments + 1];
+    __shared__ unsigned s_wave[kThreads / 64];
+    const unsigned n = __builtin_amdgcn_readfirstlane(segment_offsets(counts, nseg, s_pre, s_wave));
     const unsigned waves = gridDim.x * (kThreads / 64);
+    const unsigned lane = threadIdx.x & 63;
     unsigned cnt = 0;
-    for (unsigned i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < n; i += waves) {
-        const uint32_t tile = __builtin_amdgcn_readfirstlane(list[i].tile);
-        const uint64_t frames = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(list[i].frames >> 32)) << 32) |
-                                __builtin_amdgcn_readfirstlane((uint32_t)list[i].frames);
-        int x0, y0, z0;
-        tile_origin<Shape>(vp, tile, x0, y0, z0);
-        integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR, VARW>(vp, nframes, fb, frames, x0,
-                                                                         y0, z0, cnt);
+    for (unsigned i0 = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i0 < n; i0 += 64 * waves) {
+        const unsigned mine = i0 + lane * waves;
+        const unsigned slot = mine < n ? segment_slot(s_pre, nseg, seg_len, mine) : 0u;
+        const unsigned m = min(64u, (n - i0 + waves - 1) / waves);
+        for (unsigned j = 0; j < m; ++j) {
+            const unsigned e = __builtin_amdgcn_readfirstlane(__shfl(slot, (int)j, 64));
+            const uint32_t tile = __builtin_amdgcn_readfirstlane(list[e].tile);
+            const uint64_t frames =
+                ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(list[e].frames >> 32)) << 32) |
+                __builtin_amdgcn_readfirstlane((uint32_t)list[e].frames);
+            int x0, y0, z0;
+            tile_origin<Shape>(vp, tile, x0, y0, z0);
+            integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR, VARW>(vp, nframes, fb, frames,
+                                                                             x0, y0, z0, cnt);
+        }
     }
     if (COUNT && cnt) atomicAdd(counter, (unsigned long long)cnt);
 }
@@ -143,7 +168,7 @@ static int check_depth(const recon_depth_view* depth, const char* fn) {
 }
 
 // Grows the handle's per-call scratch (depth bounds of kMaxFusedFrames frames,
-// the cull list and its counter) to fit `vp`.  A buffer being replaced may
+// the cull list and its segment counts) to fit `vp`.  A buffer being replaced may
 // still be read by queued kernels, so the stream is drained first.
 static int ensure_scratch(recon_tsdf* h, const VolumeParams& vp, hipStream_t s) {
     const size_t per = (size_t)vp.b0x * vp.b0y + (size_t)vp.b1x * vp.b1y;
@@ -158,25 +183,35 @@ static int ensure_scratch(recon_tsdf* h, const VolumeParams& vp, hipStream_t s) 
         RECON_HIP_TRY(hipMalloc((void**)&h->bounds, need * sizeof(float)), "depth bounds scratch");
         h->bounds_floats = need;
     }
-    if (!h->list) {
-        const size_t ntiles = (size_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
-        RECON_HIP_TRY(hipMalloc(&h->list, ntiles * sizeof(ListEntry) + 256), "cull list");
-        h->list_count = reinterpret_cast<unsigned*>(static_cast<char*>(h->list) +
-                                                    ntiles * sizeof(ListEntry));
-    }
-    return RECON_OK;
+    return ensure_cull_list(h, (size_t)vp.tiles_x * vp.tiles_y * vp.tiles_z, sizeof(ListEntry));
 }
+
+// Cull segments: 256 tiles each, or the multiple of 256 that keeps their
+// number within kMaxSegments.
+static void cull_segments(int ntiles, unsigned* seg_len, int* nseg) {
+    const unsigned need = (unsigned)((ntiles + kMaxSegments - 1) / kMaxSegments);
+    *seg_len = std::max<unsigned>(kThreads, (need + kThreads - 1) / kThreads * kThreads);
+    *nseg = (int)((ntiles + *seg_len - 1) / *seg_len);
+}
+
+struct ListArgs {
+    const ListEntry* list;
+    const unsigned* counts;
+    int nseg;
+    unsigned seg_len;
+};
 
 template <int MAXF, bool VEC, bool COUNT, bool COLOR>
 static void launch_list(bool varw, dim3 grid, hipStream_t s, const VolumeParams& vp, int nframes,
-                        const FrameBlock<MAXF>& fb, const ListEntry* list, const unsigned* count,
-                        unsigned long long* ctr) {
+                        const FrameBlock<MAXF>& fb, const ListArgs& la, unsigned long long* ctr) {
     if (varw)
         hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, true>), grid,
-                           dim3(kThreads), 0, s, vp, nframes, fb, list, count, ctr);
+                           dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
+                           la.seg_len, ctr);
     else
         hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, false>), grid,
-                           dim3(kThreads), 0, s, vp, nframes, fb, list, count, ctr);
+                           dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
+                           la.seg_len, ctr);
 }
 
 // The three launches of one integrate sweep over `nframes` frames (frame f at
@@ -205,12 +240,15 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     job.b1y = vp.b1y;
     job.maxd = vp.maxd;
     hipLaunchKernelGGL(depth_bounds_kernel, dim3(vp.b1x, vp.b1y, nframes), dim3(256), 0, s, job,
-                       stride_floats, per, h->list_count);
+                       stride_floats, per);
     RECON_HIP_TRY(hipGetLastError(), "depth_bounds_kernel launch");
     const int ntiles = vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    ListEntry* list = static_cast<ListEntry*>(h->list);
-    hipLaunchKernelGGL((cull_kernel<MAXF>), dim3((ntiles + 255) / 256), dim3(256), 0, s, vp,
-                       nframes, fb, list, h->list_count);
+    ListArgs la;
+    la.list = static_cast<ListEntry*>(h->list);
+    la.counts = h->list_count;
+    cull_segments(ntiles, &la.seg_len, &la.nseg);
+    hipLaunchKernelGGL((cull_kernel<MAXF>), dim3(la.nseg), dim3(256), 0, s, vp, nframes, fb,
+                       static_cast<ListEntry*>(h->list), h->list_count, la.seg_len);
     RECON_HIP_TRY(hipGetLastError(), "cull_kernel launch");
     const bool vec = (vp.gx % 4) == 0;
     const bool varw = h->info.weight_type == RECON_WEIGHT_BY_VARIANCE;
@@ -221,20 +259,20 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     if (color) {
         if constexpr (MAXF == 1) {
             if (vec)
-                launch_list<1, true, false, true>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+                launch_list<1, true, false, true>(varw, grid, s, vp, nframes, fb, la, ctr);
             else
-                launch_list<1, false, false, true>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+                launch_list<1, false, false, true>(varw, grid, s, vp, nframes, fb, la, ctr);
         }
     } else if (ctr) {
         if (vec)
-            launch_list<MAXF, true, true, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, true, true, false>(varw, grid, s, vp, nframes, fb, la, ctr);
         else
-            launch_list<MAXF, false, true, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, false, true, false>(varw, grid, s, vp, nframes, fb, la, ctr);
     } else {
         if (vec)
-            launch_list<MAXF, true, false, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, true, false, false>(varw, grid, s, vp, nframes, fb, la, ctr);
         else
-            launch_list<MAXF, false, false, false>(varw, grid, s, vp, nframes, fb, list, h->list_count, ctr);
+            launch_list<MAXF, false, false, false>(varw, grid, s, vp, nframes, fb, la, ctr);
     }
     RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
     return RECON_OK;
