@@ -145,3 +145,26 @@ def test_cull_fused_random(cuda):
     w.poses = np.concatenate([w.poses] * 4)
     depth = np.concatenate([depth] * 4)
     _check(w, depth, fused=True, f64=False)
+
+
+@pytest.mark.parametrize("gz", [256, 258])
+def test_cull_segment_limits(cuda, gz):
+    # 2048 x 1024 x gz voxels (4 GB of D and W): 64 x 128 x gz/2 wave-tiles of
+    # 32 x 8 x 2.  gz = 256 gives 4096 x 256 tiles, the most cull segments of
+    # 256 tiles; gz = 258 gives more, so segments of 512 tiles.  Per-frame and
+    # fused calls, bit-exact, update counts equal.
+    w = scenes.tiny(grid=(2048, 1024, gz), voxel_size=0.001, image=(320, 240), num_frames=2)
+    cfg = w.config()
+    depth = w.render()
+    o = oracle_run(cfg, depth, w.poses)
+    assert min(o["counts"]) > 100_000_000
+    for fused in (False, True):
+        g = gpu_run(cfg, depth, w.poses, fused=fused, count=True)
+        assert_bitexact(g["W"], o["W"], "W")
+        assert_bitexact(g["D"], o["D"], "D")
+        if fused:
+            assert int(g["counts"][0]) == int(sum(o["counts"]))
+        else:
+            assert g["counts"].tolist() == [int(c) for c in o["counts"]]
+        g["vol"].close()
+        del g
