@@ -41,6 +41,57 @@ __device__ __forceinline__ void load_corners(const McParams& p, int x, int y, in
     v[7] = p.f[i0 + sy + sz];
 }
 
+// The corners of the cubes of one cube row, 64 at a time, for a wavefront
+// walking the row (every lane calls every step).  Each of the row's four
+// voxel rows is read once: step k loads voxels x0+64 .. x0+127 one step
+// ahead, a cube's x+1 corners come from the next lane (a DPP wave shift) and
+// lane 63's from lane 0 of that next load.  The values are load_corners'.
+struct CornerStream {
+    const float* f;
+    int gx;
+    int rows[4];  // voxel rows (y,z), (y,z+1), (y+1,z), (y+1,z+1)
+    float cur[4];
+
+    __device__ __forceinline__ CornerStream(const McParams& p, int y, int z, int lane) {
+        f = p.f;
+        gx = p.gx;
+        const int r0 = p.gx * y + p.gx * p.gy * z;
+        rows[0] = r0;
+        rows[1] = r0 + p.gx * p.gy;
+        rows[2] = r0 + p.gx;
+        rows[3] = r0 + p.gx + p.gx * p.gy;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = lane < gx ? f[rows[k] + lane] : 0.0f;
+    }
+
+    // Corners of cube (x0 + lane, y, z), x0 = 0, 64, 128, ... in turn;
+    // defined for lanes with x0 + lane < gx - 1.
+    __device__ __forceinline__ void step(int x0, int lane, float v[8]) {
+        const int xn = x0 + 64 + lane;
+        float nxt[4], hi[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nxt[k] = xn < gx ? f[rows[k] + xn] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // wave_shl:1 -- lane i receives lane i+1
+            const float sh = __int_as_float(
+                __builtin_amdgcn_update_dpp(0, __float_as_int(cur[k]), 0x130, 0xF, 0xF, false));
+            const float first = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nxt[k]), 0));
+            hi[k] = lane == 63 ? first : sh;
+        }
+        v[0] = cur[0];
+        v[1] = hi[0];
+        v[2] = hi[1];
+        v[3] = cur[1];
+        v[4] = cur[2];
+        v[5] = hi[2];
+        v[6] = hi[3];
+        v[7] = cur[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+    }
+};
+
 // define_voxel_type (marching_cubes.cu:76-84)
 __device__ __forceinline__ int cube_class(const float v[8]) {
     int c = 0;
@@ -75,10 +126,11 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 __device__ __forceinline__ int mc_row_triangles(const McParams& p, int r, int lane) {
     const int y = r % p.cyr, z = r / p.cyr;
     int n = 0;
-    for (int x = lane; x < p.cx; x += 64) {
+    CornerStream cs(p, y, z, lane);
+    for (int x0 = 0; x0 < p.cx; x0 += 64) {
         float v[8];
-        load_corners(p, x, y, z, v);
-        n += class_ntri(cube_class(v));
+        cs.step(x0, lane, v);
+        if (x0 + lane < p.cx) n += class_ntri(cube_class(v));
     }
     return wave_sum(n);
 }
@@ -121,12 +173,13 @@ __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, 
                                             float* __restrict__ verts, int32_t* __restrict__ tris,
                                             long long capacity, long long tri_index_base) {
     const int y = r % p.cyr, z = r / p.cyr;
+    CornerStream cs(p, y, z, lane);
     for (int x0 = 0; x0 < p.cx; x0 += 64) {
         const int x = x0 + lane;
         float v[8];
         int c = 0, nt = 0;
+        cs.step(x0, lane, v);
         if (x < p.cx) {
-            load_corners(p, x, y, z, v);
             c = cube_class(v);
             nt = class_ntri(c);
         }
