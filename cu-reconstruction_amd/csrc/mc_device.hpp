@@ -43,14 +43,17 @@ __device__ __forceinline__ void load_corners(const McParams& p, int x, int y, in
 
 // The corners of the cubes of one cube row, 64 at a time, for a wavefront
 // walking the row (every lane calls every step).  Each of the row's four
-// voxel rows is read once: step k loads voxels x0+64 .. x0+127 one step
-// ahead, a cube's x+1 corners come from the next lane (a DPP wave shift) and
-// lane 63's from lane 0 of that next load.  The values are load_corners'.
+// voxel rows is read once, two steps ahead (the step at x0 issues the loads
+// of x0+128 .. x0+191), so a step never waits on a load it has just issued.
+// A cube's x+1 corners come from the next lane (a DPP wave shift) and lane
+// 63's from lane 0 of the following 64 voxels.  The values are
+// load_corners'.
 struct CornerStream {
     const float* f;
     int gx;
     int rows[4];  // voxel rows (y,z), (y,z+1), (y+1,z), (y+1,z+1)
     float cur[4];
+    float nx[4];
 
     __device__ __forceinline__ CornerStream(const McParams& p, int y, int z, int lane) {
         f = p.f;
@@ -62,15 +65,20 @@ struct CornerStream {
         rows[3] = r0 + p.gx + p.gx * p.gy;
 #pragma unroll
         for (int k = 0; k < 4; ++k) cur[k] = lane < gx ? f[rows[k] + lane] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nx[k] = lane + 64 < gx ? f[rows[k] + lane + 64] : 0.0f;
     }
 
     // Corners of cube (x0 + lane, y, z), x0 = 0, 64, 128, ... in turn;
     // defined for lanes with x0 + lane < gx - 1.
     __device__ __forceinline__ void step(int x0, int lane, float v[8]) {
-        const int xn = x0 + 64 + lane;
-        float nxt[4], hi[4];
+        const int xn = x0 + 128 + lane;
+        float nn[4], hi[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nxt[k] = xn < gx ? f[rows[k] + xn] : 0.0f;
+        for (int k = 0; k < 4; ++k) nn[k] = xn < gx ? f[rows[k] + xn] : 0.0f;
+        float nxt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nxt[k] = nx[k];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             // wave_shl:1 -- lane i receives lane i+1
@@ -88,7 +96,10 @@ struct CornerStream {
         v[6] = hi[3];
         v[7] = cur[3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+        for (int k = 0; k < 4; ++k) {
+            cur[k] = nxt[k];
+            nx[k] = nn[k];
+        }
     }
 };
 

@@ -84,8 +84,22 @@ __global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
     const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
     if (row >= w.mc.gy * w.mc.gz) return;
     const McParams& p = w.mc;
+    const int y = row % p.gy, z = row / p.gy;
+    // rows with a +y and a +z neighbour row stream their corners
+    const bool interior = y < p.gy - 1 && z < p.gz - 1;
+    CornerStream cs(p, interior ? y : 0, interior ? z : 0, lane);
     for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
-        const ChunkCorners k = chunk_corners(w, c, lane);
+        ChunkCorners k;
+        if (interior) {
+            k.x = (c - row * w.cpr) * 64 + lane;
+            k.y = y;
+            k.z = z;
+            k.in = k.x < p.gx;
+            k.cube = k.x < p.gx - 1;
+            cs.step(k.x - lane, lane, k.v);
+        } else {
+            k = chunk_corners(w, c, lane);
+        }
         bool bx = false, by = false, bz = false;
         int nt = 0;
         if (k.in) {
