@@ -72,10 +72,10 @@ __device__ __forceinline__ int last_le(const int* starts, int n, int b) {
 
 __global__ __launch_bounds__(256) void dbatch_bounds_kernel(const BoundsJob* __restrict__ jobs) {
     __shared__ int s0[16];
-    __shared__ int s1;
+    __shared__ int s1[4];
     const BoundsJob j = jobs[blockIdx.z];
     if ((int)blockIdx.x >= j.b1x || (int)blockIdx.y >= j.b1y) return;  // whole workgroup
-    bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
+    bounds_block(j, blockIdx.x, blockIdx.y, s0, s1);
 }
 
 __global__ __launch_bounds__(64) void dbatch_setup_kernel(const DVol* __restrict__ vols,

@@ -73,37 +73,50 @@ struct BoundsJob {
     int32_t pitch, cols, rows;
     int32_t b0x, b0y, b1x, b1y;
     float maxd;
+    int32_t vec = 0;  // depth and pitch 16-byte aligned: four pixels per load
 };
 
+__device__ __forceinline__ int bounds_valid(float d, float maxd) {
+    return (d > 0.0f && d <= maxd) ? __float_as_int(d) : 0;
+}
+
+// Thread t reads pixels 4*(t%8) .. +3 of row t/8 of the block, so wave w
+// holds pixel rows 8w .. 8w+7: one row of four 8x8 sub-blocks, sub-block
+// (t%8)/2.  Maxima are taken across lanes (no LDS atomics): s0 receives the
+// 16 sub-block maxima and s1 the 4 wave maxima.
 __device__ __forceinline__ void bounds_block(const BoundsJob& j, int bx, int by, int* s0, int* s1) {
-    const int t = threadIdx.x;
-    if (t < 16) s0[t] = 0;
-    if (t == 0) *s1 = 0;
-    __syncthreads();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r = t >> 3, c4 = (t & 7) * 4;
     const int y = by * kBound1 + r;
     const int xb = bx * kBound1 + c4;
     int m = 0;
     if (y < j.rows) {
+        const float* row = j.depth + (size_t)y * j.pitch;
+        if (j.vec && xb + 3 < j.cols) {
+            const float4 d = *reinterpret_cast<const float4*>(row + xb);
+            m = max(max(bounds_valid(d.x, j.maxd), bounds_valid(d.y, j.maxd)),
+                    max(bounds_valid(d.z, j.maxd), bounds_valid(d.w, j.maxd)));
+        } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int x = xb + k;
-            if (x < j.cols) {
-                const float d = j.depth[y * j.pitch + x];
-                if (d > 0.0f && d <= j.maxd) m = max(m, __float_as_int(d));
-            }
+            for (int k = 0; k < 4; ++k)
+                if (xb + k < j.cols) m = max(m, bounds_valid(row[xb + k], j.maxd));
         }
     }
-    if (m) {
-        atomicMax(&s0[(r >> 3) * 4 + (c4 >> 3)], m);
-        atomicMax(s1, m);
-    }
+    // lanes of one sub-block share lane bits 1-2
+    m = max(m, __shfl_xor(m, 1, 64));
+    m = max(m, __shfl_xor(m, 8, 64));
+    m = max(m, __shfl_xor(m, 16, 64));
+    m = max(m, __shfl_xor(m, 32, 64));
+    if (lane < 8 && (lane & 1) == 0) s0[wave * 4 + (lane >> 1)] = m;
+    m = max(m, __shfl_xor(m, 2, 64));
+    m = max(m, __shfl_xor(m, 4, 64));
+    if (lane == 0) s1[wave] = m;
     __syncthreads();
     if (t < 16) {
         const int ox = bx * 4 + (t & 3), oy = by * 4 + (t >> 2);
         if (ox < j.b0x && oy < j.b0y) j.bound0[oy * j.b0x + ox] = __int_as_float(s0[t]);
     }
-    if (t == 0) j.bound1[by * j.b1x + bx] = __int_as_float(*s1);
+    if (t == 0) j.bound1[by * j.b1x + bx] = __int_as_float(max(max(s1[0], s1[1]), max(s1[2], s1[3])));
 }
 
 // ---- culling ------------------------------------------------------------------

@@ -72,15 +72,24 @@ __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchVolParams* _
     fill4(vp.D, vp.W, (long long)(b - block_start[lo]) * kFillBlock + 4 * threadIdx.x, n, vp.trunc);
 }
 
+// One workgroup per 32x32 block of every frame: job_start[f] is the first
+// workgroup of frame f (frames of any size, no idle workgroups).
 __global__ __launch_bounds__(256) void batch_bounds_kernel(const BoundsJob* __restrict__ jobs,
-                                                           unsigned* counters, int ncounters) {
+                                                           const int* __restrict__ job_start,
+                                                           int njobs, unsigned* counters,
+                                                           int ncounters) {
     __shared__ int s0[16];
-    __shared__ int s1;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && (int)threadIdx.x < ncounters)
-        counters[threadIdx.x] = 0;
-    const BoundsJob j = jobs[blockIdx.z];
-    if ((int)blockIdx.x >= j.b1x || (int)blockIdx.y >= j.b1y) return;  // whole workgroup
-    bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
+    __shared__ int s1[4];
+    if (blockIdx.x == 0 && (int)threadIdx.x < ncounters) counters[threadIdx.x] = 0;
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;  // last frame whose first workgroup <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (job_start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const BoundsJob j = jobs[lo];
+    const int k = b - job_start[lo];
+    bounds_block(j, k % j.b1x, k / j.b1x, s0, s1);
 }
 
 template <class S>
@@ -265,7 +274,8 @@ struct recon_batch {
     std::vector<BatchVolParams> h_vols;
     FrameParams* d_frames = nullptr;
     BoundsJob* d_jobs = nullptr;
-    int total_frames = 0, max_b1x = 1, max_b1y = 1;
+    int* d_job_start = nullptr;  // first bounds workgroup of each frame
+    int total_frames = 0, bounds_blocks = 0;
     int* d_fill_start = nullptr;
     int fill_blocks = 0;
     struct Group {
@@ -320,8 +330,8 @@ int recon_batch::record(hipStream_t s) {
     hipLaunchKernelGGL(batch_fill_kernel, dim3(fill_blocks), dim3(256), 0, s, d_vols, d_fill_start, n);
     RECON_HIP_TRY(hipGetLastError(), "batch_fill_kernel launch");
     if (total_frames > 0) {
-        hipLaunchKernelGGL(batch_bounds_kernel, dim3(max_b1x, max_b1y, total_frames), dim3(256), 0, s,
-                           d_jobs, d_counts, (int)groups.size());
+        hipLaunchKernelGGL(batch_bounds_kernel, dim3(bounds_blocks), dim3(256), 0, s, d_jobs,
+                           d_job_start, total_frames, d_counts, (int)groups.size());
         RECON_HIP_TRY(hipGetLastError(), "batch_bounds_kernel launch");
         for (size_t g = 0; g < groups.size(); ++g) {
             const Group& G = groups[g];
@@ -403,6 +413,7 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
     // volume and frame tables
     std::vector<FrameParams> frames;
     std::vector<BoundsJob> jobs;
+    std::vector<int> job_start;
     size_t bounds_floats = 0;
     for (int i = 0; i < n; ++i) {
         const recon_tsdf_info& in = b->info[i];
@@ -460,10 +471,11 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
             j.b1x = vp.b1x;
             j.b1y = vp.b1y;
             j.maxd = vp.maxd;
+            j.vec = (reinterpret_cast<uintptr_t>(d) % 16 == 0 && j.pitch % 4 == 0) ? 1 : 0;
             jobs.push_back(j);
+            job_start.push_back(b->bounds_blocks);
+            b->bounds_blocks += vp.b1x * vp.b1y;
             bounds_floats += (size_t)vp.b0x * vp.b0y + (size_t)vp.b1x * vp.b1y;
-            b->max_b1x = std::max(b->max_b1x, vp.b1x);
-            b->max_b1y = std::max(b->max_b1y, vp.b1y);
         }
         b->h_vols.push_back(bv);
     }
@@ -487,6 +499,7 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
     if ((rc = b->upload(&b->d_vols, b->h_vols, "batch volume table")) ||
         (rc = b->upload(&b->d_frames, frames, "batch frame table")) ||
         (rc = b->upload(&b->d_jobs, jobs, "batch bounds jobs")) ||
+        (rc = b->upload(&b->d_job_start, job_start, "batch bounds job starts")) ||
         (rc = b->upload(&b->d_fill_start, fill_start, "batch fill table")))
         return fail_out(rc);
     // cull groups: one per variant present

@@ -67,13 +67,13 @@ __global__ __launch_bounds__(256) void diff_bounds_kernel(const BoundsJob job, s
                                                           size_t bounds_stride,
                                                           unsigned* list_count) {
     __shared__ int s0[16];
-    __shared__ int s1;
+    __shared__ int s1[4];
     if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) *list_count = 0;
     BoundsJob j = job;
     j.depth += frame_stride * blockIdx.z;
     j.bound0 += bounds_stride * blockIdx.z;
     j.bound1 += bounds_stride * blockIdx.z;
-    bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
+    bounds_block(j, blockIdx.x, blockIdx.y, s0, s1);
 }
 
 // 64 tiles per block; wave g tests frames g, g + 4, ... of every tile, and

@@ -41,12 +41,12 @@ using Shape = DefaultShape;
 __global__ __launch_bounds__(256) void depth_bounds_kernel(const BoundsJob job, size_t frame_stride,
                                                            size_t bounds_stride) {
     __shared__ int s0[16];
-    __shared__ int s1;
+    __shared__ int s1[4];
     BoundsJob j = job;
     j.depth += frame_stride * blockIdx.z;
     j.bound0 += bounds_stride * blockIdx.z;
     j.bound1 += bounds_stride * blockIdx.z;
-    bounds_block(j, blockIdx.x, blockIdx.y, s0, &s1);
+    bounds_block(j, blockIdx.x, blockIdx.y, s0, s1);
 }
 
 // Footprints wider than this many 8x8 depth blocks skip the fine bound test
@@ -239,6 +239,8 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     job.b1x = vp.b1x;
     job.b1y = vp.b1y;
     job.maxd = vp.maxd;
+    job.vec = (reinterpret_cast<uintptr_t>(depth0) % 16 == 0 && pitch % 4 == 0 && stride_floats % 4 == 0)
+                  ? 1 : 0;
     hipLaunchKernelGGL(depth_bounds_kernel, dim3(vp.b1x, vp.b1y, nframes), dim3(256), 0, s, job,
                        stride_floats, per);
     RECON_HIP_TRY(hipGetLastError(), "depth_bounds_kernel launch");
