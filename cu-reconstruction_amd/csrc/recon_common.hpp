@@ -154,7 +154,6 @@ struct recon_tsdf {
     float* W = nullptr;
     int device = 0;
     int num_cus = 256;
-    int integrate_blocks_per_cu = 4;
     unsigned long long* update_counter = nullptr;
     // per-frame depth bounds, grown on demand
     float* bounds = nullptr;

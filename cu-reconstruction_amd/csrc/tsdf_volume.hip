@@ -201,15 +201,34 @@ struct ListArgs {
     unsigned seg_len;
 };
 
+// Resident workgroups per CU of one instantiation of the list kernel: its
+// grid is exactly the resident slots, because every wave takes the same
+// share of the list and a workgroup beyond them would start only when
+// another has finished its share.
+template <int MAXF, bool VEC, bool COUNT, bool COLOR, bool VARW>
+static int list_occupancy() {
+    static const int blocks = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &b, integrate_list_kernel<MAXF, VEC, COUNT, COLOR, VARW>, kThreads, 0) != hipSuccess ||
+            b < 1)
+            b = 1;
+        return b;
+    }();
+    return blocks;
+}
+
 template <int MAXF, bool VEC, bool COUNT, bool COLOR>
-static void launch_list(bool varw, dim3 grid, hipStream_t s, const VolumeParams& vp, int nframes,
+static void launch_list(bool varw, int num_cus, hipStream_t s, const VolumeParams& vp, int nframes,
                         const FrameBlock<MAXF>& fb, const ListArgs& la, unsigned long long* ctr) {
     if (varw)
-        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, true>), grid,
+        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, true>),
+                           dim3(num_cus * list_occupancy<MAXF, VEC, COUNT, COLOR, true>()),
                            dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
                            la.seg_len, ctr);
     else
-        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, false>), grid,
+        hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, false>),
+                           dim3(num_cus * list_occupancy<MAXF, VEC, COUNT, COLOR, false>()),
                            dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
                            la.seg_len, ctr);
 }
@@ -255,26 +274,25 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     const bool vec = (vp.gx % 4) == 0;
     const bool varw = h->info.weight_type == RECON_WEIGHT_BY_VARIANCE;
     unsigned long long* ctr = h->update_counter;
-    const dim3 grid(h->num_cus * h->integrate_blocks_per_cu);
     bool color = false;
     if constexpr (MAXF == 1) color = fb.f[0].rgb && vp.C;  // colour integrate (no counting)
     if (color) {
         if constexpr (MAXF == 1) {
             if (vec)
-                launch_list<1, true, false, true>(varw, grid, s, vp, nframes, fb, la, ctr);
+                launch_list<1, true, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
             else
-                launch_list<1, false, false, true>(varw, grid, s, vp, nframes, fb, la, ctr);
+                launch_list<1, false, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
         }
     } else if (ctr) {
         if (vec)
-            launch_list<MAXF, true, true, false>(varw, grid, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, true, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
         else
-            launch_list<MAXF, false, true, false>(varw, grid, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, false, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
     } else {
         if (vec)
-            launch_list<MAXF, true, false, false>(varw, grid, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, true, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
         else
-            launch_list<MAXF, false, false, false>(varw, grid, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, false, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
     }
     RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
     return RECON_OK;
@@ -291,22 +309,6 @@ static VolumeParams volume_params(const recon_tsdf* h, int cols, int rows) {
     vp.b1x = (cols + kBound1 - 1) / kBound1;
     vp.b1y = (rows + kBound1 - 1) / kBound1;
     return vp;
-}
-
-// Resident workgroups per CU of the integrate kernel (all instantiations
-// share the register budget of the fused one, the largest).
-static int integrate_occupancy() {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, integrate_list_kernel<kMaxFusedFrames, true, false>, kThreads, 0) != hipSuccess ||
-        blocks < 1)
-        blocks = 4;
-    int b1 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, integrate_list_kernel<1, true, false>,
-                                                     kThreads, 0) == hipSuccess &&
-        b1 >= 1)
-        blocks = std::min(blocks, b1);
-    return blocks;
 }
 
 }  // namespace recon
@@ -329,7 +331,6 @@ extern "C" int recon_tsdf_create(const recon_tsdf_config* cfg, recon_tsdf** out)
     recon_tsdf* h = new recon_tsdf();
     h->device = device;
     h->num_cus = num_cus;
-    h->integrate_blocks_per_cu = integrate_occupancy();
     h->cfg = *cfg;
     h->info = info;
     VolumeParams& vp = h->vp;
