@@ -13,7 +13,9 @@ covers one integrate call (its three launches: depth_bounds_kernel,
 cull_kernel, integrate_list_kernel): algorithmic bytes per call
 (16 B per updated voxel + 4 B per depth pixel, the update count measured
 exactly by the kernel's counting mode outside the timed region) divided by
-the call's average duration from HIP events on the launch stream; traffic
+the call's average duration from HIP events on the launch stream (one event
+pair around each run of consecutive calls between extractions, in the timed
+steps; a record per call would add its own ~10 us per call); traffic
 is the HBM bytes per call from the committed rocprofv3 summary
 (profiles/integrate_plain_large_512.md).
 
@@ -242,16 +244,21 @@ def bench_differentiable(steps: int) -> dict:
     loss, _, ids, lam = forward()
     gp, gd = vol.diff_adjoint(depth, rt, ids, lam, stream=stream)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    # one event pair around a loop of forward passes, one around a loop of
+    # forward + backward passes: the backward time is their difference
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
     for s in range(steps):
-        ev[s][0].record(stream)
         loss, _, ids, lam = forward()
-        ev[s][1].record(stream)
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for s in range(steps):
+        loss, _, ids, lam = forward()
         gp, gd = vol.diff_adjoint(depth, rt, ids, lam, stream=stream)
-        ev[s][2].record(stream)
+    ev[3].record(stream)
     torch.cuda.synchronize()
-    fwd = np.array([e[0].elapsed_time(e[1]) for e in ev])
-    bwd = np.array([e[1].elapsed_time(e[2]) for e in ev])
+    fwd = np.array([ev[0].elapsed_time(ev[1]) / steps])
+    bwd = np.array([ev[2].elapsed_time(ev[3]) / steps]) - fwd
     out = {"workload": w.name, "grid": list(w.grid), "frames": w.num_frames,
            "points": int(pts.shape[0]), "fwd_ms": round(float(fwd.mean()), 4),
            "bwd_ms": round(float(bwd.mean()), 4),
@@ -307,13 +314,13 @@ def bench_render(w, depth, steps: int, frames: int = 30, cpu: bool = True) -> di
     call()
     torch.cuda.synchronize()
     reps = max(steps, 1) * 10
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
-    ev[0].record(stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
     for r in range(reps):
         call()
-        ev[r + 1].record(stream)
+    e1.record(stream)
     torch.cuda.synchronize()
-    ms = np.array([ev[r].elapsed_time(ev[r + 1]) for r in range(reps)])
+    ms = np.array([e0.elapsed_time(e1) / reps])
     n = xyz.shape[0]
     hits = int((dout > 0).sum().item())
     # algorithmic bytes per call: 12 B xyz + 1 B valid per point; per pixel
@@ -537,16 +544,27 @@ def main() -> int:
     tris = torch.empty((cap, 3), dtype=torch.int32, device="cuda")
     log(f"mesh capacity {cap} triangles")
 
+    # runs of consecutive integrate calls between extractions (an event
+    # record costs ~10 us of GPU time, so the timed steps bracket runs, not
+    # single calls)
+    runs = []
+    a = 0
+    for f in range(F):
+        if f in mesh_at or f == F - 1:
+            runs.append((a, f + 1))
+            a = f + 1
+
     def step(events=None):
         vol.reset(stream)
         ntri = 0
-        for f in range(F):
+        for k, (a, b) in enumerate(runs):
             if events is not None:
-                events[f][0].record(stream)
-            vol.integrate(dev[f], w.poses[f], stream)
+                events[k][0].record(stream)
+            for f in range(a, b):
+                vol.integrate(dev[f], w.poses[f], stream)
             if events is not None:
-                events[f][1].record(stream)
-            if f in mesh_at:
+                events[k][1].record(stream)
+            if b - 1 in mesh_at:
                 v, _ = vol.extract_mesh(verts, tris, stream)
                 ntri = v.shape[0]
         return ntri
@@ -557,7 +575,7 @@ def main() -> int:
     torch.cuda.synchronize()
 
     ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(F)] for _ in range(args.steps)]
+           for _ in runs] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     ntri = 0
@@ -567,9 +585,8 @@ def main() -> int:
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     ms_per_step = 1000.0 * t_total / args.steps
-    integ_ms = [[a.elapsed_time(b) for (a, b) in ev[s]] for s in range(args.steps)]
-    integ = np.array(integ_ms)  # (steps, F) ms per launch
-    avg_launch_ms = float(integ.mean())
+    run_ms = np.array([[e0.elapsed_time(e1) for (e0, e1) in ev[s]] for s in range(args.steps)])
+    avg_launch_ms = float(run_ms.sum()) / (args.steps * F)  # ms per integrate call
 
     # fused-frame path: same frames, integrate_frames between extractions
     seg = [0] + [f + 1 for f in mesh_at]
@@ -686,9 +703,6 @@ def main() -> int:
     e0.record(stream)
     for f in range(F):
         sv.integrate(dev[f], w.poses[f], stream)
-        if f % 50 == 49:
-            torch.cuda.synchronize()
-            log(f"sparse: {f + 1} frames, {sv.stats(stream)[0]} bricks")
     e1.record(stream)
     torch.cuda.synchronize()
     sparse_ms = e0.elapsed_time(e1) / F
