@@ -177,7 +177,9 @@ __device__ __forceinline__ void edge_vertex(const McParams& p, int x, int y, int
 
 // Writes the triangles of cube row r starting at triangle `base` (one
 // wavefront): a wave-level prefix sum of the per-cube counts places each
-// cube's triangles in x order, slots in table order.  Triangles at or past
+// cube's triangles in x order, slots in table order, and the triangles of
+// 64 cubes are spread over the 64 lanes (not one lane per cube), so a cube
+// with 5 triangles does not hold up the wave.  Triangles at or past
 // `capacity` are not written.  Vertex indices (tris, may be null) are
 // 3t+k + tri_index_base.
 __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, long long base,
@@ -195,22 +197,37 @@ __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, 
             nt = class_ntri(c);
         }
         const int incl = wave_inclusive_scan(nt, lane);
+        const int excl = incl - nt;
         const int chunk_total = __shfl(incl, 63, 64);
-        long long t = base + (incl - nt);
-        if (nt) {
-            const uint64_t word = c_tri[c];
-            for (int s = 0; s < nt; ++s, ++t) {
-                if (t >= capacity) break;
+        // triangle k of the chunk (k = 0 .. chunk_total-1, in the soup's
+        // order) is written by lane k % 64: its cube is the lane whose
+        // triangles include k, its slot k minus that lane's first triangle
+        for (int k0 = 0; k0 < chunk_total; k0 += 64) {
+            const int k = k0 + lane;
+            int pos = 0;  // lanes with incl <= k
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int e = (int)((word >> (4 + 4 * (3 * s + k))) & 0xFull);
-                    float q[3];
-                    edge_vertex(p, x, y, z, v, e, q);
-                    float* dst = verts + 9 * t + 3 * k;
-                    dst[0] = q[0];
-                    dst[1] = q[1];
-                    dst[2] = q[2];
-                    if (tris) tris[3 * t + k] = (int32_t)(3 * (t - tri_index_base) + k);
+            for (int st = 32; st >= 1; st >>= 1)
+                if (__shfl(incl, pos + st - 1, 64) <= k) pos += st;
+            const int owner = min(pos, 63);
+            const int oc = __shfl(c, owner, 64);
+            const int slot = k - __shfl(excl, owner, 64);
+            float ov[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) ov[i] = __shfl(v[i], owner, 64);
+            const long long t = base + k;
+            if (k < chunk_total && t < capacity) {
+                const uint64_t word = c_tri[oc];
+                const int ox = x0 + owner;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
+                    float pt[3];
+                    edge_vertex(p, ox, y, z, ov, e, pt);
+                    float* dst = verts + 9 * t + 3 * q;
+                    dst[0] = pt[0];
+                    dst[1] = pt[1];
+                    dst[2] = pt[2];
+                    if (tris) tris[3 * t + q] = (int32_t)(3 * (t - tri_index_base) + q);
                 }
             }
         }
