@@ -133,19 +133,13 @@ __device__ __forceinline__ float weld_grad(const McParams& p, int x, int y, int 
     return (p.f[i] - p.f[i - st]) / p.vs[a];
 }
 
-// Vertices (and normals) of the edges lane `lane` of chunk c owns.
-__device__ __forceinline__ void weld_lane_vertices(const WeldParams& w, int c, int lane,
-                                                   const unsigned long long m[3], long long base,
-                                                   float* __restrict__ verts,
-                                                   float* __restrict__ normals) {
+// Vertex (and normal) `idx` on the edge from voxel (x, y, z) along axis a.
+__device__ __forceinline__ void weld_vertex(const WeldParams& w, int x, int y, int z, int a,
+                                            long long idx, float* __restrict__ verts,
+                                            float* __restrict__ normals) {
     const McParams& p = w.mc;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    long long idx = base + __popcll(m[0] & below) + __popcll(m[1] & below) + __popcll(m[2] & below);
-    const int row = c / w.cpr;
-    const int x = (c % w.cpr) * 64 + lane;
-    const int y = row % p.gy, z = row / p.gy;
-    for (int a = 0; a < 3; ++a) {
-        if (!((m[a] >> lane) & 1ull)) continue;
+    const int row = y + p.gy * z;
+    {
         const int q[3] = {x + (a == 0), y + (a == 1), z + (a == 2)};
         const int64_t i = (int64_t)x + (int64_t)p.gx * row;
         const int64_t j = (int64_t)q[0] + (int64_t)p.gx * (q[1] + (int64_t)p.gy * q[2]);
@@ -185,7 +179,44 @@ __device__ __forceinline__ void weld_lane_vertices(const WeldParams& w, int c, i
                 for (int k = 0; k < 3; ++k) normals[3 * idx + k] = 0.0f;
             }
         }
-        ++idx;
+    }
+}
+
+// Position of element k among per-lane counts whose inclusive wave scan is
+// `incl` (k < incl of lane 63): the lane that owns it (the number of lanes
+// with incl <= k).  Every lane must call.
+__device__ __forceinline__ int wave_owner(int incl, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1)
+        if (__shfl(incl, pos + st - 1, 64) <= k) pos += st;
+    return min(pos, 63);
+}
+
+// The vertices (and normals) of chunk c, numbered base + k in the order lane,
+// then axis: vertex k is computed by lane k % 64 (a lane owning three edges
+// does not hold up the wave).  Every lane must call.
+__device__ __forceinline__ void weld_chunk_vertices(const WeldParams& w, int c, int lane,
+                                                    const unsigned long long m[3], long long base,
+                                                    float* __restrict__ verts,
+                                                    float* __restrict__ normals) {
+    const McParams& p = w.mc;
+    const int row = c / w.cpr;
+    const int x0 = (c % w.cpr) * 64;
+    const int y = row % p.gy, z = row / p.gy;
+    const int b0 = (int)((m[0] >> lane) & 1ull), b1 = (int)((m[1] >> lane) & 1ull),
+              b2 = (int)((m[2] >> lane) & 1ull);
+    const int cnt = b0 + b1 + b2;
+    const int incl = wave_inclusive_scan(cnt, lane);
+    const int excl = incl - cnt;
+    const int total = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]);
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        const int k = k0 + lane;
+        const int owner = wave_owner(incl, k);
+        const int r = k - __shfl(excl, owner, 64);  // rank among the owner's set axes
+        const int o0 = (int)((m[0] >> owner) & 1ull), o1 = (int)((m[1] >> owner) & 1ull);
+        const int a = r == 0 ? (o0 ? 0 : (o1 ? 1 : 2)) : (r == 1 ? (o0 && o1 ? 1 : 2) : 2);
+        if (k < total) weld_vertex(w, x0 + owner, y, z, a, base + k, verts, normals);
     }
 }
 
@@ -231,7 +262,7 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
         if (verts || normals) {
             const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
                                              masks[3 * (int64_t)c + 2]};
-            if (m[0] | m[1] | m[2]) weld_lane_vertices(w, c, lane, m, vbase[c], verts, normals);
+            if (m[0] | m[1] | m[2]) weld_chunk_vertices(w, c, lane, m, vbase[c], verts, normals);
         }
         if (!tris || tcount[c] == 0) continue;
         const ChunkCorners k = chunk_corners(w, c, lane);
@@ -241,15 +272,25 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
             nt = class_ntri(cls);
         }
         const int incl = wave_inclusive_scan(nt, lane);
-        long long t = tbase[c] + (incl - nt);
-        if (nt) {
-            const uint64_t word = c_tri[cls];
-            for (int s = 0; s < nt; ++s, ++t)
+        const int excl = incl - nt;
+        const int total = __shfl(incl, 63, 64);
+        const long long t0 = tbase[c];
+        const int x0 = k.x - lane;
+        // triangle kk of the chunk is written by lane kk % 64 (see mc_emit_row)
+        for (int k0 = 0; k0 < total; k0 += 64) {
+            const int kk = k0 + lane;
+            const int owner = wave_owner(incl, kk);
+            const int ocls = __shfl(cls, owner, 64);
+            const int slot = kk - __shfl(excl, owner, 64);
+            if (kk < total) {
+                const long long t = t0 + kk;
+                const uint64_t word = c_tri[ocls];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const int e = (int)((word >> (4 + 4 * (3 * s + q))) & 0xFull);
-                    tris[3 * t + q] = weld_index(w, masks, vbase, k.x, k.y, k.z, e);
+                    const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
+                    tris[3 * t + q] = weld_index(w, masks, vbase, x0 + owner, k.y, k.z, e);
                 }
+            }
         }
     }
 }
