@@ -88,6 +88,8 @@ __global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
     // rows with a +y and a +z neighbour row stream their corners
     const bool interior = y < p.gy - 1 && z < p.gz - 1;
     CornerStream cs(p, interior ? y : 0, interior ? z : 0, lane);
+    unsigned long long km[3] = {0ull, 0ull, 0ull};
+    int kv = 0, kt = 0;
     for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
         ChunkCorners k;
         if (interior) {
@@ -111,13 +113,29 @@ __global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
             if (k.cube) nt = class_ntri(cube_class(k.v));
         }
         const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
-        nt = wave_sum(nt);
-        if (lane == 0) {
-            masks[3 * (int64_t)c + 0] = mx;
-            masks[3 * (int64_t)c + 1] = my;
-            masks[3 * (int64_t)c + 2] = mz;
-            vcount[c] = __popcll(mx) + __popcll(my) + __popcll(mz);
-            tcount[c] = nt;
+        // the chunk's triangles (nt <= 5 per cube) from ballots of nt's bits
+        const int ntot = __popcll(__ballot(nt & 1)) + 2 * __popcll(__ballot(nt & 2)) +
+                         4 * __popcll(__ballot(nt & 4));
+        // lane j of the wave keeps chunk j of each 64 chunks of the row;
+        // every 64 chunks and at the row's end the kept chunks are stored
+        // together
+        const int j = c - row * w.cpr;
+        if (lane == (j & 63)) {
+            km[0] = mx;
+            km[1] = my;
+            km[2] = mz;
+            kv = __popcll(mx) + __popcll(my) + __popcll(mz);
+            kt = ntot;
+        }
+        if ((j & 63) == 63 || j == w.cpr - 1) {
+            const int64_t cc = (int64_t)row * w.cpr + (j & ~63) + lane;
+            if (lane <= (j & 63)) {
+                masks[3 * cc + 0] = km[0];
+                masks[3 * cc + 1] = km[1];
+                masks[3 * cc + 2] = km[2];
+                vcount[cc] = kv;
+                tcount[cc] = kt;
+            }
         }
     }
 }
