@@ -86,7 +86,7 @@ def bench_small(steps: int, cpu: bool = True) -> dict:
     of 640x480) in full: the GPU per-frame integrate and the fused ten-frame
     integrate (HIP events on the launch stream, `steps` passes each) beside
     the oracle's float32 integrate of all 10 frames on one host thread and on
-    up to 16 threads."""
+    up to 16 threads (best of five passes each)."""
     import torch
 
     import recon
@@ -138,13 +138,15 @@ def bench_small(steps: int, cpu: bool = True) -> dict:
 
         ov = O.volume_from_config(w.config())
         for threads in (1, min(os.cpu_count() or 1, 16)):
-            Dc, Wc = O.init(ov)
-            t0 = time.perf_counter()
-            for f in range(F):
-                O.integrate_f32(ov, Dc, Wc, depth[f], w.poses[f], nthreads=threads)
-            dt = time.perf_counter() - t0
-            out[f"cpu_{threads}t_ms_per_pass"] = round(1000.0 * dt, 1)
-            out[f"cpu_{threads}t_gvu_s"] = round(w.num_voxels * F / dt / 1e9, 4)
+            best = float("inf")
+            for _ in range(5):  # a pass is 0.01-0.2 s: best of five against host noise
+                Dc, Wc = O.init(ov)
+                t0 = time.perf_counter()
+                for f in range(F):
+                    O.integrate_f32(ov, Dc, Wc, depth[f], w.poses[f], nthreads=threads)
+                best = min(best, time.perf_counter() - t0)
+            out[f"cpu_{threads}t_ms_per_pass"] = round(1000.0 * best, 1)
+            out[f"cpu_{threads}t_gvu_s"] = round(w.num_voxels * F / best / 1e9, 4)
     return out
 
 
