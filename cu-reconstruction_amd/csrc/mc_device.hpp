@@ -122,6 +122,17 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
+// The lane owning element k of per-lane counts whose inclusive wave scan is
+// `incl` (k < incl of lane 63): the number of lanes with incl <= k.  Every
+// lane must call.
+__device__ __forceinline__ int wave_owner(int incl, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1)
+        if (__shfl(incl, pos + st - 1, 64) <= k) pos += st;
+    return min(pos, 63);
+}
+
 // Inclusive prefix sum over the 64 lanes of a wavefront.
 __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 #pragma unroll
@@ -204,11 +215,7 @@ __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, 
         // triangles include k, its slot k minus that lane's first triangle
         for (int k0 = 0; k0 < chunk_total; k0 += 64) {
             const int k = k0 + lane;
-            int pos = 0;  // lanes with incl <= k
-#pragma unroll
-            for (int st = 32; st >= 1; st >>= 1)
-                if (__shfl(incl, pos + st - 1, 64) <= k) pos += st;
-            const int owner = min(pos, 63);
+            const int owner = wave_owner(incl, k);
             const int oc = __shfl(c, owner, 64);
             const int slot = k - __shfl(excl, owner, 64);
             float ov[8];

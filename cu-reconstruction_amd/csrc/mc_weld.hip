@@ -157,58 +157,45 @@ __device__ __forceinline__ void weld_vertex(const WeldParams& w, int x, int y, i
                                             float* __restrict__ normals) {
     const McParams& p = w.mc;
     const int row = y + p.gy * z;
-    {
-        const int q[3] = {x + (a == 0), y + (a == 1), z + (a == 2)};
-        const int64_t i = (int64_t)x + (int64_t)p.gx * row;
-        const int64_t j = (int64_t)q[0] + (int64_t)p.gx * (q[1] + (int64_t)p.gy * q[2]);
-        const bool neg_first = p.f[i] < 0.0f;
-        const int pn[3] = {neg_first ? x : q[0], neg_first ? y : q[1], neg_first ? z : q[2]};
-        const int pm[3] = {neg_first ? q[0] : x, neg_first ? q[1] : y, neg_first ? q[2] : z};
-        const float dn = neg_first ? p.f[i] : p.f[j];
-        const float dm = neg_first ? p.f[j] : p.f[i];
-        const float r = (-dn) / (dm - dn);
-        if (verts) {
+    const int q[3] = {x + (a == 0), y + (a == 1), z + (a == 2)};
+    const int64_t i = (int64_t)x + (int64_t)p.gx * row;
+    const int64_t j = (int64_t)q[0] + (int64_t)p.gx * (q[1] + (int64_t)p.gy * q[2]);
+    const bool neg_first = p.f[i] < 0.0f;
+    const int pn[3] = {neg_first ? x : q[0], neg_first ? y : q[1], neg_first ? z : q[2]};
+    const int pm[3] = {neg_first ? q[0] : x, neg_first ? q[1] : y, neg_first ? q[2] : z};
+    const float dn = neg_first ? p.f[i] : p.f[j];
+    const float dm = neg_first ? p.f[j] : p.f[i];
+    const float r = (-dn) / (dm - dn);
+    if (verts) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float sn = voxel_centre(pn[k], p.vs[k], p.off[k]);
-                const float sm = voxel_centre(pm[k], p.vs[k], p.off[k]);
-                const float e = sm - sn;
-                const float o = r * e;
-                verts[3 * idx + k] = sn + o;
-            }
-        }
-        if (normals) {
-            float g[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float gn = weld_grad(p, pn[0], pn[1], pn[2], k);
-                const float gm = weld_grad(p, pm[0], pm[1], pm[2], k);
-                const float e = gm - gn;
-                const float o = r * e;
-                g[k] = gn + o;
-            }
-            const float len2 = (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
-            if (len2 > 0.0f) {
-                const float len = sqrtf(len2);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) normals[3 * idx + k] = g[k] / len;
-            } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) normals[3 * idx + k] = 0.0f;
-            }
+        for (int k = 0; k < 3; ++k) {
+            const float sn = voxel_centre(pn[k], p.vs[k], p.off[k]);
+            const float sm = voxel_centre(pm[k], p.vs[k], p.off[k]);
+            const float e = sm - sn;
+            const float o = r * e;
+            verts[3 * idx + k] = sn + o;
         }
     }
-}
-
-// Position of element k among per-lane counts whose inclusive wave scan is
-// `incl` (k < incl of lane 63): the lane that owns it (the number of lanes
-// with incl <= k).  Every lane must call.
-__device__ __forceinline__ int wave_owner(int incl, int k) {
-    int pos = 0;
+    if (normals) {
+        float g[3];
 #pragma unroll
-    for (int st = 32; st >= 1; st >>= 1)
-        if (__shfl(incl, pos + st - 1, 64) <= k) pos += st;
-    return min(pos, 63);
+        for (int k = 0; k < 3; ++k) {
+            const float gn = weld_grad(p, pn[0], pn[1], pn[2], k);
+            const float gm = weld_grad(p, pm[0], pm[1], pm[2], k);
+            const float e = gm - gn;
+            const float o = r * e;
+            g[k] = gn + o;
+        }
+        const float len2 = (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+        if (len2 > 0.0f) {
+            const float len = sqrtf(len2);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) normals[3 * idx + k] = g[k] / len;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) normals[3 * idx + k] = 0.0f;
+        }
+    }
 }
 
 // The vertices (and normals) of chunk c, numbered base + k in the order lane,
