@@ -97,3 +97,45 @@ def test_random_configuration(cuda, seed):
     assert_bitexact(v.cpu().numpy(), wv, "welded vertices")
     assert_bitexact(n.cpu().numpy(), wn, "welded normals")
     assert np.array_equal(t.cpu().numpy(), wt)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_configuration_variance_weights(cuda, seed):
+    # WEIGHT_BY_VARIANCE on the same sweep: per-frame, fused and colour
+    # integrate bit-exact with the oracle; a random noise model, or the
+    # library's default (all zeros) on odd seeds
+    w, cfg = _case(100 + seed)
+    rng = np.random.default_rng(9000 + seed)
+    cfg.weight_type = 1  # RECON_WEIGHT_BY_VARIANCE
+    if seed % 2 == 0:
+        nm = (float(rng.uniform(0.0005, 0.003)), float(rng.uniform(0.0005, 0.004)),
+              float(rng.uniform(0.2, 0.6)))
+        for i in range(3):
+            cfg.noise_model[i] = nm[i]
+    depth = w.render()
+    print(f"variance weights {seed}: grid {w.grid} image {w.image} frames {w.num_frames}", flush=True)
+    o = oracle_run(cfg, depth, w.poses)
+    for fused in (False, True):
+        g = gpu_run(cfg, depth, w.poses, fused=fused)
+        assert_bitexact(g["W"], o["W"], f"W fused={fused}")
+        assert_bitexact(g["D"], o["D"], f"D fused={fused}")
+        g["vol"].close()
+    cols, rows = w.image
+    rgbs = rng.integers(0, 256, (w.num_frames, rows, cols, 3), dtype=np.uint8)
+    ov = O.volume_from_config(cfg)
+    Dc, Wc = O.init(ov)
+    Cc = np.zeros(Dc.shape + (4,), np.float32)
+    for f in range(w.num_frames):
+        O.integrate_color_f32(ov, Dc, Wc, Cc, depth[f], rgbs[f], w.poses[f])
+    vol = recon.TSDFVolume(cfg)
+    vol.enable_color(True)
+    dev = cuda.from_numpy(np.ascontiguousarray(depth)).cuda()
+    drgb = cuda.from_numpy(rgbs).cuda()
+    for f in range(w.num_frames):
+        vol.integrate(dev[f], w.poses[f], rgb=drgb[f])
+    cuda.cuda.synchronize()
+    D, W = vol.download()
+    assert_bitexact(D, Dc, "colour D")
+    assert_bitexact(W, Wc, "colour W")
+    assert_bitexact(vol.download_colors(), Cc, "colour")
+    vol.close()
