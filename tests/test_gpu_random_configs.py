@@ -4,7 +4,9 @@ each checked against the oracle: per-frame, fused and colour integrate
 oracle), the triangle soup and the welded mesh (bit-exact).  Grids of 1-96
 voxels per axis (not multiples of the tile or chunk sizes), anisotropic
 voxels, offsets off the origin, images of 16-200 x 12-160 pixels with
-off-centre principal points, 1-5 frames, weight caps from 1.5 to 128."""
+off-centre principal points, 1-5 frames, weight caps from 1.5 to 128.
+The same sweep runs WEIGHT_BY_VARIANCE configurations and unbounded sparse
+brick volumes."""
 import numpy as np
 import pytest
 
@@ -139,3 +141,28 @@ def test_random_configuration_variance_weights(cuda, seed):
     assert_bitexact(W, Wc, "colour W")
     assert_bitexact(vol.download_colors(), Cc, "colour")
     vol.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_configuration_sparse(cuda, seed):
+    # sparse brick volume on the same sweep (unbounded, so bricks reach
+    # negative coordinates and far outside the dense grid): the same brick set
+    # and bit-identical brick contents as the oracle after every frame
+    w, cfg = _case(200 + seed)
+    depth = w.render()
+    print(f"sparse {seed}: grid {w.grid} image {w.image} frames {w.num_frames}", flush=True)
+    sv = recon.SparseVolume(cfg, 1 << 16)
+    so = O.Sparse(O.volume_from_config(cfg))
+    dev = cuda.from_numpy(np.ascontiguousarray(depth)).cuda()
+    for f in range(w.num_frames):
+        sv.integrate(dev[f], w.poses[f])
+        so.integrate(depth[f], w.poses[f])
+        cuda.cuda.synchronize()
+        g, o = sv.bricks(), so.bricks()
+        assert set(g) == set(o), f"frame {f}: {len(g)} vs {len(o)} bricks"
+        for k, (od, ow) in o.items():
+            gd, gw = g[k]
+            assert np.array_equal(gd.view(np.uint32), od.view(np.uint32)), f"D of brick {k}"
+            assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32)), f"W of brick {k}"
+    assert sv.stats()[1] == 0
+    sv.close()
