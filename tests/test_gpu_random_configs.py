@@ -5,8 +5,8 @@ oracle), the triangle soup and the welded mesh (bit-exact).  Grids of 1-96
 voxels per axis (not multiples of the tile or chunk sizes), anisotropic
 voxels, offsets off the origin, images of 16-200 x 12-160 pixels with
 off-centre principal points, 1-5 frames, weight caps from 1.5 to 128.
-The same sweep runs WEIGHT_BY_VARIANCE configurations and unbounded sparse
-brick volumes."""
+The same sweep runs WEIGHT_BY_VARIANCE configurations, unbounded sparse
+brick volumes and the differentiable forward."""
 import numpy as np
 import pytest
 
@@ -166,3 +166,21 @@ def test_random_configuration_sparse(cuda, seed):
             assert np.array_equal(gw.view(np.uint32), ow.view(np.uint32)), f"W of brick {k}"
     assert sv.stats()[1] == 0
     sv.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_configuration_differentiable_forward(cuda, seed):
+    # the differentiable (bilinear) integrate on the same sweep: D and W
+    # bit-exact with the float32 oracle of the forward
+    w, cfg = _case(300 + seed)
+    depth = w.render()
+    print(f"differentiable {seed}: grid {w.grid} image {w.image} frames {w.num_frames}", flush=True)
+    rt32 = recon.pose_to_matrix(w.poses)
+    vol = recon.TSDFVolume(cfg)
+    vol.diff_integrate(cuda.from_numpy(np.ascontiguousarray(depth)).cuda(), cuda.from_numpy(rt32).cuda())
+    cuda.cuda.synchronize()
+    D, W = vol.download()
+    D32, W32 = O.diff_forward_f32(O.volume_from_config(cfg), depth, rt32)
+    assert_bitexact(W, W32, "W")
+    assert_bitexact(D, D32, "D")
+    vol.close()
