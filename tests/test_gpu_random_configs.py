@@ -6,7 +6,8 @@ voxels per axis (not multiples of the tile or chunk sizes), anisotropic
 voxels, offsets off the origin, images of 16-200 x 12-160 pixels with
 off-centre principal points, 1-5 frames, weight caps from 1.5 to 128.
 The same sweep runs WEIGHT_BY_VARIANCE configurations, unbounded sparse
-brick volumes and the differentiable forward."""
+brick volumes, the differentiable forward, and depth -> points with the
+z-buffer render of the resulting cloud."""
 import numpy as np
 import pytest
 
@@ -184,3 +185,41 @@ def test_random_configuration_differentiable_forward(cuda, seed):
     assert_bitexact(W, W32, "W")
     assert_bitexact(D, D32, "D")
     vol.close()
+
+
+def _camera_to_world(pose):
+    """The inverse (camera->world) of a world->camera pose (qw, qx, qy, qz, t)."""
+    R, t = scenes.pose_matrix(pose)
+    return np.r_[pose[0], -np.asarray(pose[1:4]), -(R.T @ t)]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_configuration_points_and_render(cuda, seed):
+    # depth -> points of every frame (with colours) and the z-buffer render of
+    # the whole cloud at another pose of the arc: bit-exact with the oracle
+    w, _ = _case(400 + seed)
+    depth = w.render()
+    cols, rows = w.image
+    rng = np.random.default_rng(seed)
+    rgb = rng.integers(0, 256, (rows, cols, 3), dtype=np.uint8)
+    print(f"points and render {seed}: image {w.image} frames {w.num_frames}", flush=True)
+    xyz, valid, colors = [], [], []
+    for f in range(w.num_frames):
+        T = _camera_to_world(w.poses[f])
+        p, v, c = recon.depth_to_points(cuda.from_numpy(depth[f]).cuda(), w.intrinsics, T,
+                                        rgb=cuda.from_numpy(rgb).cuda(), with_colors=True)
+        op, ovl, oc = O.depth_to_points(depth[f], w.intrinsics, T, rgb=rgb)
+        assert np.array_equal(v.cpu().numpy(), ovl)
+        assert_bitexact(p.cpu().numpy(), op, "points")
+        assert np.array_equal(c.cpu().numpy(), oc)
+        xyz.append(op)
+        valid.append(ovl)
+        colors.append(oc)
+    xyz, valid, colors = np.concatenate(xyz), np.concatenate(valid), np.concatenate(colors)
+    view = scenes.arc_poses(1, radius=1.1, height=0.2, start_deg=float(rng.uniform(0, 360)))[0]
+    d, c = recon.render_points(cuda.from_numpy(xyz).cuda(), view, w.intrinsics, w.image,
+                               valid=cuda.from_numpy(valid).cuda(), colors=cuda.from_numpy(colors).cuda(),
+                               with_colors=True)
+    od, oc = O.render_points(xyz, view, w.intrinsics, w.image, valid=valid, colors=colors)
+    assert_bitexact(d.cpu().numpy(), od, "rendered depth")
+    assert np.array_equal(c.cpu().numpy(), oc)
