@@ -5,8 +5,9 @@
 // kernel per z layer, synchronises, and compacts each layer on the host from
 // managed memory.  Here the whole volume is done on the device in four
 // launches with a fixed, deterministic output order:
-//   1. row_count  - one wavefront per cube row (y,z): classify every cube
-//                   (marching_cubes.cu:76-84,118-130) and count its triangles;
+//   1. row_count  - one wavefront per 4 cube rows (y..y+3, z): classify every
+//                   cube (marching_cubes.cu:76-84,118-130) and count each
+//                   row's triangles;
 //   2-4. scan     - exclusive scan of the row counts (block sums, top-level
 //                   scan, apply) giving each row's first triangle;
 //   5. emit       - rows with triangles only: a wave-level prefix sum of the
@@ -25,13 +26,42 @@ namespace recon {
 
 constexpr int kRowsPerBlock = 4;  // one wavefront per row, 256 threads
 
+// Cube rows per wavefront in the count pass: 4 rows of one layer read 10
+// voxel rows instead of 16 (count 207 -> 152 us at 512^3; 2 rows: 165 us).
+constexpr int kCountRows = 4;
+
 __global__ __launch_bounds__(256) void mc_row_count_kernel(const McParams p,
                                                            int32_t* __restrict__ row_count) {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
-    if (r >= p.rows) return;
-    const int n = mc_row_triangles(p, r, lane);
-    if (lane == 0) row_count[r] = n;
+    const int groups = (p.cyr + kCountRows - 1) / kCountRows;
+    const int q = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
+    const int z = q / groups, y = kCountRows * (q % groups);
+    if (z >= p.rows / p.cyr) return;
+    const int r = y + p.cyr * z;
+    if (y + kCountRows > p.cyr) {  // the layer's last, partial group: one row at a time
+        for (int i = 0; y + i < p.cyr; ++i) {
+            const int n = mc_row_triangles(p, r + i, lane);
+            if (lane == 0) row_count[r + i] = n;
+        }
+        return;
+    }
+    CornerStreamK<kCountRows> cs(p, y, z, lane);
+    int n[kCountRows];
+#pragma unroll
+    for (int i = 0; i < kCountRows; ++i) n[i] = 0;
+    for (int x0 = 0; x0 < p.cx; x0 += 64) {
+        float v[kCountRows][8];
+        cs.step(x0, lane, v);
+        if (x0 + lane < p.cx) {
+#pragma unroll
+            for (int i = 0; i < kCountRows; ++i) n[i] += class_ntri(cube_class(v[i]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kCountRows; ++i) {
+        const int t = wave_sum(n[i]);
+        if (lane == 0) row_count[r + i] = t;
+    }
 }
 
 __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
@@ -183,7 +213,9 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     long long* row_offset = reinterpret_cast<long long*>(sb + off_offset);
     long long* sums = reinterpret_cast<long long*>(sb + off_sums);
     const int blocks = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
-    hipLaunchKernelGGL(mc_row_count_kernel, dim3(blocks), dim3(256), 0, s, p, row_count);
+    const int count_blocks =
+        (((p.cyr + kCountRows - 1) / kCountRows) * (grid[2] - 1) + kRowsPerBlock - 1) / kRowsPerBlock;
+    hipLaunchKernelGGL(mc_row_count_kernel, dim3(count_blocks), dim3(256), 0, s, p, row_count);
     RECON_HIP_TRY(hipGetLastError(), "mc_row_count_kernel launch");
     int rc = launch_exclusive_scan(row_count, rows, count_only ? nullptr : row_offset, sums, s);
     if (rc) return rc;

@@ -103,6 +103,61 @@ struct CornerStream {
     }
 };
 
+// K consecutive cube rows (y .. y+K-1, z) of one wavefront: their 2(K+1)
+// voxel rows, each read once, streamed like CornerStream (instead of 4 per
+// cube row).
+template <int K>
+struct CornerStreamK {
+    static constexpr int R = 2 * (K + 1);  // voxel rows: (y+i, z) and (y+i, z+1)
+    const float* f;
+    int gx;
+    int rows[R];
+    float cur[R];
+    float nx[R];
+
+    __device__ __forceinline__ CornerStreamK(const McParams& p, int y, int z, int lane) {
+        f = p.f;
+        gx = p.gx;
+        const int sz = p.gx * p.gy;
+        const int r0 = p.gx * y + sz * z;
+#pragma unroll
+        for (int i = 0; i <= K; ++i) {
+            rows[2 * i] = r0 + i * p.gx;
+            rows[2 * i + 1] = r0 + i * p.gx + sz;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) cur[k] = lane < gx ? f[rows[k] + lane] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < R; ++k) nx[k] = lane + 64 < gx ? f[rows[k] + lane + 64] : 0.0f;
+    }
+
+    // v[i] = corners of cube (x0 + lane, y + i, z), i < K
+    __device__ __forceinline__ void step(int x0, int lane, float v[K][8]) {
+        const int xn = x0 + 128 + lane;
+        float nn[R], hi[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) nn[k] = xn < gx ? f[rows[k] + xn] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const float sh = __int_as_float(
+                __builtin_amdgcn_update_dpp(0, __float_as_int(cur[k]), 0x130, 0xF, 0xF, false));
+            const float first = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nx[k]), 0));
+            hi[k] = lane == 63 ? first : sh;
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int a = 2 * i, b = 2 * i + 2;  // rows y+i and y+i+1 at z (z+1 at +1)
+            v[i][0] = cur[a]; v[i][1] = hi[a]; v[i][2] = hi[a + 1]; v[i][3] = cur[a + 1];
+            v[i][4] = cur[b]; v[i][5] = hi[b]; v[i][6] = hi[b + 1]; v[i][7] = cur[b + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            cur[k] = nx[k];
+            nx[k] = nn[k];
+        }
+    }
+};
+
 // define_voxel_type (marching_cubes.cu:76-84)
 __device__ __forceinline__ int cube_class(const float v[8]) {
     int c = 0;
