@@ -133,15 +133,42 @@ __global__ __launch_bounds__(kThreads) void batch_integrate_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void batch_mc_count_kernel(const McParams* __restrict__ mcs,
-                                                             const int2* __restrict__ wg_rows,
-                                                             int32_t* __restrict__ row_count) {
+// One wavefront (64-thread workgroup) per work-table entry of kMcRowsPerBlock
+// rows: when the entry's rows lie in one layer they are streamed together
+// (their 2 * (kMcRowsPerBlock + 1) voxel rows read once), else row by row.
+__global__ __launch_bounds__(64) void batch_mc_count_kernel(const McParams* __restrict__ mcs,
+                                                            const int2* __restrict__ wg_rows,
+                                                            int32_t* __restrict__ row_count) {
     const int2 e = wg_rows[blockIdx.x];  // (volume, first local row)
     const McParams& p = mcs[e.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r = e.y + w;
-    const int n = r < p.rows ? mc_row_triangles(p, r, lane) : 0;
-    if (lane == 0) row_count[blockIdx.x * kMcRowsPerBlock + w] = n;
+    const int lane = threadIdx.x;
+    const int y = e.y % p.cyr, z = e.y / p.cyr;
+    int32_t* out = row_count + blockIdx.x * kMcRowsPerBlock;
+    if (y + kMcRowsPerBlock <= p.cyr) {
+        CornerStreamK<kMcRowsPerBlock> cs(p, y, z, lane);
+        int n[kMcRowsPerBlock];
+#pragma unroll
+        for (int i = 0; i < kMcRowsPerBlock; ++i) n[i] = 0;
+        for (int x0 = 0; x0 < p.cx; x0 += 64) {
+            float v[kMcRowsPerBlock][8];
+            cs.step(x0, lane, v);
+            if (x0 + lane < p.cx) {
+#pragma unroll
+                for (int i = 0; i < kMcRowsPerBlock; ++i) n[i] += class_ntri(cube_class(v[i]));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kMcRowsPerBlock; ++i) {
+            const int t = wave_sum(n[i]);
+            if (lane == 0) out[i] = t;
+        }
+    } else {
+        for (int i = 0; i < kMcRowsPerBlock; ++i) {
+            const int r = e.y + i;
+            const int t = r < p.rows ? mc_row_triangles(p, r, lane) : 0;
+            if (lane == 0) out[i] = t;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McParams* __restrict__ mcs,
@@ -342,7 +369,7 @@ int recon_batch::record(hipStream_t s) {
         }
     }
     if (with_mesh && mc_blocks > 0) {
-        hipLaunchKernelGGL(batch_mc_count_kernel, dim3(mc_blocks), dim3(256), 0, s, d_mc, d_mc_wg,
+        hipLaunchKernelGGL(batch_mc_count_kernel, dim3(mc_blocks), dim3(64), 0, s, d_mc, d_mc_wg,
                            row_count);
         RECON_HIP_TRY(hipGetLastError(), "batch_mc_count_kernel launch");
         int rc = launch_exclusive_scan(row_count, total_rows, row_offset, sums, s);
