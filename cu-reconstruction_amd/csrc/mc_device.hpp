@@ -188,14 +188,20 @@ __device__ __forceinline__ int wave_owner(int incl, int k) {
     return min(pos, 63);
 }
 
-// Inclusive prefix sum over the 64 lanes of a wavefront.
+// Inclusive prefix sum over the 64 lanes of a wavefront, by DPP moves
+// (no LDS round trips): row shifts build the prefix within each row of 16
+// lanes, row broadcasts of lanes 15 and 31 carry it across rows.
 __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
+    (void)lane;
+    int t = v;
+    t += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    t += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    t += __builtin_amdgcn_update_dpp(0, v, 0x113, 0xF, 0xF, true);   // row_shr:3
+    t += __builtin_amdgcn_update_dpp(0, t, 0x114, 0xF, 0xE, true);   // row_shr:4, banks 1-3
+    t += __builtin_amdgcn_update_dpp(0, t, 0x118, 0xF, 0xC, true);   // row_shr:8, banks 2-3
+    t += __builtin_amdgcn_update_dpp(0, t, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1, 3
+    t += __builtin_amdgcn_update_dpp(0, t, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2, 3
+    return t;
 }
 
 // Triangles of cube row r = y + (gy-1) * z (one wavefront; every lane returns
