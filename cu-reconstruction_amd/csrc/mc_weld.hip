@@ -76,66 +76,123 @@ __device__ __forceinline__ ChunkCorners chunk_corners(const WeldParams& w, int c
     return k;
 }
 
+// Edge masks and triangle count of chunk j of a voxel row (cube corners k of
+// lane `lane`).  Lane j % 64 keeps the chunk's results in km / kv / kt; every
+// 64 chunks and at the row's end the kept chunks are stored together.
+// Every lane must call.
+__device__ __forceinline__ void weld_chunk_out(const WeldParams& w, const ChunkCorners& k, int lane,
+                                               int j, int64_t row_chunk0,
+                                               unsigned long long km[3], int& kv, int& kt,
+                                               unsigned long long* __restrict__ masks,
+                                               int32_t* __restrict__ vcount,
+                                               int32_t* __restrict__ tcount) {
+    const McParams& p = w.mc;
+    bool bx = false, by = false, bz = false;
+    int nt = 0;
+    if (k.in) {
+        // corners 1, 4, 3 are the +x, +y, +z neighbours (marching_cubes.cu:118-127)
+        const bool n0 = k.v[0] < 0.0f;
+        bx = k.x + 1 < p.gx && (k.v[1] < 0.0f) != n0;
+        by = k.y + 1 < p.gy && (k.v[4] < 0.0f) != n0;
+        bz = k.z + 1 < p.gz && (k.v[3] < 0.0f) != n0;
+        if (k.cube) nt = class_ntri(cube_class(k.v));
+    }
+    const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
+    // the chunk's triangles (nt <= 5 per cube) from ballots of nt's bits
+    const int ntot = __popcll(__ballot(nt & 1)) + 2 * __popcll(__ballot(nt & 2)) +
+                     4 * __popcll(__ballot(nt & 4));
+    if (lane == (j & 63)) {
+        km[0] = mx;
+        km[1] = my;
+        km[2] = mz;
+        kv = __popcll(mx) + __popcll(my) + __popcll(mz);
+        kt = ntot;
+    }
+    if ((j & 63) == 63 || j == w.cpr - 1) {
+        const int64_t cc = row_chunk0 + (j & ~63) + lane;
+        if (lane <= (j & 63)) {
+            masks[3 * cc + 0] = km[0];
+            masks[3 * cc + 1] = km[1];
+            masks[3 * cc + 2] = km[2];
+            vcount[cc] = kv;
+            tcount[cc] = kt;
+        }
+    }
+}
+
+// One voxel row (y, z): rows with a +y and a +z neighbour row stream their
+// corners, the grid's last y and z rows read them per chunk.
+__device__ __forceinline__ void weld_count_row(const WeldParams& w, int y, int z, int lane,
+                                               unsigned long long* __restrict__ masks,
+                                               int32_t* __restrict__ vcount,
+                                               int32_t* __restrict__ tcount) {
+    const McParams& p = w.mc;
+    const int row = y + p.gy * z;
+    const bool interior = y < p.gy - 1 && z < p.gz - 1;
+    CornerStream cs(p, interior ? y : 0, interior ? z : 0, lane);
+    unsigned long long km[3] = {0ull, 0ull, 0ull};
+    int kv = 0, kt = 0;
+    for (int j = 0; j < w.cpr; ++j) {
+        const int c = row * w.cpr + j;
+        ChunkCorners k;
+        if (interior) {
+            k.x = j * 64 + lane;
+            k.y = y;
+            k.z = z;
+            k.in = k.x < p.gx;
+            k.cube = k.x < p.gx - 1;
+            cs.step(j * 64, lane, k.v);
+        } else {
+            k = chunk_corners(w, c, lane);
+        }
+        weld_chunk_out(w, k, lane, j, (int64_t)row * w.cpr, km, kv, kt, masks, vcount, tcount);
+    }
+}
+
+// Voxel rows per wavefront in the count pass: 2 interior rows of a layer
+// stream their 6 shared voxel rows once (8 if each read its own 4): 427 ->
+// 352 us at 512^3.  4 rows held 130+ VGPRs (3 waves/SIMD) and took 496 us.
+constexpr int kWeldCountRows = 2;
+
 __global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
                                                          unsigned long long* __restrict__ masks,
                                                          int32_t* __restrict__ vcount,
                                                          int32_t* __restrict__ tcount) {
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    if (row >= w.mc.gy * w.mc.gz) return;
     const McParams& p = w.mc;
-    const int y = row % p.gy, z = row / p.gy;
-    // rows with a +y and a +z neighbour row stream their corners
-    const bool interior = y < p.gy - 1 && z < p.gz - 1;
-    CornerStream cs(p, interior ? y : 0, interior ? z : 0, lane);
-    unsigned long long km[3] = {0ull, 0ull, 0ull};
-    int kv = 0, kt = 0;
-    for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
-        ChunkCorners k;
-        if (interior) {
-            k.x = (c - row * w.cpr) * 64 + lane;
-            k.y = y;
+    const int groups = (p.gy + kWeldCountRows - 1) / kWeldCountRows;
+    const int q = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
+    const int z = q / groups, y = kWeldCountRows * (q % groups);
+    if (z >= p.gz) return;
+    if (!(y + kWeldCountRows <= p.gy - 1 && z < p.gz - 1)) {
+        // a group touching the grid's last y or z row: one row at a time
+        for (int i = 0; i < kWeldCountRows && y + i < p.gy; ++i)
+            weld_count_row(w, y + i, z, lane, masks, vcount, tcount);
+        return;
+    }
+    CornerStreamK<kWeldCountRows> cs(p, y, z, lane);
+    unsigned long long km[kWeldCountRows][3];
+    int kv[kWeldCountRows], kt[kWeldCountRows];
+#pragma unroll
+    for (int i = 0; i < kWeldCountRows; ++i) {
+        km[i][0] = km[i][1] = km[i][2] = 0ull;
+        kv[i] = kt[i] = 0;
+    }
+    for (int j = 0; j < w.cpr; ++j) {
+        float v[kWeldCountRows][8];
+        cs.step(j * 64, lane, v);
+#pragma unroll
+        for (int i = 0; i < kWeldCountRows; ++i) {
+            ChunkCorners k;
+            k.x = j * 64 + lane;
+            k.y = y + i;
             k.z = z;
             k.in = k.x < p.gx;
             k.cube = k.x < p.gx - 1;
-            cs.step(k.x - lane, lane, k.v);
-        } else {
-            k = chunk_corners(w, c, lane);
-        }
-        bool bx = false, by = false, bz = false;
-        int nt = 0;
-        if (k.in) {
-            // corners 1, 4, 3 are the +x, +y, +z neighbours (marching_cubes.cu:118-127)
-            const bool n0 = k.v[0] < 0.0f;
-            bx = k.x + 1 < p.gx && (k.v[1] < 0.0f) != n0;
-            by = k.y + 1 < p.gy && (k.v[4] < 0.0f) != n0;
-            bz = k.z + 1 < p.gz && (k.v[3] < 0.0f) != n0;
-            if (k.cube) nt = class_ntri(cube_class(k.v));
-        }
-        const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
-        // the chunk's triangles (nt <= 5 per cube) from ballots of nt's bits
-        const int ntot = __popcll(__ballot(nt & 1)) + 2 * __popcll(__ballot(nt & 2)) +
-                         4 * __popcll(__ballot(nt & 4));
-        // lane j of the wave keeps chunk j of each 64 chunks of the row;
-        // every 64 chunks and at the row's end the kept chunks are stored
-        // together
-        const int j = c - row * w.cpr;
-        if (lane == (j & 63)) {
-            km[0] = mx;
-            km[1] = my;
-            km[2] = mz;
-            kv = __popcll(mx) + __popcll(my) + __popcll(mz);
-            kt = ntot;
-        }
-        if ((j & 63) == 63 || j == w.cpr - 1) {
-            const int64_t cc = (int64_t)row * w.cpr + (j & ~63) + lane;
-            if (lane <= (j & 63)) {
-                masks[3 * cc + 0] = km[0];
-                masks[3 * cc + 1] = km[1];
-                masks[3 * cc + 2] = km[2];
-                vcount[cc] = kv;
-                tcount[cc] = kt;
-            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m) k.v[m] = v[i][m];
+            weld_chunk_out(w, k, lane, j, (int64_t)(y + i + p.gy * z) * w.cpr, km[i], kv[i], kt[i],
+                           masks, vcount, tcount);
         }
     }
 }
@@ -350,7 +407,10 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     auto* vsums = reinterpret_cast<long long*>(sb + o_vsums);
     auto* tsums = reinterpret_cast<long long*>(sb + o_tsums);
     const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
-    hipLaunchKernelGGL(weld_count_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vcount, tcount);
+    const int count_blocks =
+        (((grid[1] + kWeldCountRows - 1) / kWeldCountRows) * grid[2] + kWeldWaves - 1) / kWeldWaves;
+    hipLaunchKernelGGL(weld_count_kernel, dim3(count_blocks), dim3(256), 0, s, w, masks, vcount,
+                       tcount);
     RECON_HIP_TRY(hipGetLastError(), "weld_count_kernel launch");
     int rc = launch_exclusive_scan(vcount, w.nchunks, vbase, vsums, s);
     if (rc) return rc;
