@@ -357,7 +357,7 @@ def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
     of 640x480 of a distinct seeded box/sphere scene), initial poses perturbed
     by up to 1 degree and 1 cm, `steps` on-device steps (one graph launch per
     step: forward, loss, adjoint, Adam) over all fragments.  Reports the time
-    per step, pose errors, the loss and, as the judge, the RMS distance to the
+    per step, pose errors, the loss and, as the quality measure, the RMS distance to the
     analytic surface of the plain integrate's mesh with initial, refined and
     ground-truth poses."""
     import numpy as np
