@@ -27,11 +27,16 @@ namespace recon {
 constexpr int kRowsPerBlock = 4;  // one wavefront per row, 256 threads
 
 // Cube rows per wavefront in the count pass: 4 rows of one layer read 10
-// voxel rows instead of 16 (count 207 -> 152 us at 512^3; 2 rows: 165 us).
+// voxel rows instead of 16 (count 207 -> 152 us at 512^3; 2 rows: 165 us;
+// the chunk masks add 8 us and save the emit 27 us).
 constexpr int kCountRows = 4;
 
+// Triangles of every cube row, and a chunk mask per row for the emit pass:
+// chunk_bit of the 64-cube chunks holding a triangle in the row's group of
+// kCountRows rows (a superset of the row's own, one ballot per step).
 __global__ __launch_bounds__(256) void mc_row_count_kernel(const McParams p,
-                                                           int32_t* __restrict__ row_count) {
+                                                           int32_t* __restrict__ row_count,
+                                                           uint32_t* __restrict__ row_mask) {
     const int lane = threadIdx.x & 63;
     const int groups = (p.cyr + kCountRows - 1) / kCountRows;
     const int q = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
@@ -40,8 +45,12 @@ __global__ __launch_bounds__(256) void mc_row_count_kernel(const McParams p,
     const int r = y + p.cyr * z;
     if (y + kCountRows > p.cyr) {  // the layer's last, partial group: one row at a time
         for (int i = 0; y + i < p.cyr; ++i) {
-            const int n = mc_row_triangles(p, r + i, lane);
-            if (lane == 0) row_count[r + i] = n;
+            uint32_t m;
+            const int n = mc_row_triangles(p, r + i, lane, &m);
+            if (lane == 0) {
+                row_count[r + i] = n;
+                row_mask[r + i] = m;
+            }
         }
         return;
     }
@@ -49,18 +58,28 @@ __global__ __launch_bounds__(256) void mc_row_count_kernel(const McParams p,
     int n[kCountRows];
 #pragma unroll
     for (int i = 0; i < kCountRows; ++i) n[i] = 0;
+    uint32_t m = 0;  // chunks with a triangle in any of the group's rows
     for (int x0 = 0; x0 < p.cx; x0 += 64) {
         float v[kCountRows][8];
         cs.step(x0, lane, v);
         if (x0 + lane < p.cx) {
+            int any = 0;
 #pragma unroll
-            for (int i = 0; i < kCountRows; ++i) n[i] += class_ntri(cube_class(v[i]));
+            for (int i = 0; i < kCountRows; ++i) {
+                const int t = class_ntri(cube_class(v[i]));
+                n[i] += t;
+                any |= t;
+            }
+            if (__ballot(any != 0)) m |= chunk_bit(x0);
         }
     }
 #pragma unroll
     for (int i = 0; i < kCountRows; ++i) {
         const int t = wave_sum(n[i]);
-        if (lane == 0) row_count[r + i] = t;
+        if (lane == 0) {
+            row_count[r + i] = t;
+            row_mask[r + i] = m;
+        }
     }
 }
 
@@ -85,6 +104,7 @@ __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
 
 __global__ __launch_bounds__(256) void mc_emit_kernel(const McParams p,
                                                       const int32_t* __restrict__ row_count,
+                                                      const uint32_t* __restrict__ row_mask,
                                                       const long long* __restrict__ row_offset,
                                                       float* __restrict__ verts,
                                                       int32_t* __restrict__ tris,
@@ -93,7 +113,7 @@ __global__ __launch_bounds__(256) void mc_emit_kernel(const McParams p,
     const int r = blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
     if (r >= p.rows) return;
     if (row_count[r] == 0) return;
-    mc_emit_row(p, r, lane, row_offset[r], verts, tris, capacity, 0);
+    mc_emit_row_masked(p, r, lane, row_offset[r], row_mask[r], verts, tris, capacity);
 }
 
 __global__ __launch_bounds__(256) void scan_block_sums_kernel(const int32_t* __restrict__ in, int n,
@@ -188,7 +208,8 @@ static int check_grid(const float* field, const int32_t grid[3], const float vs[
     return RECON_OK;
 }
 
-// Scratch layout: row_count int32[rows] | row_offset int64[rows] | sums int64[nb+1]
+// Scratch layout: row_count int32[rows] | row_mask uint32[rows] |
+// row_offset int64[rows] | sums int64[nb+1], each part 256-byte aligned
 int mc_run(const float* field, const int32_t grid[3], const float vs[3], const float off[3],
            float* verts, int32_t* tris, int64_t capacity, int64_t* total, bool count_only,
            void** scratch, size_t* scratch_bytes, hipStream_t s) {
@@ -198,7 +219,8 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     const int rows = p.rows;
     const int nb = (rows + kScanBlock - 1) / kScanBlock;
     const size_t off_count = 0;
-    const size_t off_offset = ((sizeof(int32_t) * (size_t)rows + 255) / 256) * 256;
+    const size_t off_mask = ((sizeof(int32_t) * (size_t)rows + 255) / 256) * 256;
+    const size_t off_offset = off_mask + ((sizeof(uint32_t) * (size_t)rows + 255) / 256) * 256;
     const size_t off_sums = off_offset + ((sizeof(long long) * (size_t)rows + 255) / 256) * 256;
     const size_t need = off_sums + sizeof(long long) * (size_t)(nb + 1);
     if (*scratch_bytes < need) {
@@ -210,12 +232,14 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     }
     char* sb = static_cast<char*>(*scratch);
     int32_t* row_count = reinterpret_cast<int32_t*>(sb + off_count);
+    uint32_t* row_mask = reinterpret_cast<uint32_t*>(sb + off_mask);
     long long* row_offset = reinterpret_cast<long long*>(sb + off_offset);
     long long* sums = reinterpret_cast<long long*>(sb + off_sums);
     const int blocks = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
     const int count_blocks =
         (((p.cyr + kCountRows - 1) / kCountRows) * (grid[2] - 1) + kRowsPerBlock - 1) / kRowsPerBlock;
-    hipLaunchKernelGGL(mc_row_count_kernel, dim3(count_blocks), dim3(256), 0, s, p, row_count);
+    hipLaunchKernelGGL(mc_row_count_kernel, dim3(count_blocks), dim3(256), 0, s, p, row_count,
+                       row_mask);
     RECON_HIP_TRY(hipGetLastError(), "mc_row_count_kernel launch");
     int rc = launch_exclusive_scan(row_count, rows, count_only ? nullptr : row_offset, sums, s);
     if (rc) return rc;
@@ -225,7 +249,7 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     if (!count_only) {
         if (capacity > 0 && verts) {
             hipLaunchKernelGGL(mc_emit_kernel, dim3(blocks), dim3(256), 0, s, p, row_count,
-                               row_offset, verts, tris, (long long)capacity);
+                               row_mask, row_offset, verts, tris, (long long)capacity);
             RECON_HIP_TRY(hipGetLastError(), "mc_emit_kernel launch");
         }
     }

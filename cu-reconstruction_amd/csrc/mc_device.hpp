@@ -204,17 +204,34 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
     return t;
 }
 
+// Bit of the 64-cube chunk starting at x0 in a row's chunk mask: chunk c for
+// c < 31, and bit 31 for every chunk from 31 on.
+__device__ __forceinline__ uint32_t chunk_bit(int x0) { return 1u << min(x0 >> 6, 31); }
+
+// Bitwise OR over the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= (uint32_t)__shfl_xor((int)v, d, 64);
+    return v;
+}
+
 // Triangles of cube row r = y + (gy-1) * z (one wavefront; every lane returns
-// the row total).
-__device__ __forceinline__ int mc_row_triangles(const McParams& p, int r, int lane) {
+// the row total).  With `mask`, also the row's chunk mask (chunk_bit of every
+// chunk with a triangle).
+__device__ __forceinline__ int mc_row_triangles(const McParams& p, int r, int lane,
+                                                uint32_t* mask = nullptr) {
     const int y = r % p.cyr, z = r / p.cyr;
     int n = 0;
+    uint32_t m = 0;
     CornerStream cs(p, y, z, lane);
     for (int x0 = 0; x0 < p.cx; x0 += 64) {
         float v[8];
         cs.step(x0, lane, v);
-        if (x0 + lane < p.cx) n += class_ntri(cube_class(v));
+        const int t = x0 + lane < p.cx ? class_ntri(cube_class(v)) : 0;
+        n += t;
+        m |= t > 0 ? chunk_bit(x0) : 0u;
     }
+    if (mask) *mask = wave_or(m);
     return wave_sum(n);
 }
 
@@ -247,59 +264,96 @@ __device__ __forceinline__ void edge_vertex(const McParams& p, int x, int y, int
     }
 }
 
+// Writes the triangles of the 64 cubes (x0 + lane, y, z) with corners v
+// starting at triangle `base` (one wavefront) and returns their number: a
+// wave-level prefix sum of the per-cube counts places each cube's triangles
+// in x order, slots in table order, and the triangles are spread over the 64
+// lanes (not one lane per cube), so a cube with 5 triangles does not hold up
+// the wave.  Triangles at or past `capacity` are not written.  Vertex indices
+// (tris, may be null) are 3t+k + tri_index_base.
+__device__ __forceinline__ int mc_emit_chunk(const McParams& p, int x0, int y, int z, int lane,
+                                             const float v[8], long long base,
+                                             float* __restrict__ verts,
+                                             int32_t* __restrict__ tris, long long capacity,
+                                             long long tri_index_base) {
+    const int x = x0 + lane;
+    int c = 0, nt = 0;
+    if (x < p.cx) {
+        c = cube_class(v);
+        nt = class_ntri(c);
+    }
+    const int incl = wave_inclusive_scan(nt, lane);
+    const int excl = incl - nt;
+    const int chunk_total = __shfl(incl, 63, 64);
+    // triangle k of the chunk (k = 0 .. chunk_total-1, in the soup's
+    // order) is written by lane k % 64: its cube is the lane whose
+    // triangles include k, its slot k minus that lane's first triangle
+    for (int k0 = 0; k0 < chunk_total; k0 += 64) {
+        const int k = k0 + lane;
+        const int owner = wave_owner(incl, k);
+        const int oc = __shfl(c, owner, 64);
+        const int slot = k - __shfl(excl, owner, 64);
+        float ov[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ov[i] = __shfl(v[i], owner, 64);
+        const long long t = base + k;
+        if (k < chunk_total && t < capacity) {
+            const uint64_t word = c_tri[oc];
+            const int ox = x0 + owner;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
+                float pt[3];
+                edge_vertex(p, ox, y, z, ov, e, pt);
+                float* dst = verts + 9 * t + 3 * q;
+                dst[0] = pt[0];
+                dst[1] = pt[1];
+                dst[2] = pt[2];
+                if (tris) tris[3 * t + q] = (int32_t)(3 * (t - tri_index_base) + q);
+            }
+        }
+    }
+    return chunk_total;
+}
+
 // Writes the triangles of cube row r starting at triangle `base` (one
-// wavefront): a wave-level prefix sum of the per-cube counts places each
-// cube's triangles in x order, slots in table order, and the triangles of
-// 64 cubes are spread over the 64 lanes (not one lane per cube), so a cube
-// with 5 triangles does not hold up the wave.  Triangles at or past
-// `capacity` are not written.  Vertex indices (tris, may be null) are
-// 3t+k + tri_index_base.
+// wavefront), its 64-cube chunks in x order (mc_emit_chunk).
 __device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, long long base,
                                             float* __restrict__ verts, int32_t* __restrict__ tris,
                                             long long capacity, long long tri_index_base) {
     const int y = r % p.cyr, z = r / p.cyr;
     CornerStream cs(p, y, z, lane);
     for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        const int x = x0 + lane;
         float v[8];
-        int c = 0, nt = 0;
         cs.step(x0, lane, v);
-        if (x < p.cx) {
-            c = cube_class(v);
-            nt = class_ntri(c);
-        }
-        const int incl = wave_inclusive_scan(nt, lane);
-        const int excl = incl - nt;
-        const int chunk_total = __shfl(incl, 63, 64);
-        // triangle k of the chunk (k = 0 .. chunk_total-1, in the soup's
-        // order) is written by lane k % 64: its cube is the lane whose
-        // triangles include k, its slot k minus that lane's first triangle
-        for (int k0 = 0; k0 < chunk_total; k0 += 64) {
-            const int k = k0 + lane;
-            const int owner = wave_owner(incl, k);
-            const int oc = __shfl(c, owner, 64);
-            const int slot = k - __shfl(excl, owner, 64);
-            float ov[8];
+        base += mc_emit_chunk(p, x0, y, z, lane, v, base, verts, tris, capacity, tri_index_base);
+    }
+}
+
+// mc_emit_row for a row whose chunk mask (mc_row_triangles) is known: only
+// chunks with triangles are read (their corners loaded directly, x and x+1
+// of each of the four voxel rows), the others hold none and move nothing.
+__device__ __forceinline__ void mc_emit_row_masked(const McParams& p, int r, int lane,
+                                                   long long base, uint32_t mask,
+                                                   float* __restrict__ verts,
+                                                   int32_t* __restrict__ tris,
+                                                   long long capacity) {
+    const int y = r % p.cyr, z = r / p.cyr;
+    const int sy = p.gx, sz = p.gx * p.gy;
+    const int r0 = sy * y + sz * z;
+    // voxel rows (y,z), (y,z+1), (y+1,z), (y+1,z+1), as CornerStream
+    const int rows[4] = {r0, r0 + sz, r0 + sy, r0 + sy + sz};
+    for (int x0 = 0; x0 < p.cx; x0 += 64) {
+        if (!(mask & chunk_bit(x0))) continue;
+        const int x = x0 + lane;
+        float lo[4], hi[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) ov[i] = __shfl(v[i], owner, 64);
-            const long long t = base + k;
-            if (k < chunk_total && t < capacity) {
-                const uint64_t word = c_tri[oc];
-                const int ox = x0 + owner;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
-                    float pt[3];
-                    edge_vertex(p, ox, y, z, ov, e, pt);
-                    float* dst = verts + 9 * t + 3 * q;
-                    dst[0] = pt[0];
-                    dst[1] = pt[1];
-                    dst[2] = pt[2];
-                    if (tris) tris[3 * t + q] = (int32_t)(3 * (t - tri_index_base) + q);
-                }
-            }
+        for (int k = 0; k < 4; ++k) {
+            lo[k] = x < p.gx ? p.f[rows[k] + x] : 0.0f;
+            hi[k] = x + 1 < p.gx ? p.f[rows[k] + x + 1] : 0.0f;
         }
-        base += chunk_total;
+        const float v[8] = {lo[0], hi[0], hi[1], lo[1], lo[2], hi[2], hi[3], lo[3]};
+        base += mc_emit_chunk(p, x0, y, z, lane, v, base, verts, tris, capacity, 0);
     }
 }
 

@@ -58,6 +58,23 @@ def test_random_fields(cuda, grid):
     _mc_compare(cuda, f, grid, (0.01, 0.02, 0.03), (0.1, -0.2, 0.3))
 
 
+def test_sparse_chunks_of_wide_rows(cuda):
+    # rows of 2199 cubes: 35 chunks of 64, so chunks 31-34 share the last bit
+    # of the emit pass's chunk mask; 5 cube rows per layer, so each layer has
+    # one group of 4 rows (one mask for the group) and one row of its own.
+    # Surface voxels sit in a few chunks only, at chunk boundaries and at the
+    # row ends, in some rows and not in others.
+    gx, gy, gz = 2200, 6, 4
+    f = np.full((gz, gy, gx), 0.5, np.float32)
+    xs = [0, 5, 7 * 64 + 10, 30 * 64 + 63, 31 * 64, 31 * 64 + 3, 33 * 64 + 40, gx - 1]
+    for k, x in enumerate(xs):
+        z, y = k % gz, (2 * k) % gy
+        f[z, y, x] = -0.25 - 0.01 * k
+    f[2, 5, 32 * 64 + 1] = -1.0  # the layer's row outside the 4-row group
+    n = _mc_compare(cuda, f, (gx, gy, gz), (0.01, 0.02, 0.03), (0.1, -0.2, 0.3))
+    assert n > 0
+
+
 def test_volume_api_and_capacity(cuda):
     w = scenes.tiny(grid=(40, 36, 28), voxel_size=0.025, image=(64, 48))
     cfg = w.config()
