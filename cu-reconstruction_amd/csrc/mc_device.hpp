@@ -317,22 +317,10 @@ __device__ __forceinline__ int mc_emit_chunk(const McParams& p, int x0, int y, i
 }
 
 // Writes the triangles of cube row r starting at triangle `base` (one
-// wavefront), its 64-cube chunks in x order (mc_emit_chunk).
-__device__ __forceinline__ void mc_emit_row(const McParams& p, int r, int lane, long long base,
-                                            float* __restrict__ verts, int32_t* __restrict__ tris,
-                                            long long capacity, long long tri_index_base) {
-    const int y = r % p.cyr, z = r / p.cyr;
-    CornerStream cs(p, y, z, lane);
-    for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        float v[8];
-        cs.step(x0, lane, v);
-        base += mc_emit_chunk(p, x0, y, z, lane, v, base, verts, tris, capacity, tri_index_base);
-    }
-}
-
-// mc_emit_row for a row whose chunk mask (mc_row_triangles) is known: only
-// chunks with triangles are read (their corners loaded directly, x and x+1
-// of each of the four voxel rows), the others hold none and move nothing.
+// wavefront), its 64-cube chunks in x order (mc_emit_chunk).  Only chunks
+// that `mask` marks (chunk_bit; a superset of those holding a triangle) are
+// read, their corners loaded directly (x and x+1 of each of the four voxel
+// rows); the others hold no triangle and move nothing.
 __device__ __forceinline__ void mc_emit_row_masked(const McParams& p, int r, int lane,
                                                    long long base, uint32_t mask,
                                                    float* __restrict__ verts,

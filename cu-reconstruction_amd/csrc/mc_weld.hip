@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
         const int total = __shfl(incl, 63, 64);
         const long long t0 = tbase[c];
         const int x0 = k.x - lane;
-        // triangle kk of the chunk is written by lane kk % 64 (see mc_emit_row)
+        // triangle kk of the chunk is written by lane kk % 64 (as in mc_emit_chunk)
         for (int k0 = 0; k0 < total; k0 += 64) {
             const int kk = k0 + lane;
             const int owner = wave_owner(incl, kk);

@@ -135,38 +135,55 @@ __global__ __launch_bounds__(kThreads) void batch_integrate_kernel(
 
 // One wavefront (64-thread workgroup) per work-table entry of kMcRowsPerBlock
 // rows: when the entry's rows lie in one layer they are streamed together
-// (their 2 * (kMcRowsPerBlock + 1) voxel rows read once), else row by row.
+// (their 2 * (kMcRowsPerBlock + 1) voxel rows read once, one chunk mask for
+// the entry's rows), else row by row (a chunk mask per row).
 __global__ __launch_bounds__(64) void batch_mc_count_kernel(const McParams* __restrict__ mcs,
                                                             const int2* __restrict__ wg_rows,
-                                                            int32_t* __restrict__ row_count) {
+                                                            int32_t* __restrict__ row_count,
+                                                            uint32_t* __restrict__ row_mask) {
     const int2 e = wg_rows[blockIdx.x];  // (volume, first local row)
     const McParams& p = mcs[e.x];
     const int lane = threadIdx.x;
     const int y = e.y % p.cyr, z = e.y / p.cyr;
     int32_t* out = row_count + blockIdx.x * kMcRowsPerBlock;
+    uint32_t* out_mask = row_mask + blockIdx.x * kMcRowsPerBlock;
     if (y + kMcRowsPerBlock <= p.cyr) {
         CornerStreamK<kMcRowsPerBlock> cs(p, y, z, lane);
         int n[kMcRowsPerBlock];
 #pragma unroll
         for (int i = 0; i < kMcRowsPerBlock; ++i) n[i] = 0;
+        uint32_t m = 0;
         for (int x0 = 0; x0 < p.cx; x0 += 64) {
             float v[kMcRowsPerBlock][8];
             cs.step(x0, lane, v);
             if (x0 + lane < p.cx) {
+                int any = 0;
 #pragma unroll
-                for (int i = 0; i < kMcRowsPerBlock; ++i) n[i] += class_ntri(cube_class(v[i]));
+                for (int i = 0; i < kMcRowsPerBlock; ++i) {
+                    const int t = class_ntri(cube_class(v[i]));
+                    n[i] += t;
+                    any |= t;
+                }
+                if (__ballot(any != 0)) m |= chunk_bit(x0);
             }
         }
 #pragma unroll
         for (int i = 0; i < kMcRowsPerBlock; ++i) {
             const int t = wave_sum(n[i]);
-            if (lane == 0) out[i] = t;
+            if (lane == 0) {
+                out[i] = t;
+                out_mask[i] = m;
+            }
         }
     } else {
         for (int i = 0; i < kMcRowsPerBlock; ++i) {
             const int r = e.y + i;
-            const int t = r < p.rows ? mc_row_triangles(p, r, lane) : 0;
-            if (lane == 0) out[i] = t;
+            uint32_t m = 0;
+            const int t = r < p.rows ? mc_row_triangles(p, r, lane, &m) : 0;
+            if (lane == 0) {
+                out[i] = t;
+                out_mask[i] = m;
+            }
         }
     }
 }
@@ -174,6 +191,7 @@ __global__ __launch_bounds__(64) void batch_mc_count_kernel(const McParams* __re
 __global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McParams* __restrict__ mcs,
                                                             const int2* __restrict__ wg_rows,
                                                             const int32_t* __restrict__ row_count,
+                                                            const uint32_t* __restrict__ row_mask,
                                                             const long long* __restrict__ row_offset,
                                                             float* __restrict__ verts,
                                                             long long capacity) {
@@ -183,7 +201,7 @@ __global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McParams* __re
     const int r = e.y + w;
     const int g = blockIdx.x * kMcRowsPerBlock + w;
     if (r >= p.rows || row_count[g] == 0) return;
-    mc_emit_row(p, r, lane, row_offset[g], verts, nullptr, capacity, 0);
+    mc_emit_row_masked(p, r, lane, row_offset[g], row_mask[g], verts, nullptr, capacity);
 }
 
 // ranges[2v], ranges[2v+1] = first triangle and triangle count of volume v.
@@ -318,6 +336,7 @@ struct recon_batch {
     int2* d_mc_wg = nullptr;
     int mc_blocks = 0, total_rows = 0;
     int32_t* row_count = nullptr;
+    uint32_t* row_mask = nullptr;
     long long* row_offset = nullptr;
     long long* sums = nullptr;
     int nb = 0;
@@ -370,12 +389,12 @@ int recon_batch::record(hipStream_t s) {
     }
     if (with_mesh && mc_blocks > 0) {
         hipLaunchKernelGGL(batch_mc_count_kernel, dim3(mc_blocks), dim3(64), 0, s, d_mc, d_mc_wg,
-                           row_count);
+                           row_count, row_mask);
         RECON_HIP_TRY(hipGetLastError(), "batch_mc_count_kernel launch");
         int rc = launch_exclusive_scan(row_count, total_rows, row_offset, sums, s);
         if (rc) return rc;
         hipLaunchKernelGGL(batch_mc_emit_kernel, dim3(mc_blocks), dim3(256), 0, s, d_mc, d_mc_wg,
-                           row_count, row_offset, verts, capacity);
+                           row_count, row_mask, row_offset, verts, capacity);
         RECON_HIP_TRY(hipGetLastError(), "batch_mc_emit_kernel launch");
         hipLaunchKernelGGL(batch_mesh_ranges_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_first_row,
                            n, total_rows, row_offset, sums + nb, d_ranges);
@@ -583,6 +602,7 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
             (rc = b->upload(&b->d_mc_wg, mc_wg, "batch mc work table")) ||
             (rc = b->upload(&b->d_first_row, first_row, "batch mc first rows")) ||
             (rc = b->alloc(&b->row_count, b->total_rows, "batch row counts")) ||
+            (rc = b->alloc(&b->row_mask, b->total_rows, "batch row chunk masks")) ||
             (rc = b->alloc(&b->row_offset, b->total_rows, "batch row offsets")) ||
             (rc = b->alloc(&b->sums, b->nb + 1, "batch scan sums")) ||
             (rc = b->alloc(&b->d_ranges, 2 * (size_t)n, "batch mesh ranges")) ||
