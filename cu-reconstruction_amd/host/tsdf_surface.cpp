@@ -112,7 +112,15 @@ bool TSDFSurface::reconstruct(const PtrStepSz<float>& depth, const unsigned char
     return true;
 }
 
-bool TSDFSurface::reconstruct_surface(PointCloud& cloud) {
+bool TSDFSurface::reconstruct_surface(const cudaPointCloud::Ptr cuda_pc) {
+    if (!cuda_pc) {
+        std::fprintf(stderr, "[cuphoto] TSDFSurface::reconstruct_surface() -- point cloud is null\n");
+        return false;
+    }
+    return reconstruct_surface(*cuda_pc);
+}
+
+bool TSDFSurface::reconstruct_surface(cudaPointCloud& cloud) {
     const int rows = (int)cfg_.camera_height, cols = (int)cfg_.camera_width;
     if (rows <= 0 || cols <= 0) {
         std::fprintf(stderr, "[cuphoto] TSDFSurface::reconstruct_surface() -- camera size is zero\n");

@@ -190,27 +190,29 @@ void generate_triangular_surface(const TSDFVolume& tsdf_volume, std::vector<floa
     }
 }
 
-PointCloud::PointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
-                       hipStream_t stream)
+cudaPointCloud::cudaPointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
+                               hipStream_t stream)
     : K_(camera_mat), total_(total_number_pts), stream_(stream) {
     if (total_ == 0) return;
     if (hipMalloc((void**)&xyz_, 12 * total_) != hipSuccess ||
         hipMalloc((void**)&valid_, total_) != hipSuccess ||
         hipMalloc((void**)&rgb_, 3 * total_) != hipSuccess) {
-        std::fprintf(stderr, "[cuphoto] PointCloud -- can't allocate %zu points\n", total_);
+        std::fprintf(stderr, "[cuphoto] cudaPointCloud::reserve_memory() -- can't allocate %zu points\n",
+                     total_);
         total_ = 0;
     }
 }
 
-PointCloud::~PointCloud() {
+cudaPointCloud::~cudaPointCloud() {
     if (xyz_) (void)hipFree(xyz_);
     if (valid_) (void)hipFree(valid_);
     if (rgb_) (void)hipFree(rgb_);
     if (zbuffer_) (void)hipFree(zbuffer_);
+    if (packed_) (void)hipHostFree(packed_);
 }
 
-bool PointCloud::render_raw(float* depth, size_t step, int rows, int cols, uint8_t* color,
-                            size_t color_step, const std::array<r64, 7>& pose, const char* who) {
+bool cudaPointCloud::render_raw(float* depth, size_t step, int rows, int cols, uint8_t* color,
+                                size_t color_step, const std::array<r64, 7>& pose, const char* who) {
     if (!depth) {  // cuda_point_cloud.cu:231-234
         std::fprintf(stderr, "[cuphoto] %s -- depth map is null pointer\n", who);
         return false;
@@ -239,17 +241,17 @@ bool PointCloud::render_raw(float* depth, size_t step, int rows, int cols, uint8
     return sync(stream_, who);
 }
 
-bool PointCloud::extract_raw(const float* depth, size_t step, int rows, int cols,
-                             const uint8_t* rgb, size_t rgb_step, const std::array<r64, 7>& T,
-                             int frame_idx) {
+bool cudaPointCloud::extract_raw(const float* depth, size_t step, int rows, int cols,
+                                 const uint8_t* rgb, size_t rgb_step, const std::array<r64, 7>& T,
+                                 int frame_idx) {
     if (!depth) {
-        std::fprintf(stderr, "[cuphoto] PointCloud::extract_points() -- depth map is null pointer\n");
+        std::fprintf(stderr, "[cuphoto] cudaPointCloud::extract_points() -- depth map is null pointer\n");
         return false;
     }
     const size_t n = (size_t)rows * (size_t)cols;
     const size_t start = (size_t)frame_idx * n;
     if (frame_idx < 0 || start + n > total_) {
-        std::fprintf(stderr, "[cuphoto] PointCloud::extract_points() -- frame %d does not fit\n",
+        std::fprintf(stderr, "[cuphoto] cudaPointCloud::extract_points() -- frame %d does not fit\n",
                      frame_idx);
         return false;
     }
@@ -257,14 +259,61 @@ bool PointCloud::extract_raw(const float* depth, size_t step, int rows, int cols
     // column-major K: cx, cy at K[2][0], K[2][1] (cuda_point_cloud.cu:44-47)
     if (recon_depth_to_points(&dv, K_[0], K_[4], K_[6], K_[7], T.data(), rgb, rgb_step,
                               xyz_ + 3 * start, valid_ + start, rgb_ + 3 * start, stream_) != RECON_OK) {
-        log_error("PointCloud::extract_points()");
+        log_error("cudaPointCloud::extract_points()");
         return false;
     }
     num_ += n;  // cuda_point_cloud.cu:221 counts every pixel
-    return sync(stream_, "PointCloud::extract_points()");
+    dirty_ = true;
+    return sync(stream_, "cudaPointCloud::extract_points()");
 }
 
-bool PointCloud::save_ply(const std::string& path) const {
+cudaPointCloud::Vertex* cudaPointCloud::get_points() const {
+    if (total_ == 0) return nullptr;
+    if (!packed_) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, sizeof(Vertex) * total_, hipHostMallocDefault) != hipSuccess) {
+            std::fprintf(stderr, "[cuphoto] cudaPointCloud::get_points() -- can't allocate %zu "
+                                 "vertices\n", total_);
+            return nullptr;
+        }
+        packed_ = static_cast<Vertex*>(p);
+        dirty_ = true;
+    }
+    if (!dirty_) return packed_;
+    std::vector<float> xyz(3 * total_);
+    std::vector<uint8_t> rgb(3 * total_);
+    if (!sync(stream_, "cudaPointCloud::get_points()") ||
+        hipMemcpy(xyz.data(), xyz_, 12 * total_, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(rgb.data(), rgb_, 3 * total_, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::fprintf(stderr, "[cuphoto] cudaPointCloud::get_points() -- copy failed\n");
+        return nullptr;
+    }
+    for (size_t i = 0; i < total_; ++i) {
+        Vertex& v = packed_[i];
+        v.pos.x = xyz[3 * i];
+        v.pos.y = xyz[3 * i + 1];
+        v.pos.z = xyz[3 * i + 2];
+        v.normal.x = v.normal.y = v.normal.z = v.normal.w = 0.f;
+        v.color.x = rgb[3 * i];
+        v.color.y = rgb[3 * i + 1];
+        v.color.z = rgb[3 * i + 2];
+        v.classID = 0;
+    }
+    dirty_ = false;
+    return packed_;
+}
+
+void cudaPointCloud::clear() {
+    if (total_ && (hipMemsetAsync(xyz_, 0, 12 * total_, stream_) != hipSuccess ||
+                   hipMemsetAsync(valid_, 0, total_, stream_) != hipSuccess ||
+                   hipMemsetAsync(rgb_, 0, 3 * total_, stream_) != hipSuccess))
+        std::fprintf(stderr, "[cuphoto] cudaPointCloud::clear() -- memset failed\n");
+    (void)sync(stream_, "cudaPointCloud::clear()");
+    num_ = 0;
+    dirty_ = true;
+}
+
+bool cudaPointCloud::save_ply(const std::string& path) const {
     if (path.empty() || num_ == 0) return false;
     std::vector<float> xyz(3 * num_);
     std::vector<uint8_t> valid(num_), rgb(3 * num_);

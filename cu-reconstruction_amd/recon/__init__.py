@@ -59,12 +59,65 @@ def describe(cfg: TsdfConfig) -> TsdfInfo:
     return info
 
 
+def _device_tensor(t, name, dtype):
+    """Raises ValueError unless `t` is a torch tensor of `dtype` in GPU
+    memory: the kernels read raw device addresses, so a host tensor or a
+    wrong element type must be refused before any call."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be in GPU memory")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, not {t.dtype}")
+    return t
+
+
 def depth_view(t) -> DepthView:
     """View of a 2-D float32 device tensor (row stride may exceed cols)."""
     torch = _torch()
-    if t.dtype != torch.float32 or t.dim() != 2 or t.stride(1) != 1:
+    _device_tensor(t, "depth", torch.float32)
+    if t.dim() != 2 or t.stride(1) != 1 or t.stride(0) < t.shape[1]:
         raise ValueError("depth must be a 2-D float32 tensor with unit column stride")
     return DepthView(t.data_ptr(), t.stride(0) * 4, t.shape[0], t.shape[1])
+
+
+def _frames_view(depths, name="depths"):
+    """View of frame 0 of a (F, H, W) float32 device stack."""
+    torch = _torch()
+    _device_tensor(depths, name, torch.float32)
+    if depths.dim() != 3 or depths.stride(2) != 1 or depths.stride(1) < depths.shape[2] or \
+            (depths.shape[0] > 1 and depths.stride(0) < depths.shape[1] * depths.stride(1)):
+        raise ValueError(f"{name} must be (F, H, W) with unit column stride and non-overlapping frames")
+    return DepthView(depths.data_ptr(), depths.stride(1) * 4, depths.shape[1], depths.shape[2])
+
+
+def _rgb_arg(rgb, rows, cols):
+    """(pointer, row step) of an optional (rows, cols, 3) uint8 device image
+    with packed pixels (the row stride may exceed 3*cols)."""
+    if rgb is None:
+        return None, 0
+    torch = _torch()
+    _device_tensor(rgb, "rgb", torch.uint8)
+    if rgb.dim() != 3 or rgb.shape[2] != 3 or rgb.stride(2) != 1 or rgb.stride(1) != 3 or \
+            rgb.stride(0) < 3 * rgb.shape[1]:
+        raise ValueError("rgb must be a (rows, cols, 3) uint8 tensor with packed pixels")
+    if rgb.shape[0] < rows or rgb.shape[1] < cols:
+        raise ValueError(f"rgb ({rgb.shape[0]}x{rgb.shape[1]}) is smaller than the depth "
+                         f"({rows}x{cols})")
+    return rgb.data_ptr(), rgb.stride(0)
+
+
+def _field_arg(field, grid):
+    """Checks a dense float32[gz][gy][gx] device field covers `grid`."""
+    torch = _torch()
+    _device_tensor(field, "field", torch.float32)
+    g = [int(v) for v in grid]
+    if len(g) != 3 or min(g) < 0:
+        raise ValueError("grid must be three non-negative sizes")
+    if not field.is_contiguous() or field.numel() < g[0] * g[1] * g[2]:
+        raise ValueError(f"field must be contiguous with at least {g[0] * g[1] * g[2]} elements")
+    return field.data_ptr()
 
 
 class TSDFVolume:
@@ -117,19 +170,16 @@ class TSDFVolume:
     def integrate(self, depth, pose, stream=None, rgb=None):
         """TSDFVolume::integrate (tsdf_volume.cu:257-290); async on the stream."""
         dv = depth_view(depth)
-        rgb_ptr = rgb.data_ptr() if rgb is not None else None
-        rgb_step = rgb.stride(0) if rgb is not None else 0
+        rgb_ptr, rgb_step = _rgb_arg(rgb, dv.rows, dv.cols)
         check(self._lib.recon_tsdf_integrate(self._h, C.byref(dv), rgb_ptr, rgb_step,
                                              ffi.pose_array(pose), _stream(stream)))
 
     def integrate_frames(self, depths, poses, stream=None):
         """All frames of a (F, H, W) float32 tensor in one sweep per 16 frames."""
-        if depths.dim() != 3 or depths.stride(2) != 1:
-            raise ValueError("depths must be (F, H, W) with unit column stride")
+        dv = _frames_view(depths)
         poses = np.asarray(poses, dtype=np.float64).reshape(-1, 7)
         if poses.shape[0] != depths.shape[0]:
             raise ValueError("one pose per frame")
-        dv = DepthView(depths.data_ptr(), depths.stride(1) * 4, depths.shape[1], depths.shape[2])
         check(self._lib.recon_tsdf_integrate_frames(
             self._h, C.byref(dv), depths.stride(0) * 4, int(depths.shape[0]),
             ffi.pose_array(poses.ravel()), _stream(stream)))
@@ -137,9 +187,7 @@ class TSDFVolume:
     # ---- differentiable integrate -------------------------------------------
     @staticmethod
     def _frames_view(depths):
-        if depths.dim() != 3 or depths.stride(2) != 1:
-            raise ValueError("depths must be (F, H, W) with unit column stride")
-        return DepthView(depths.data_ptr(), depths.stride(1) * 4, depths.shape[1], depths.shape[2])
+        return _frames_view(depths)
 
     @staticmethod
     def _check_rt(rt, nframes):
@@ -161,7 +209,8 @@ class TSDFVolume:
         """(loss (1,) float64, residuals (K,), lambda ids (8K,) int64, lambda
         (8K,) float32) for (K, 3) float32 device points."""
         torch = _torch()
-        if points.dtype != torch.float32 or not points.is_contiguous() or points.dim() != 2:
+        _device_tensor(points, "points", torch.float32)
+        if not points.is_contiguous() or points.dim() != 2 or points.shape[1] != 3:
             raise ValueError("points must be a contiguous (K, 3) float32 device tensor")
         K = int(points.shape[0])
         dev = points.device
@@ -171,7 +220,8 @@ class TSDFVolume:
         lam = torch.empty(8 * K, dtype=torch.float32, device=dev) if want_lambda else None
         wptr = None
         if weights is not None:
-            if weights.dtype != torch.float32 or not weights.is_contiguous() or weights.numel() != K:
+            _device_tensor(weights, "weights", torch.float32)
+            if not weights.is_contiguous() or weights.numel() != K:
                 raise ValueError("weights must be a contiguous (K,) float32 device tensor")
             wptr = weights.data_ptr()
         check(self._lib.recon_tsdf_surface_loss(
@@ -187,6 +237,8 @@ class TSDFVolume:
         torch = _torch()
         F = int(depths.shape[0])
         self._check_rt(poses_rt, F)
+        _device_tensor(lambda_ids, "lambda_ids", torch.int64)
+        _device_tensor(lam, "lam", torch.float32)
         if lambda_ids.dtype != torch.int64 or lam.dtype != torch.float32 or \
                 lambda_ids.numel() != lam.numel() or not lambda_ids.is_contiguous() or \
                 not lam.is_contiguous():
@@ -202,7 +254,14 @@ class TSDFVolume:
         return gp, gd
 
     def set_update_counter(self, counter_tensor):
-        ptr = counter_tensor.data_ptr() if counter_tensor is not None else None
+        """Later integrate calls add their update counts to element 0 of
+        `counter_tensor` (int64 on the GPU); None switches counting off."""
+        ptr = None
+        if counter_tensor is not None:
+            _device_tensor(counter_tensor, "counter", _torch().int64)
+            if counter_tensor.numel() < 1:
+                raise ValueError("counter must hold at least one element")
+            ptr = counter_tensor.data_ptr()
         check(self._lib.recon_tsdf_set_update_counter(self._h, ptr))
 
     def download(self, stream=None):
@@ -232,6 +291,9 @@ class TSDFVolume:
     def sample_colors(self, points, stream=None):
         """Colours (N, 3) uint8 at (N, 3) float32 device points."""
         torch = _torch()
+        _device_tensor(points, "points", torch.float32)
+        if points.numel() % 3:
+            raise ValueError("points must hold 3 floats per point")
         pts = points.reshape(-1, 3).contiguous()
         out = torch.empty((pts.shape[0], 3), dtype=torch.uint8, device=pts.device)
         check(self._lib.recon_tsdf_sample_colors(self._h, pts.data_ptr(), pts.shape[0],
@@ -261,6 +323,7 @@ class TSDFVolume:
             n = self.count_triangles(stream)
             verts = torch.empty((max(n, 1), 3, 3), dtype=torch.float32, device="cuda")
             tris = torch.empty((max(n, 1), 3), dtype=torch.int32, device="cuda")
+        _check_soup_buffers(verts, tris)
         cap = verts.shape[0]
         n = C.c_int64()
         check(self._lib.recon_tsdf_extract_mesh(self._h, verts.data_ptr(),
@@ -350,6 +413,18 @@ class SparseVolume:
         return {tuple(int(q) for q in c[i]): (D[i], W[i]) for i in range(int(got.value))}
 
 
+def _check_soup_buffers(verts, tris):
+    torch = _torch()
+    _device_tensor(verts, "verts", torch.float32)
+    if verts.dim() != 3 or tuple(verts.shape[1:]) != (3, 3) or not verts.is_contiguous():
+        raise ValueError("verts must be a contiguous (T, 3, 3) float32 device tensor")
+    if tris is not None:
+        _device_tensor(tris, "tris", torch.int32)
+        if tris.dim() != 2 or tris.shape[1] != 3 or not tris.is_contiguous() or \
+                tris.shape[0] < verts.shape[0]:
+            raise ValueError("tris must be a contiguous (T, 3) int32 device tensor as long as verts")
+
+
 def _welded(call, with_normals, stream, out=None):
     torch = _torch()
     nv, nt = C.c_int64(), C.c_int64()
@@ -362,6 +437,16 @@ def _welded(call, with_normals, stream, out=None):
         tris = torch.empty((max(T, 1), 3), dtype=torch.int32, device="cuda")
     else:
         verts, normals, tris = out
+        for name, t, dt in (("vertices", verts, torch.float32), ("normals", normals, torch.float32),
+                            ("triangles", tris, torch.int32)):
+            if t is None and name != "normals":
+                raise ValueError(f"{name} buffer missing")
+            if t is not None:
+                _device_tensor(t, name, dt)
+                if t.dim() != 2 or t.shape[1] != 3 or not t.is_contiguous():
+                    raise ValueError(f"{name} must be a contiguous (n, 3) device tensor")
+        if normals is not None and normals.shape[0] < verts.shape[0]:
+            raise ValueError("normals must be as long as vertices")
     check(call(verts.data_ptr(), normals.data_ptr() if normals is not None else None,
                verts.shape[0], tris.data_ptr(), tris.shape[0], C.byref(nv), C.byref(nt),
                _stream(stream)))
@@ -371,8 +456,9 @@ def _welded(call, with_normals, stream, out=None):
 
 def mc_extract_welded(field, grid, voxel_size, offset, with_normals=True, out=None, stream=None):
     """Welded marching cubes of a dense field (recon_mc_extract_welded)."""
+    fp = _field_arg(field, grid)
     g, vs, off = _grid_args(grid, voxel_size, offset)
-    return _welded(lambda *a: ffi.load().recon_mc_extract_welded(field.data_ptr(), g, vs, off, *a),
+    return _welded(lambda *a: ffi.load().recon_mc_extract_welded(fp, g, vs, off, *a),
                    with_normals, stream, out)
 
 
@@ -381,21 +467,23 @@ def _grid_args(grid, voxel_size, offset):
 
 
 def mc_count(field, grid, voxel_size, offset, stream=None) -> int:
+    fp = _field_arg(field, grid)
     g, vs, off = _grid_args(grid, voxel_size, offset)
     n = C.c_int64()
-    check(ffi.load().recon_mc_count(field.data_ptr(), g, vs, off, C.byref(n), _stream(stream)))
+    check(ffi.load().recon_mc_count(fp, g, vs, off, C.byref(n), _stream(stream)))
     return int(n.value)
 
 
 def mc_extract(field, grid, voxel_size, offset, capacity=None, stream=None, with_tris=True):
     torch = _torch()
+    fp = _field_arg(field, grid)
     if capacity is None:
         capacity = mc_count(field, grid, voxel_size, offset, stream)
     verts = torch.empty((max(capacity, 1), 3, 3), dtype=torch.float32, device=field.device)
     tris = torch.empty((max(capacity, 1), 3), dtype=torch.int32, device=field.device) if with_tris else None
     g, vs, off = _grid_args(grid, voxel_size, offset)
     n = C.c_int64()
-    st = ffi.load().recon_mc_extract(field.data_ptr(), g, vs, off, verts.data_ptr(),
+    st = ffi.load().recon_mc_extract(fp, g, vs, off, verts.data_ptr(),
                                      tris.data_ptr() if tris is not None else None, capacity,
                                      C.byref(n), _stream(stream))
     check(st)
@@ -405,8 +493,9 @@ def mc_extract(field, grid, voxel_size, offset, capacity=None, stream=None, with
 
 def mc_class_histogram(field, grid, stream=None):
     torch = _torch()
+    fp = _field_arg(field, grid)
     hist = torch.zeros(256, dtype=torch.int64, device=field.device)
-    check(ffi.load().recon_mc_class_histogram(field.data_ptr(), ffi.i32x3(grid), hist.data_ptr(),
+    check(ffi.load().recon_mc_class_histogram(fp, ffi.i32x3(grid), hist.data_ptr(),
                                               _stream(stream)))
     return hist
 
@@ -426,6 +515,7 @@ def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, s
     colours (H*W, 3) uint8."""
     torch = _torch()
     dv = depth_view(depth)
+    rgb_ptr, rgb_step = _rgb_arg(rgb, dv.rows, dv.cols)
     n = depth.shape[0] * depth.shape[1]
     xyz = torch.empty((n, 3), dtype=torch.float32, device=depth.device)
     valid = torch.empty((n,), dtype=torch.uint8, device=depth.device)
@@ -433,7 +523,7 @@ def depth_to_points(depth, intrinsics, transform, rgb=None, with_colors=False, s
     fx, fy, cx, cy = [float(v) for v in intrinsics]
     check(ffi.load().recon_depth_to_points(
         C.byref(dv), fx, fy, cx, cy, ffi.pose_array(transform),
-        rgb.data_ptr() if rgb is not None else None, rgb.stride(0) if rgb is not None else 0,
+        rgb_ptr, rgb_step,
         xyz.data_ptr(), valid.data_ptr(), colors.data_ptr() if colors is not None else None,
         _stream(stream)))
     return (xyz, valid, colors) if with_colors else (xyz, valid)
@@ -449,12 +539,19 @@ def render_points(xyz, pose, intrinsics, image, valid=None, colors=None, with_co
     torch = _torch()
     cols, rows = int(image[0]), int(image[1])
     dev = xyz.device
-    if xyz.dtype != torch.float32 or xyz.dim() != 2 or xyz.shape[1] != 3 or not xyz.is_contiguous():
+    _device_tensor(xyz, "xyz", torch.float32)
+    if xyz.dim() != 2 or xyz.shape[1] != 3 or not xyz.is_contiguous():
         raise ValueError("xyz must be a contiguous (N, 3) float32 tensor")
     n = xyz.shape[0]
     for name, t, shape in (("valid", valid, (n,)), ("colors", colors, (n, 3))):
-        if t is not None and (t.dtype != torch.uint8 or tuple(t.shape) != shape or not t.is_contiguous()):
-            raise ValueError(f"{name} must be a contiguous uint8 tensor of shape {shape}")
+        if t is not None:
+            _device_tensor(t, name, torch.uint8)
+            if tuple(t.shape) != shape or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous uint8 tensor of shape {shape}")
+    for name, t, dt in (("depth_out", depth_out, torch.float32), ("color_out", color_out, torch.uint8),
+                        ("zbuffer", zbuffer, torch.int64)):
+        if t is not None:
+            _device_tensor(t, name, dt)
     if depth_out is None:
         depth_out = torch.empty((rows, cols), dtype=torch.float32, device=dev)
     if with_colors and color_out is None:
@@ -496,7 +593,8 @@ class Batch:
             v.config = cfg
             v.num_frames = int(poses.shape[0])
             if v.num_frames:
-                if depth.dim() != 3 or depth.shape[0] != v.num_frames or depth.stride(2) != 1:
+                _frames_view(depth, "depth")
+                if depth.shape[0] != v.num_frames:
                     raise ValueError("depth must be (F, H, W) with one frame per pose")
                 v.rows, v.cols = int(depth.shape[1]), int(depth.shape[2])
                 v.depth_step = depth.stride(1) * 4
@@ -516,6 +614,10 @@ class Batch:
         keep = []
         for i, (cfg, depth, poses) in enumerate(volumes):
             poses = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, 7))
+            if poses.shape[0]:
+                _frames_view(depth, "depth")
+                if depth.shape[0] != poses.shape[0]:
+                    raise ValueError("depth must be (F, H, W) with one frame per pose")
             keep.append(poses)
             b[i].config = cfg
             b[i].num_frames = int(poses.shape[0])
@@ -581,10 +683,14 @@ class DiffBatch:
         arr = (ffi.DiffVolume * len(volumes))()
         frames = 0
         for i, (cfg, depth, points, weights) in enumerate(volumes):
-            if depth.dim() != 3 or depth.stride(2) != 1 or depth.dtype != torch.float32:
-                raise ValueError("depth must be a (F, H, W) float32 tensor")
-            if points.dtype != torch.float32 or not points.is_contiguous() or points.dim() != 2:
+            _frames_view(depth, "depth")
+            _device_tensor(points, "points", torch.float32)
+            if not points.is_contiguous() or points.dim() != 2 or points.shape[1] != 3:
                 raise ValueError("points must be a contiguous (K, 3) float32 tensor")
+            if weights is not None:
+                _device_tensor(weights, "weights", torch.float32)
+                if not weights.is_contiguous() or weights.numel() != points.shape[0]:
+                    raise ValueError("weights must be a contiguous (K,) float32 tensor")
             self._keep.append((depth, points, weights))
             v = arr[i]
             v.config = cfg
@@ -613,6 +719,7 @@ class DiffBatch:
 
     def _check_rt(self, rt):
         torch = _torch()
+        _device_tensor(rt, "poses_rt", torch.float32)
         if rt.dtype != torch.float32 or not rt.is_contiguous() or \
                 tuple(rt.shape) != (self.total_frames, 12):
             raise ValueError("poses_rt must be a contiguous (total frames, 12) float32 tensor")

@@ -113,11 +113,12 @@ public:
     bool reconstruct_surface(const PtrStepSz<float>& depth) { return reconstruct(depth, nullptr, 0); }
     // tsdf_surface_reconstruction.cpp:94-120: renders the cloud at the global
     // pose into a camera_width x camera_height depth and colour pair
-    // (PointCloud::project_to_color_depth), integrates it and runs marching
-    // cubes.  The reference stops after the render (its integrate and
-    // meshing calls are commented out, :114-118) and computes normals that
-    // nothing reads.
-    bool reconstruct_surface(PointCloud& cloud);
+    // (cudaPointCloud::project_to_color_depth), integrates it and runs
+    // marching cubes.  The reference stops after the render (its integrate
+    // and meshing calls are commented out, :114-118) and computes normals
+    // that nothing reads.  A null cloud fails (logged).
+    bool reconstruct_surface(const cudaPointCloud::Ptr cuda_pc);
+    bool reconstruct_surface(cudaPointCloud& cloud);
     const Mesh& get_mesh() const { return surface_mesh; }
 
 private:

@@ -5,7 +5,9 @@
 //   TSDFVolumeConfig             modules/include/cuda/tsdf_volume.cuh:17-29
 //   TSDFVolume                   modules/include/cuda/tsdf_volume.cuh:31-105
 //   generate_triangular_surface  modules/include/cuda/marching_cubes.cuh:13-15
-//   cudaPointCloud::extract_points  modules/include/cuda/cuda_point_cloud.cuh:55-58
+//   cudaPointCloud               modules/include/cuda/cuda_point_cloud.cuh:14-88
+//                                (create, extract_points, project_to_depth,
+//                                project_to_color_depth, get_points, ...)
 //   write_mesh_to_ply            modules/src/utils.cpp:96-118
 // Calls block until the device work is done (the reference synchronises the
 // device after every kernel) and report failure as `false` plus a log line on
@@ -13,8 +15,9 @@
 // data, step in bytes, rows, cols), so OpenCV's types pass through unchanged.
 // Differences that follow the build contract: colour is optional, the weight
 // cap is applied, the sdf uses camera-space z, marching cubes uses the gx*gy
-// layer stride, and voxel centres are computed, not stored (there is no
-// transformation_voxels_data()).
+// layer stride, and voxel centres are computed, not stored
+// (transformation_voxels_data() synthesises the reference's records on its
+// first call).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -23,6 +26,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -84,9 +88,18 @@ public:
     TSDFVolume& operator=(const TSDFVolume&) = delete;
 
     // tsdf_volume.cu:257-290; `color` is ignored, as in the reference, unless
-    // enable_color() was called (then band voxels average it).
+    // enable_color() was called (then band voxels average it).  A colour
+    // image given with data must cover the depth image (rows and cols in
+    // pixels, as cv::cuda::PtrStepSzb of a CV_8UC3 GpuMat counts them), or
+    // the call fails, as the reference fails on a zero-sized one (:265-268).
     template <typename Color, typename Depth>
     bool integrate(const Color& color, const Depth& depth, const std::array<r64, 7>& camera_pose) {
+        if (color.data && (color.rows < depth.rows || color.cols < depth.cols)) {
+            std::fprintf(stderr, "[cuphoto] TSDFVolume::integrate() -- color map (%dx%d) is smaller "
+                                 "than the depth map (%dx%d)\n", (int)color.cols, (int)color.rows,
+                         (int)depth.cols, (int)depth.rows);
+            return false;
+        }
         return integrate_raw(reinterpret_cast<const uint8_t*>(color.data), color.step,
                              reinterpret_cast<const float*>(depth.data), depth.step, depth.rows,
                              depth.cols, camera_pose);
@@ -150,18 +163,44 @@ void generate_triangular_surface(const TSDFVolume& tsdf_volume, std::vector<floa
 void generate_welded_surface(const TSDFVolume& tsdf_volume, std::vector<float3>& verts,
                              std::vector<float3>& normals, std::vector<int3>& triangles);
 
-// Device point buffer filled by extract_points (cudaPointCloud's role on the
-// TSDF path): xyz float3, a validity byte and an rgb triple per point.
-class PointCloud {
+// cudaPointCloud (cuda_point_cloud.cuh:14-88): the device point buffer that
+// extract_points fills and project_to_depth / project_to_color_depth render.
+// Points are kept as three device planes -- xyz float3, a validity byte and
+// an rgb triple -- which the kernels read and write coalesced; get_points()
+// packs them into the reference's 33-byte Vertex records on request.
+class cudaPointCloud {
 public:
-    // camera_mat: column-major 3x3 intrinsics, as the reference passes them
-    // (K[0][0], K[1][1], K[2][0], K[2][1] are fx, fy, cx, cy;
-    // cuda_point_cloud.cu:44-47).
-    PointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
-               hipStream_t stream = nullptr);
-    ~PointCloud();
-    PointCloud(const PointCloud&) = delete;
-    PointCloud& operator=(const PointCloud&) = delete;
+    // cuda_point_cloud.cuh:17-28: packed {float3 pos; float4 normal; uchar3
+    // color; ui16 classID}.  Plain members: HIP's vector types are not POD,
+    // so a packed struct of them would not be 33 bytes; .x/.y/.z access is
+    // the same.
+    struct Vertex {
+        using Ptr = Vertex*;
+        struct { float x, y, z; } pos;
+        struct { float x, y, z, w; } normal;
+        struct { uint8_t x, y, z; } color;
+        uint16_t classID;
+    } __attribute__((packed));
+    static_assert(sizeof(Vertex) == 33, "packed 33-byte vertex");
+
+    using Ptr = std::shared_ptr<cudaPointCloud>;
+
+    // cuda_point_cloud.cu:121-124: camera_mat is the column-major 3x3
+    // intrinsics (K[0], K[4], K[6], K[7] are fx, fy, cx, cy;
+    // cuda_point_cloud.cu:44-47).  Throws std::bad_alloc when the device
+    // buffers cannot be allocated (cuda_point_cloud.cu:107-112).
+    static Ptr create(const std::array<r64, 9>& camera_mat, size_t total_number_pts) {
+        Ptr p = std::make_shared<cudaPointCloud>(camera_mat, total_number_pts);
+        if (p->get_total_num_points() != total_number_pts) throw std::bad_alloc();
+        return p;
+    }
+    // As create(), with the stream the calls run on; allocation failure is
+    // logged and leaves get_total_num_points() == 0.
+    cudaPointCloud(const std::array<r64, 9>& camera_mat, size_t total_number_pts,
+                   hipStream_t stream = nullptr);
+    ~cudaPointCloud();
+    cudaPointCloud(const cudaPointCloud&) = delete;
+    cudaPointCloud& operator=(const cudaPointCloud&) = delete;
 
     // cuda_point_cloud.cu:179-226: back-projects frame `frame_idx` into
     // points [frame_idx*W*H, (frame_idx+1)*W*H).
@@ -183,27 +222,46 @@ public:
     template <typename Depth>
     bool project_to_depth(Depth& depth, const std::array<r64, 7>& camera_pose) {
         return render_raw(reinterpret_cast<float*>(depth.data), depth.step, depth.rows, depth.cols,
-                          nullptr, 0, camera_pose, "PointCloud::project_to_depth()");
+                          nullptr, 0, camera_pose, "cudaPointCloud::project_to_depth()");
     }
     template <typename Color, typename Depth>
     bool project_to_color_depth(Color& color, Depth& depth, const std::array<r64, 7>& camera_pose) {
         if (!color.data) {
-            std::fprintf(stderr, "[cuphoto] PointCloud::project_to_color_depth() -- color map is "
-                                 "null pointer\n");
+            std::fprintf(stderr, "[cuphoto] cudaPointCloud::project_to_color_depth() -- color map "
+                                 "is null pointer\n");
             return false;
         }
         return render_raw(reinterpret_cast<float*>(depth.data), depth.step, depth.rows, depth.cols,
                           reinterpret_cast<uint8_t*>(color.data), color.step, camera_pose,
-                          "PointCloud::project_to_color_depth()");
+                          "cudaPointCloud::project_to_color_depth()");
     }
-    float* xyz() const { return xyz_; }
-    uint8_t* valid() const { return valid_; }
-    uint8_t* colors() const { return rgb_; }
+    // cuda_point_cloud.cuh:40-53
     size_t get_total_num_points() const { return total_; }
     size_t get_num_points() const { return num_; }
+    size_t get_size() const { return num_ * sizeof(Vertex); }
+    std::array<r64, 9> get_camera_parameters() const { return K_; }
+    void set_total_number_pts() { num_ = total_; dirty_ = true; }
+    // The first get_total_num_points() points as Vertex records (normal 0,
+    // classID 0; an invalid pixel is pos (0,0,0), colour (0,0,0)) in pinned
+    // host memory that the host and the device both read, as the reference's
+    // managed buffer (cuda_point_cloud.cu:170).  Packed again only after
+    // extract_points, clear or set_total_number_pts changed the points;
+    // nullptr on failure (logged).
+    Vertex* get_points() const;
+    Vertex* get_vertex(size_t index) const {
+        Vertex* p = get_points();
+        return p ? p + index : nullptr;
+    }
+    // cuda_point_cloud.cu:142-147: zeroes the points, num_points = 0.
+    void clear();
     // copies the valid points to the host and writes an ASCII PLY
     // (cuda_point_cloud.cu:459-495, including its y/z sign flip)
     bool save_ply(const std::string& path) const;
+
+    // The planes (device memory, get_total_num_points() entries each).
+    float* xyz() const { return xyz_; }
+    uint8_t* valid() const { return valid_; }
+    uint8_t* colors() const { return rgb_; }
 
 private:
     bool extract_raw(const float* depth, size_t step, int rows, int cols, const uint8_t* rgb,
@@ -218,7 +276,11 @@ private:
     uint8_t* valid_ = nullptr;
     uint8_t* rgb_ = nullptr;
     hipStream_t stream_ = nullptr;
+    mutable Vertex* packed_ = nullptr;  // pinned host memory, total_ records
+    mutable bool dirty_ = true;
 };
+// The name this library used for the same class.
+using PointCloud = cudaPointCloud;
 
 // utils.cpp:96-118: ASCII PLY of vertices and triangles.
 bool write_mesh_to_ply(const std::string& file_name, const std::vector<float3>& vertices,

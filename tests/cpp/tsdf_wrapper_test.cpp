@@ -105,6 +105,11 @@ int main() {
     null_depth.cols = cols;
     null_depth.step = cols * 4;
     CHECK(!vol.integrate(no_color, null_depth, poses[0]), "null depth must fail");
+    {
+        cuphoto::PtrStepSzb small_color{(unsigned char*)ddepth, 3 * (size_t)cols, rows - 1, cols};
+        cuphoto::PtrStepSzf d0{ddepth, (size_t)cols * 4, rows, cols};
+        CHECK(!vol.integrate(small_color, d0, poses[0]), "a colour map smaller than the depth must fail");
+    }
 
     const dim3 g = vol.voxel_grid_size();
     const size_t nvox = (size_t)g.x * g.y * g.z;
@@ -178,22 +183,24 @@ int main() {
 
     // depth -> points for frame 2 into slot 1 of a two-frame cloud
     const std::array<double, 9> K = {fx, 0, 0, 0, fy, 0, ccx, ccy, 1};  // column-major
-    cuphoto::PointCloud pc(K, 2 * (size_t)rows * cols);
+    cuphoto::cudaPointCloud::Ptr pc = cuphoto::cudaPointCloud::create(K, 2 * (size_t)rows * cols);
+    CHECK(pc && pc->get_total_num_points() == 2 * (size_t)rows * cols && pc->get_num_points() == 0,
+          "cudaPointCloud::create");
     cuphoto::PtrStepSzf d2;
     d2.data = ddepth + 2 * (size_t)rows * cols;
     d2.step = cols * 4;
     d2.rows = rows;
     d2.cols = cols;
-    CHECK(pc.extract_points(d2, no_color, poses[2], 1), "extract_points");
+    CHECK(pc->extract_points(d2, no_color, poses[2], 1), "extract_points");
     std::vector<float> xyz(3 * (size_t)rows * cols), oxyz(3 * (size_t)rows * cols);
     std::vector<uint8_t> valid((size_t)rows * cols), ovalid((size_t)rows * cols);
-    CHECK(hipMemcpy(xyz.data(), pc.xyz() + 3 * (size_t)rows * cols, xyz.size() * 4, hipMemcpyDeviceToHost) == hipSuccess, "xyz");
-    CHECK(hipMemcpy(valid.data(), pc.valid() + (size_t)rows * cols, valid.size(), hipMemcpyDeviceToHost) == hipSuccess, "valid");
+    CHECK(hipMemcpy(xyz.data(), pc->xyz() + 3 * (size_t)rows * cols, xyz.size() * 4, hipMemcpyDeviceToHost) == hipSuccess, "xyz");
+    CHECK(hipMemcpy(valid.data(), pc->valid() + (size_t)rows * cols, valid.size(), hipMemcpyDeviceToHost) == hipSuccess, "valid");
     oracle_depth_to_points(depth.data() + 2 * (size_t)rows * cols, rows, cols, cols, fx, fy, ccx, ccy,
                            poses[2].data(), nullptr, 0, oxyz.data(), ovalid.data(), nullptr);
     CHECK(std::memcmp(xyz.data(), oxyz.data(), xyz.size() * 4) == 0, "points differ from the oracle");
     CHECK(valid == ovalid, "validity differs");
-    CHECK(!pc.extract_points(d2, no_color, poses[2], 2), "frame 2 does not fit and must fail");
+    CHECK(!pc->extract_points(d2, no_color, poses[2], 2), "frame 2 does not fit and must fail");
 
     // z-buffer render (project_to_color_depth / project_to_depth) of a
     // two-frame cloud: slot 0 gets frame 1 in world coordinates
@@ -207,15 +214,15 @@ int main() {
         for (int i = 0; i < 3; ++i)
             inv[4 + i] = -(R[i] * P[4] + R[3 + i] * P[5] + R[6 + i] * P[6]);  // -R^T t
         cuphoto::PtrStepSzf d1{ddepth + (size_t)rows * cols, (size_t)cols * 4, rows, cols};
-        CHECK(pc.extract_points(d1, no_color, inv, 0), "extract_points slot 0");
+        CHECK(pc->extract_points(d1, no_color, inv, 0), "extract_points slot 0");
     }
-    const size_t np = pc.get_num_points(), n1 = (size_t)rows * cols;
+    const size_t np = pc->get_num_points(), n1 = (size_t)rows * cols;
     CHECK(np == 2 * n1, "points %zu", np);
     std::vector<float> all_xyz(3 * np);
     std::vector<uint8_t> all_valid(np), all_rgb(3 * np);
-    CHECK(hipMemcpy(all_xyz.data(), pc.xyz(), 12 * np, hipMemcpyDeviceToHost) == hipSuccess, "xyz");
-    CHECK(hipMemcpy(all_valid.data(), pc.valid(), np, hipMemcpyDeviceToHost) == hipSuccess, "valid");
-    CHECK(hipMemcpy(all_rgb.data(), pc.colors(), 3 * np, hipMemcpyDeviceToHost) == hipSuccess, "rgb");
+    CHECK(hipMemcpy(all_xyz.data(), pc->xyz(), 12 * np, hipMemcpyDeviceToHost) == hipSuccess, "xyz");
+    CHECK(hipMemcpy(all_valid.data(), pc->valid(), np, hipMemcpyDeviceToHost) == hipSuccess, "valid");
+    CHECK(hipMemcpy(all_rgb.data(), pc->colors(), 3 * np, hipMemcpyDeviceToHost) == hipSuccess, "rgb");
     const size_t dstep = 4 * ((size_t)cols + 7), cstep = 3 * (size_t)cols + 5;
     float* rdepth = nullptr;
     uint8_t* rcolor = nullptr;
@@ -223,7 +230,7 @@ int main() {
     CHECK(hipMalloc((void**)&rcolor, cstep * rows) == hipSuccess, "hipMalloc");
     cuphoto::PtrStepSzf rd{rdepth, dstep, rows, cols};
     cuphoto::PtrStepSzb rc{rcolor, cstep, rows, cols};
-    CHECK(pc.project_to_color_depth(rc, rd, poses[3]), "project_to_color_depth");
+    CHECK(pc->project_to_color_depth(rc, rd, poses[3]), "project_to_color_depth");
     std::vector<float> gdepth(dstep / 4 * rows), odepth(dstep / 4 * rows);
     std::vector<uint8_t> gcolor(cstep * rows), ocolor(cstep * rows);
     CHECK(hipMemcpy(gdepth.data(), rdepth, dstep * rows, hipMemcpyDeviceToHost) == hipSuccess, "dl");
@@ -240,14 +247,30 @@ int main() {
     }
     CHECK(same, "render differs from the oracle");
     CHECK(hits > n1 / 4, "render hits %zu", hits);
-    CHECK(pc.project_to_depth(rd, poses[3]), "project_to_depth");
+    CHECK(pc->project_to_depth(rd, poses[3]), "project_to_depth");
     CHECK(hipMemcpy(gdepth.data(), rdepth, dstep * rows, hipMemcpyDeviceToHost) == hipSuccess, "dl");
     same = true;
     for (int r = 0; r < rows; ++r)
         same &= std::memcmp(&gdepth[r * (dstep / 4)], &odepth[r * (dstep / 4)], 4 * (size_t)cols) == 0;
     CHECK(same, "project_to_depth differs from the oracle");
+    // get_points(): the reference's packed 33-byte records, readable on the host
+    {
+        const cuphoto::cudaPointCloud::Vertex* v = pc->get_points();
+        CHECK(v != nullptr && pc->get_size() == 33 * np, "get_points / get_size");
+        bool vok = v != nullptr;
+        for (size_t i = 0; vok && i < np; ++i) {
+            vok &= std::memcmp(&v[i].pos, &all_xyz[3 * i], 12) == 0;
+            vok &= v[i].color.x == all_rgb[3 * i] && v[i].color.y == all_rgb[3 * i + 1] &&
+                   v[i].color.z == all_rgb[3 * i + 2];
+            vok &= v[i].normal.x == 0.f && v[i].normal.w == 0.f && v[i].classID == 0;
+        }
+        CHECK(vok, "packed vertices differ from the planes");
+        CHECK(pc->get_vertex(n1 + 5) == v + n1 + 5, "get_vertex");
+        const std::array<double, 9> Kc = pc->get_camera_parameters();
+        CHECK(Kc == K, "get_camera_parameters");
+    }
     cuphoto::PtrStepSzf null_rd{nullptr, dstep, rows, cols};
-    CHECK(!pc.project_to_depth(null_rd, poses[3]), "null depth must fail");
+    CHECK(!pc->project_to_depth(null_rd, poses[3]), "null depth must fail");
     (void)hipFree(rdepth);
     (void)hipFree(rcolor);
 
@@ -272,6 +295,8 @@ int main() {
         cuphoto::TSDFSurface surface(scfg);
         surface.set_global_transformation(poses[3]);
         CHECK(surface.reconstruct_surface(pc), "reconstruct_surface(cloud)");
+        CHECK(!surface.reconstruct_surface(cuphoto::cudaPointCloud::Ptr()),
+              "a null cloud must fail");
         const cuphoto::Mesh& mesh = surface.get_mesh();
         // oracle: the contiguous render, one integrate, marching cubes
         std::vector<float> cdepth(n1);

@@ -128,3 +128,30 @@ def test_mesh_capacity_overflow(cuda):
         b.mesh_ranges()
     assert e.value.status == recon.ffi.ERR_CAPACITY
     b.close()
+
+
+def test_batch_full_size_64_volumes(cuda):
+    """The full 64-volume workload (grids 64-256^3, 1-40 frames, three image
+    sizes; about 2.6 GiB of D and W, so arena offsets pass 2^31 bytes): two
+    replays equal the per-volume loop for every volume, and one volume of
+    each grid size equals the oracle."""
+    ws = scenes.batched_uneven()
+    assert len(ws) == 64
+    print("batched 64 volumes: rendering", flush=True)
+    depths = [cuda.from_numpy(w.render()).cuda() for w in ws]
+    total = sum(w.num_voxels for w in ws)
+    assert 8 * total > 2 ** 31
+    eager = _check_batch(cuda, ws, depths, replays=2)
+    print("batched 64 volumes: oracle", flush=True)
+    seen = {}
+    for i, w in enumerate(ws):
+        if w.grid not in seen or w.num_frames < ws[seen[w.grid]].num_frames:
+            seen[w.grid] = i
+    assert len(seen) >= 4
+    for g, i in sorted(seen.items()):
+        o = oracle_run(ws[i].config(), depths[i].cpu().numpy(), ws[i].poses)
+        assert_bitexact(eager[i][1], o["W"], f"volume {i} ({g}) W vs oracle")
+        assert_bitexact(eager[i][0], o["D"], f"volume {i} ({g}) D vs oracle")
+        ov, _, _, _ = O.mc_extract(o["D"], ws[i].grid, list(o["vol"].voxel_size),
+                                   list(o["vol"].offset))
+        assert_bitexact(eager[i][2], ov, f"volume {i} ({g}) mesh vs oracle")

@@ -41,19 +41,18 @@ def _single(torch, w, depth, rt, pts, wts):
     return D, W, loss.item(), gp.cpu().numpy()
 
 
-def test_batch_forward_backward_equal_single_volume(cuda):
-    vols = _fragment_set(cuda)
+def _check_forward_backward(torch, vols, reps):
     b = recon.DiffBatch([(w.config(), d, p, q) for w, d, p, q in vols])
-    rts = [cuda.from_numpy(recon.pose_to_matrix(w.meta["init"])).cuda() for w, _, _, _ in vols]
-    rt_all = cuda.cat(rts).contiguous()
-    print("diff batch forward/backward", flush=True)
-    for rep in range(2):
+    rts = [torch.from_numpy(recon.pose_to_matrix(w.meta["init"])).cuda() for w, _, _, _ in vols]
+    rt_all = torch.cat(rts).contiguous()
+    print(f"diff batch forward/backward, {len(vols)} volumes", flush=True)
+    for rep in range(reps):
         loss = b.forward(rt_all)
         grad = b.backward()
-        cuda.cuda.synchronize()
+        torch.cuda.synchronize()
         f0 = 0
         for i, (w, d, p, q) in enumerate(vols):
-            D, W, L, gp = _single(cuda, w, d, rts[i], p, q)
+            D, W, L, gp = _single(torch, w, d, rts[i], p, q)
             Db, Wb = b.download(i)
             assert_bitexact(Wb, W, f"volume {i} W")
             assert_bitexact(Db, D, f"volume {i} D")
@@ -65,9 +64,21 @@ def test_batch_forward_backward_equal_single_volume(cuda):
     # run-to-run identical gradients
     b.forward(rt_all)
     g2 = b.backward()
-    cuda.cuda.synchronize()
-    assert cuda.equal(g2, grad)
+    torch.cuda.synchronize()
+    assert torch.equal(g2, grad)
     b.close()
+
+
+def test_batch_forward_backward_equal_single_volume(cuda):
+    _check_forward_backward(cuda, _fragment_set(cuda), reps=2)
+
+
+def test_batch_forward_backward_full_size(cuda):
+    """All 32 fragments of the refinement workload (128^3 at 8 mm, 10 frames
+    of 640x480, 20k points each): losses, D, W and gradients equal the
+    single-volume path's."""
+    _check_forward_backward(cuda, _fragment_set(cuda, n=32, grid=128, frames=10, npts=20000,
+                                                ragged=False), reps=1)
 
 
 def test_batch_volume_without_points(cuda):
