@@ -57,6 +57,17 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str | None:
+    """The host CPU's model name (/proc/cpuinfo), None where it is not readable."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, depth, seconds: float) -> dict:
     """Times the CPU restatement (oracle) of the per-frame integrate on the
     host cores over the same workload's frames in order, all of them unless
@@ -76,7 +87,7 @@ def cpu_baseline(w, depth, seconds: float) -> dict:
             break
     dt = time.perf_counter() - t0
     return {"value": round(w.num_voxels * frames / dt / 1e9, 4), "unit": "GVoxel-updates/s",
-            "cores": threads, "host_cpus": os.cpu_count(), "kind": "port",
+            "cores": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(), "kind": "port",
             "sample": f"oracle float32 integrate of the first {frames} frames of the "
                       f"{w.name} workload ({dt:.1f} s, OpenMP over voxel rows)"}
 
@@ -596,20 +607,36 @@ def main() -> int:
     if seg[-1] != F:
         seg.append(F)
 
-    def step_fused():
+    def step_fused(events=None):
         vol.reset(stream)
-        for a, b in zip(seg[:-1], seg[1:]):
+        for k, (a, b) in enumerate(zip(seg[:-1], seg[1:])):
+            if events is not None:
+                events[k][0].record(stream)
             vol.integrate_frames(dev[a:b], w.poses[a:b], stream)
+            if events is not None:
+                events[k][1].record(stream)
             if b - 1 in mesh_at:
                 vol.extract_mesh(verts, tris, stream)
 
+    # voxels each fused sweep (16 frames) reads and writes: the touch counter
+    # outside the timed region
+    touched = torch.zeros(1, dtype=torch.int64, device="cuda")
+    vol.set_touch_counter(touched)
     step_fused()
+    vol.set_touch_counter(None)
     torch.cuda.synchronize()
+    touched_per_step = int(touched.item())
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step_fused()
     torch.cuda.synchronize()
     fused_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    fev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in seg[:-1]] for _ in range(args.steps)]
+    for s_ in range(args.steps):
+        step_fused(fev[s_])
+    torch.cuda.synchronize()
+    fused_integrate_ms = float(sum(e0.elapsed_time(e1) for st in fev for (e0, e1) in st)) / args.steps
 
     swept = w.num_voxels * F
     value = swept / (ms_per_step / 1000.0) / 1e9
@@ -645,6 +672,18 @@ def main() -> int:
         "fused_frames_ms_per_step": round(fused_ms, 3),
         "fused_frames_gvu_s": round(swept / (fused_ms / 1000.0) / 1e9, 3),
     }
+    # fused path (integrate_frames, 16 frames per sweep): algorithmic bytes
+    # 16 B per voxel a sweep updates at least once + 4 B per depth pixel per
+    # frame, over the integrate calls' own time (HIP events)
+    fused_bytes = 16.0 * touched_per_step + 4.0 * pix * F
+    fused_achieved = fused_bytes / (fused_integrate_ms / 1000.0) / 1e9
+    result["roofline_fused"] = {
+        "bound": "hbm", "achieved": round(fused_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(fused_achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "algorithmic_bytes_per_step": int(fused_bytes),
+        "touched_voxels_per_step": touched_per_step,
+        "integrate_ms_per_step": round(fused_integrate_ms, 3),
+        "sweeps_per_step": sum((b - a + 15) // 16 for a, b in zip(seg[:-1], seg[1:]))}
     # meshing of the final volume: the triangle soup (count + emit, the
     # drop-in generate_triangular_surface path) against the welded mesh with
     # normals (count + vertices/normals + indexed triangles); both are

@@ -343,7 +343,7 @@ template <class S, class Frames, bool VEC, bool COUNT, bool COLOR = false, bool 
 __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int nframes,
                                                const Frames& frames, const uint64_t live,
                                                const int x0, const int y0, const int z0,
-                                               unsigned& cnt) {
+                                               unsigned& cnt, unsigned& touched) {
     constexpr int NR = S::kRows, NV = 4 * S::kRows;
     float* __restrict__ D = vp.D;
     float* __restrict__ W = vp.W;
@@ -373,6 +373,7 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
     bool loaded[NR], dirty[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) loaded[r] = dirty[r] = false;
+    unsigned upd = 0;  // COUNT: bit j set once voxel j is updated
 
     // single-frame blocks: a compile-time trip count lets the frame loop
     // unroll (91 -> 59 VGPRs for the plain kernel)
@@ -448,7 +449,10 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
                 w[j] = nw < vp.maxw ? nw : vp.maxw;
             }
             dirty[j >> 2] = true;
-            if (COUNT) ++cnt;
+            if (COUNT) {
+                ++cnt;
+                upd |= 1u << j;
+            }
             if (COLOR && band[j]) {
                 float4* cp = reinterpret_cast<float4*>(vp.C) + base[j >> 2] + (j & 3);
                 float4 c = *cp;
@@ -462,6 +466,7 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
             }
         }
     }
+    if (COUNT) touched += (unsigned)__builtin_popcount(upd);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         if (!dirty[r]) continue;

@@ -155,6 +155,7 @@ struct recon_tsdf {
     int device = 0;
     int num_cus = 256;
     unsigned long long* update_counter = nullptr;
+    unsigned long long* touch_counter = nullptr;
     // per-frame depth bounds, grown on demand
     float* bounds = nullptr;
     size_t bounds_floats = 0;
@@ -181,11 +182,15 @@ int describe(const recon_tsdf_config* cfg, recon_tsdf_info* info);
 // the count words) on first use.
 int ensure_cull_list(recon_tsdf* h, size_t ntiles, size_t entry_bytes);
 
+// Device scratch bytes marching cubes needs for `grid`.
+size_t mc_scratch_need(const int32_t grid[3]);
+
 // Marching cubes over a dense field; `scratch` / `scratch_bytes` may point
-// at a reusable buffer owned by the caller (grown by the callee).
+// at a reusable buffer owned by the caller, grown by the callee when
+// `may_allocate`; otherwise a buffer smaller than mc_scratch_need fails.
 int mc_run(const float* field, const int32_t grid[3], const float vs[3], const float off[3],
            float* verts, int32_t* tris, int64_t capacity, int64_t* total, bool count_only,
-           void** scratch, size_t* scratch_bytes, hipStream_t stream);
+           void** scratch, size_t* scratch_bytes, hipStream_t stream, bool may_allocate = true);
 
 }  // namespace recon
 

@@ -127,9 +127,11 @@ __global__ __launch_bounds__(kThreads) void batch_integrate_kernel(
         tile_origin<S>(bv.vp, tile, x0, y0, z0);
         const FrameTable ft{frames + bv.first_frame};
         if (bv.vp.gx % 4 == 0)
-            integrate_tile<S, FrameTable, true, false>(bv.vp, bv.nframes, ft, mask, x0, y0, z0, cnt);
+            integrate_tile<S, FrameTable, true, false>(bv.vp, bv.nframes, ft, mask, x0, y0, z0, cnt,
+                                                       cnt);
         else
-            integrate_tile<S, FrameTable, false, false>(bv.vp, bv.nframes, ft, mask, x0, y0, z0, cnt);
+            integrate_tile<S, FrameTable, false, false>(bv.vp, bv.nframes, ft, mask, x0, y0, z0, cnt,
+                                                        cnt);
     }
 }
 

@@ -85,13 +85,13 @@ template <int MAXF, bool VEC, bool COUNT, bool COLOR = false, bool VARW = false>
 __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
     const VolumeParams vp, const int nframes, const FrameBlock<MAXF> fb,
     const ListEntry* __restrict__ list, const unsigned* __restrict__ counts, const int nseg,
-    const unsigned seg_len, unsigned long long* counter) {
+    const unsigned seg_len, unsigned long long* counter, unsigned long long* touch_counter) {
     __shared__ unsigned s_pre[kMaxSegments + 1];
     __shared__ unsigned s_wave[kThreads / 64];
     const unsigned n = __builtin_amdgcn_readfirstlane(segment_offsets(counts, nseg, s_pre, s_wave));
     const unsigned waves = gridDim.x * (kThreads / 64);
     const unsigned lane = threadIdx.x & 63;
-    unsigned cnt = 0;
+    unsigned cnt = 0, touched = 0;
     for (unsigned i0 = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i0 < n; i0 += 64 * waves) {
         const unsigned mine = i0 + lane * waves;
         const unsigned slot = mine < n ? segment_slot(s_pre, nseg, seg_len, mine) : 0u;
@@ -105,10 +105,12 @@ __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
             int x0, y0, z0;
             tile_origin<Shape>(vp, tile, x0, y0, z0);
             integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR, VARW>(vp, nframes, fb, frames,
-                                                                             x0, y0, z0, cnt);
+                                                                             x0, y0, z0, cnt,
+                                                                             touched);
         }
     }
-    if (COUNT && cnt) atomicAdd(counter, (unsigned long long)cnt);
+    if (COUNT && counter && cnt) atomicAdd(counter, (unsigned long long)cnt);
+    if (COUNT && touch_counter && touched) atomicAdd(touch_counter, (unsigned long long)touched);
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ D, float* __restrict__ W,
@@ -225,17 +227,18 @@ static int list_occupancy() {
 
 template <int MAXF, bool VEC, bool COUNT, bool COLOR>
 static void launch_list(bool varw, int num_cus, hipStream_t s, const VolumeParams& vp, int nframes,
-                        const FrameBlock<MAXF>& fb, const ListArgs& la, unsigned long long* ctr) {
+                        const FrameBlock<MAXF>& fb, const ListArgs& la, unsigned long long* ctr,
+                        unsigned long long* tctr) {
     if (varw)
         hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, true>),
                            dim3(num_cus * list_occupancy<MAXF, VEC, COUNT, COLOR, true>()),
                            dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
-                           la.seg_len, ctr);
+                           la.seg_len, ctr, tctr);
     else
         hipLaunchKernelGGL((integrate_list_kernel<MAXF, VEC, COUNT, COLOR, false>),
                            dim3(num_cus * list_occupancy<MAXF, VEC, COUNT, COLOR, false>()),
                            dim3(kThreads), 0, s, vp, nframes, fb, la.list, la.counts, la.nseg,
-                           la.seg_len, ctr);
+                           la.seg_len, ctr, tctr);
 }
 
 // The three launches of one integrate sweep over `nframes` frames (frame f at
@@ -279,25 +282,26 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     const bool vec = (vp.gx % 4) == 0;
     const bool varw = h->info.weight_type == RECON_WEIGHT_BY_VARIANCE;
     unsigned long long* ctr = h->update_counter;
+    unsigned long long* tctr = h->touch_counter;
     bool color = false;
     if constexpr (MAXF == 1) color = fb.f[0].rgb && vp.C;  // colour integrate (no counting)
     if (color) {
         if constexpr (MAXF == 1) {
             if (vec)
-                launch_list<1, true, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+                launch_list<1, true, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
             else
-                launch_list<1, false, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+                launch_list<1, false, false, true>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
         }
-    } else if (ctr) {
+    } else if (ctr || tctr) {
         if (vec)
-            launch_list<MAXF, true, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, true, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
         else
-            launch_list<MAXF, false, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, false, true, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
     } else {
         if (vec)
-            launch_list<MAXF, true, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, true, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
         else
-            launch_list<MAXF, false, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr);
+            launch_list<MAXF, false, false, false>(varw, h->num_cus, s, vp, nframes, fb, la, ctr, tctr);
     }
     RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
     return RECON_OK;
@@ -436,6 +440,13 @@ extern "C" int recon_tsdf_upload(recon_tsdf* h, const float* host_d, const float
     if (host_d) RECON_HIP_TRY(hipMemcpyAsync(h->D, host_d, bytes, hipMemcpyHostToDevice, s), "upload D");
     if (host_w) RECON_HIP_TRY(hipMemcpyAsync(h->W, host_w, bytes, hipMemcpyHostToDevice, s), "upload W");
     RECON_HIP_TRY(hipStreamSynchronize(s), "recon_tsdf_upload: synchronize");
+    return RECON_OK;
+}
+
+extern "C" int recon_tsdf_set_touch_counter(recon_tsdf* h, unsigned long long* counter) {
+    clear_error();
+    if (!h) return fail(RECON_ERR_NULL, "recon_tsdf_set_touch_counter: handle is NULL");
+    h->touch_counter = counter;
     return RECON_OK;
 }
 

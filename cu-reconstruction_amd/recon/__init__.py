@@ -264,6 +264,19 @@ class TSDFVolume:
             ptr = counter_tensor.data_ptr()
         check(self._lib.recon_tsdf_set_update_counter(self._h, ptr))
 
+    def set_touch_counter(self, counter_tensor):
+        """Later integrate calls add to element 0 of `counter_tensor` (int64
+        on the GPU) the number of distinct voxels each sweep updates (one
+        sweep per call, or per 16 frames of integrate_frames); None switches
+        it off."""
+        ptr = None
+        if counter_tensor is not None:
+            _device_tensor(counter_tensor, "counter", _torch().int64)
+            if counter_tensor.numel() < 1:
+                raise ValueError("counter must hold at least one element")
+            ptr = counter_tensor.data_ptr()
+        check(self._lib.recon_tsdf_set_touch_counter(self._h, ptr))
+
     def download(self, stream=None):
         D = np.empty(self.shape, np.float32)
         W = np.empty(self.shape, np.float32)

@@ -108,6 +108,7 @@ SIGNATURES = {
     "recon_tsdf_integrate_frames": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _d7,
                                               _vp]),
     "recon_tsdf_set_update_counter": (C.c_int, [_vp, _vp]),
+    "recon_tsdf_set_touch_counter": (C.c_int, [_vp, _vp]),
     "recon_tsdf_count_triangles": (C.c_int, [_vp, _pi64, _vp]),
     "recon_tsdf_extract_mesh": (C.c_int, [_vp, _vp, _vp, _i64, _pi64, _vp]),
     "recon_mc_count": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),

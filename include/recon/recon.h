@@ -166,6 +166,12 @@ RECON_API int recon_tsdf_integrate_frames(recon_tsdf* h, const recon_depth_view*
  * number of voxel updates they perform to it (diagnostic; slower path).
  * Pass NULL to switch counting off. */
 RECON_API int recon_tsdf_set_update_counter(recon_tsdf* h, unsigned long long* counter);
+/* The same for the number of distinct voxels each sweep over the volume
+ * updates at least once: one sweep per recon_tsdf_integrate call, one per
+ * 16 frames of a recon_tsdf_integrate_frames call (the voxels whose D and W
+ * a sweep reads and writes).  No counterpart in the reference; colour
+ * integrates do not count. */
+RECON_API int recon_tsdf_set_touch_counter(recon_tsdf* h, unsigned long long* counter);
 
 /* ---- marching cubes (generate_triangular_surface, marching_cubes.cu) --- */
 /* The mesh is the reference's triangle soup (marching_cubes.cu:159-189):

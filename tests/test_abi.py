@@ -239,3 +239,4 @@ def test_colour_arguments_rejected_before_any_launch():
 def test_transformation_voxels_rejects_null_before_any_launch():
     lib = ffi.load()
     assert lib.recon_tsdf_transformation_voxels(None, C.c_void_p(4096), None) == ffi.ERR_NULL
+

@@ -77,11 +77,12 @@ def _euler_pose():
     return np.array(out[:])
 
 
-def _expected_ply(raw, pose, welded=False, rgb=None):
-    scale = np.float64(np.float32(SCALE))      # read as float, widened
+def _expected_ply(raw, pose, welded=False, rgb=None, grid=GRID, phys=PHYS, offset=OFFSET,
+                  intr=INTR, depth_scale=SCALE):
+    scale = np.float64(np.float32(depth_scale))      # read as float, widened
     depth = (raw.astype(np.float64) * scale).astype(np.float32)
-    cfg = recon.make_config(GRID, physical_size=PHYS, offset=OFFSET, max_weight=15.0,
-                            intrinsics=INTR, image=(640, 480))
+    cfg = recon.make_config(grid, physical_size=phys, offset=offset, max_weight=15.0,
+                            intrinsics=intr, image=(640, 480))
     vol = O.volume_from_config(cfg)
     D, W = O.init(vol)
     if rgb is None:
@@ -91,12 +92,12 @@ def _expected_ply(raw, pose, welded=False, rgb=None):
         O.integrate_color_f32(vol, D, W, Cv, depth, rgb, pose)
     props = ["property float x", "property float y", "property float z"]
     if welded:
-        v, nrm, tri = O.mc_weld(D, GRID, list(vol.voxel_size), list(vol.offset))
+        v, nrm, tri = O.mc_weld(D, grid, list(vol.voxel_size), list(vol.offset))
         props += ["property float nx", "property float ny", "property float nz"]
         rows = ["%g %g %g %g %g %g" % tuple(float(q) for q in (*a, *b)) for a, b in zip(v, nrm)]
         faces = ["3 %d %d %d" % tuple(int(q) for q in f) for f in tri]
     else:
-        verts, _, _, _ = O.mc_extract(D, GRID, list(vol.voxel_size), list(vol.offset))
+        verts, _, _, _ = O.mc_extract(D, grid, list(vol.voxel_size), list(vol.offset))
         v = verts.reshape(-1, 3)
         rows = ["%g %g %g" % (float(a), float(b), float(c)) for a, b, c in v]
         faces = [f"3 {3 * t} {3 * t + 1} {3 * t + 2}" for t in range(v.shape[0] // 3)]
@@ -149,3 +150,80 @@ def test_driver_ply_equals_oracle(cuda, tmp_path, welded, colour):
         k = next((i for i, (x, y) in enumerate(zip(a, b)) if x != y), min(len(a), len(b)))
         pytest.fail(f"PLY differs at line {k}: {a[k:k + 1]} vs {b[k:k + 1]} ({len(a)} vs {len(b)} lines)")
     assert f"{ntri} triangles" in r.stdout
+
+
+# The values of the reference's only parameter file
+# (data/tsdf_surface_parameters.yaml:9-52): a 320 x 240 x 32 grid over a 1 m
+# cube, so voxels are 3.125 x 4.17 x 31.25 mm (10:1 anisotropy), offset
+# (-0.2, 0, 0.6), weight cap 15, fx = fy = 527.01, cx = 320, cy = 240, depth
+# scale 0.0001 and the identity pose (every translation and angle 0).  At the
+# identity pose the sdf's camera z and world z agree, so this pins the
+# reference's literal formula (tsdf_volume.cu:172-173).
+SHIPPED = dict(grid=(320, 240, 32), phys=(1.0, 1.0, 1.0), offset=(-0.2, 0.0, 0.6),
+               intr=(527.01, 527.01, 320.0, 240.0), depth_scale=0.0001)
+SHIPPED_PARAMS = """%YAML:1.0
+
+rgbd.dataset.path: "{data}"
+dataset.color.mode: 0 # 0 - false, 1 - true
+mesh_store.path: "{mesh}"
+
+camera.fx: 527.01
+camera.fy: 527.01
+camera.cx: 320.0
+camera.cy: 240.0
+camera.width: 640
+camera.height: 480
+
+depth_scale: 0.0001
+
+tsdf.volume.voxel_grid_size.x: 320
+tsdf.volume.voxel_grid_size.y: 240
+tsdf.volume.voxel_grid_size.z: 32
+tsdf.volume.physical_size.x: 1.0
+tsdf.volume.physical_size.y: 1.0
+tsdf.volume.physical_size.z: 1.0
+tsdf.volume.max_weight: 15.0
+
+tsdf.volume.global_offset.x: -0.2
+tsdf.volume.global_offset.y: 0
+tsdf.volume.global_offset.z: 0.6
+
+tsdf.volume.camera.pose.translation.x: 0.0
+tsdf.volume.camera.pose.translation.y: 0.0
+tsdf.volume.camera.pose.translation.z: 0.0
+tsdf.volume.camera.pose.euler.roll: 0.0 # degree
+tsdf.volume.camera.pose.euler.yaw: 0.0 # degree
+tsdf.volume.camera.pose.euler.pitch: 0.0 # degree
+"""
+
+
+def test_driver_shipped_parameters(cuda, tmp_path):
+    pose = np.array([1.0, 0, 0, 0, 0, 0, 0])
+    w = scenes.Workload(
+        name="shipped", grid=SHIPPED["grid"], voxel_size=1.0 / 320, offset=SHIPPED["offset"],
+        truncation=0.0, max_weight=15.0, intrinsics=SHIPPED["intr"], image=(640, 480),
+        prims=[(scenes.PRIM_SPHERE, (0.25, 0.3, 1.05, 0.22)),
+               (scenes.PRIM_BOX, (-0.15, 0.5, 1.15, 0.35, 0.9, 1.35)),
+               (scenes.PRIM_BOX, (-2.0, -2.0, 1.42, 2.0, 2.0, 3.0))],
+        poses=pose[None, :], noise=(0.0008, 0.0, 0.01, 0x5EED0006))
+    depth = w.render()[0]
+    raw = np.clip(np.rint(depth / SHIPPED["depth_scale"]), 0, 65535).astype(np.uint16)
+    data = tmp_path / "scene"
+    data.mkdir()
+    Image.fromarray(np.zeros((480, 640, 3), np.uint8), "RGB").save(data / "000_color.png")
+    Image.fromarray(raw).save(data / "000_depth.png")
+    mesh = tmp_path / "mesh.ply"
+    cfg = tmp_path / "params.yaml"
+    cfg.write_text(SHIPPED_PARAMS.format(data=data, mesh=mesh))
+    print("tsdf_surface driver, shipped parameters", flush=True)
+    r = subprocess.run([str(EXE), f"--config_path={cfg}"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    expected, ntri = _expected_ply(raw, pose, **SHIPPED)
+    assert ntri > 20000
+    got = mesh.read_text()
+    if got != expected:
+        a, b = got.splitlines(), expected.splitlines()
+        k = next((i for i, (x, y) in enumerate(zip(a, b)) if x != y), min(len(a), len(b)))
+        pytest.fail(f"PLY differs at line {k}: {a[k:k + 1]} vs {b[k:k + 1]} ({len(a)} vs {len(b)} lines)")

@@ -168,3 +168,42 @@ def test_cull_segment_limits(cuda, gz):
             assert g["counts"].tolist() == [int(c) for c in o["counts"]]
         g["vol"].close()
         del g
+
+
+def test_touch_counter_is_union_per_sweep(cuda):
+    """recon_tsdf_set_touch_counter: per-frame calls count each updated voxel
+    once (= the update count); a fused call counts, per sweep of 16 frames,
+    the voxels that any of its frames updates (the oracle's masks OR-ed)."""
+    import recon
+    import oracle_ffi as O
+
+    w = scenes.tiny(grid=(40, 36, 28), voxel_size=0.025, image=(64, 48), num_frames=21)
+    depth = w.render()
+    cfg = w.config()
+    ov = O.volume_from_config(cfg)
+    D, W = O.init(ov)
+    masks = []
+    for f in range(w.num_frames):
+        n, m = O.integrate_f32(ov, D, W, depth[f], w.poses[f], mask=True)
+        assert int((m != 0).sum()) == n
+        masks.append(m != 0)
+    dev = cuda.from_numpy(depth).cuda()
+    vol = recon.TSDFVolume(cfg)
+    t = cuda.zeros(w.num_frames, dtype=cuda.int64, device="cuda")
+    for f in range(w.num_frames):
+        vol.set_touch_counter(t[f:f + 1])
+        vol.integrate(dev[f], w.poses[f])
+    vol.set_touch_counter(None)
+    vol.reset()
+    u = cuda.zeros(1, dtype=cuda.int64, device="cuda")
+    vol.set_touch_counter(u)
+    vol.integrate_frames(dev, w.poses)
+    vol.set_touch_counter(None)
+    cuda.cuda.synchronize()
+    assert t.cpu().numpy().tolist() == [int(m.sum()) for m in masks]
+    union = sum(int(np.logical_or.reduce(masks[a:a + 16]).sum()) for a in range(0, w.num_frames, 16))
+    assert int(u.item()) == union > 0
+    Dg, Wg = vol.download()
+    assert_bitexact(Wg, W, "fused W")
+    assert_bitexact(Dg, D, "fused D")
+    vol.close()
