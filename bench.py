@@ -363,11 +363,12 @@ def bench_render(w, depth, steps: int, frames: int = 30, cpu: bool = True) -> di
 
 
 def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
-                          frames: int = 10, points: int = 20000) -> dict:
+                          frames: int = 10, points: int = 20000, frame_weight: float = 1.0) -> dict:
     """Batched pose refinement: `fragments` fragments (128^3, 8 mm, 10 frames
     of 640x480 of a distinct seeded box/sphere scene), initial poses perturbed
     by up to 1 degree and 1 cm, `steps` on-device steps (one graph launch per
-    step: forward, loss, adjoint, Adam) over all fragments.  Reports the time
+    step: forward, fused-TSDF surface loss plus the model-to-frame term with
+    weight `frame_weight`, adjoint, Adam) over all fragments.  Reports the time
     per step, pose errors, the loss and, as the quality measure, the RMS distance to the
     analytic surface of the plain integrate's mesh with initial, refined and
     ground-truth poses."""
@@ -380,14 +381,17 @@ def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
     ws = scenes.pose_refinement(num_fragments=fragments, frames=frames, grid=grid,
                                 voxel_size=1.024 / grid)
     spec, depths_np = [], []
+    npairs = 0
     t0 = time.perf_counter()
     for i, w in enumerate(ws):
         depth = w.render(nthreads=min(os.cpu_count() or 1, 16))
-        pts, wts = scenes.refinement_points(w, depth, points, seed=1000 + i)
+        pts, wts, nrm = scenes.refinement_points(w, depth, points, seed=1000 + i, with_normals=True)
+        pairs = scenes.frame_pairs(w, depth, pts, nrm)
+        npairs += len(pairs)
         depths_np.append(depth)
         spec.append((w.config(), torch.from_numpy(depth).cuda(),
                      torch.from_numpy(pts.astype(np.float32)).cuda(),
-                     torch.from_numpy(wts).cuda()))
+                     torch.from_numpy(wts).cuda(), torch.from_numpy(pairs).cuda(), frame_weight))
     log(f"pose refinement: {fragments} fragments prepared in {time.perf_counter() - t0:.1f} s")
     init = np.concatenate([recon.pose_to_matrix(w.meta["init"]) for w in ws])
     gt = np.concatenate([recon.pose_to_matrix(w.poses) for w in ws])
@@ -441,6 +445,7 @@ def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
         return round(float(np.mean(tr)), 3), round(float(np.mean(rot)), 4)
 
     return {"fragments": fragments, "grid": grid, "frames": frames, "points_per_fragment": points,
+            "frame_pairs": int(npairs), "frame_weight": frame_weight,
             "steps": steps, "ms_per_step": round(1000.0 * dt / max(steps, 1), 3),
             "loss_initial": float(h[0].sum()), "loss_final": float(h[-1].sum()),
             "loss_ground_truth": float(loss_gt),

@@ -51,6 +51,10 @@ struct DVol {
     const float* weights;
     int32_t first_frame, nframes;
     int32_t first_loss_block, nloss_blocks;
+    const int32_t* fpairs;   // model-to-frame pairs (frame, point), by frame
+    int64_t nfpairs;
+    double fweight;
+    double ftrunc;           // residuals of |r| >= ftrunc count ftrunc^2 (0: off)
 };
 
 struct DFrame {
@@ -193,6 +197,7 @@ __global__ __launch_bounds__(256) void dbatch_loss_kernel(const DVol* __restrict
 // single-volume path's fixed_sum_kernel, times 1/K.
 __global__ __launch_bounds__(256) void dbatch_loss_sum_kernel(const DVol* __restrict__ vols,
                                                               const double* __restrict__ partials,
+                                                              const double* __restrict__ floss,
                                                               double* __restrict__ loss) {
     __shared__ double s[256];
     const DVol& dv = vols[blockIdx.x];
@@ -205,7 +210,87 @@ __global__ __launch_bounds__(256) void dbatch_loss_sum_kernel(const DVol* __rest
         if ((int)threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss[blockIdx.x] = dv.npoints > 0 ? s[0] * (1.0 / (double)dv.npoints) : 0.0;
+    if (threadIdx.x == 0) {
+        double l = dv.npoints > 0 ? s[0] * (1.0 / (double)dv.npoints) : 0.0;
+        if (floss && dv.nfpairs > 0)
+            for (int f = 0; f < dv.nframes; ++f) l += floss[dv.first_frame + f];
+        loss[blockIdx.x] = l;
+    }
+}
+
+// Model-to-frame term: one workgroup per frame g over its pairs
+// [fr_range[2g], fr_range[2g+1]) of its volume, in float64 (the frame table's
+// float32 R, t widened): the weighted sum of min(r^2, trunc^2) / K into
+// floss[g] (a truncated residual has no gradient) and its
+// gradient w.r.t. the frame's left perturbation into fgrad[6g..6g+5] (the
+// adjoint's formula at the point), each summed in a fixed order.
+__global__ __launch_bounds__(256) void dbatch_frame_loss_kernel(
+    const DVol* __restrict__ vols, const DFrame* __restrict__ fr,
+    const FrameParams* __restrict__ table, const int64_t* __restrict__ fr_range,
+    double* __restrict__ floss, double* __restrict__ fgrad) {
+    __shared__ double s_red[7][4];
+    const int g = blockIdx.x;
+    const DVol& dv = vols[fr[g].volume];
+    const VolumeParams& vp = dv.vp;
+    const FrameParams& L = table[g];
+    const int64_t k0 = fr_range[2 * g], k1 = fr_range[2 * g + 1];
+    const double scale = dv.nfpairs > 0 ? dv.fweight / (double)dv.nfpairs : 0.0;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        const int64_t q = dv.fpairs[2 * k + 1];
+        const double c[3] = {(double)dv.points[3 * q], (double)dv.points[3 * q + 1],
+                             (double)dv.points[3 * q + 2]};
+        double pc[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            pc[r] = (double)L.R[3 * r] * c[0] + (double)L.R[3 * r + 1] * c[1] +
+                    (double)L.R[3 * r + 2] * c[2] + (double)L.t[r];
+        if (!(pc[2] > 0.0)) continue;
+        const double u = pc[0] * (double)vp.fx / pc[2] + (double)vp.cx;
+        const double v = pc[1] * (double)vp.fy / pc[2] + (double)vp.cy;
+        const double fu = floor(u), fv = floor(v);
+        if (!(fu >= 0.0 && fu <= vp.cols - 2 && fv >= 0.0 && fv <= vp.rows - 2)) continue;
+        const int i0 = (int)fu, j0 = (int)fv;
+        const double a = u - fu, b = v - fv;
+        const float* t0 = L.depth + (int64_t)j0 * L.pitch + i0;
+        const double d00 = t0[0], d10 = t0[1], d01 = t0[L.pitch], d11 = t0[L.pitch + 1];
+        const double maxd = vp.maxd;
+        if (!(d00 > 0.0 && d00 <= maxd && d10 > 0.0 && d10 <= maxd && d01 > 0.0 && d01 <= maxd &&
+              d11 > 0.0 && d11 <= maxd))
+            continue;
+        const double d = (1 - a) * (1 - b) * d00 + a * (1 - b) * d10 + (1 - a) * b * d01 + a * b * d11;
+        const double res = d - pc[2];
+        if (dv.ftrunc > 0.0 && !(fabs(res) < dv.ftrunc)) {
+            acc[0] += dv.ftrunc * dv.ftrunc;
+            continue;
+        }
+        acc[0] += res * res;
+        const double gs = 2.0 * res * scale;
+        const double dd_du = (1 - b) * (d10 - d00) + b * (d11 - d01);
+        const double dd_dv = (1 - a) * (d01 - d00) + a * (d11 - d10);
+        const double iz = 1.0 / pc[2];
+        const double gx = gs * dd_du * (double)vp.fx * iz;
+        const double gy = gs * dd_dv * (double)vp.fy * iz;
+        const double gz = -(gx * pc[0] + gy * pc[1]) * iz - gs;
+        acc[1] += gx;
+        acc[2] += gy;
+        acc[3] += gz;
+        acc[4] += pc[1] * gz - pc[2] * gy;
+        acc[5] += pc[2] * gx - pc[0] * gz;
+        acc[6] += pc[0] * gy - pc[1] * gx;
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        const double w = wave_sum_f64(acc[c]);
+        if ((threadIdx.x & 63) == 0) s_red[c][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const int c = threadIdx.x;
+        const double sum = ((s_red[c][0] + s_red[c][1]) + s_red[c][2]) + s_red[c][3];
+        if (c == 0) floss[g] = sum * scale;
+        else fgrad[6 * g + c - 1] = sum;
+    }
 }
 
 // Per volume, the range of the merged list it owns and its work units.
@@ -268,7 +353,7 @@ __global__ __launch_bounds__(256) void dbatch_adjoint_kernel(
                 vx = (int64_t)uid[e] - dv.voxel_base;
                 l = lam_u[e];
             }
-            adjoint_lane_pass(dv.vp, ly, L, vx, l, nullptr, acc);
+            adjoint_lane_pass(dv.vp, ly, L, vx, l, nullptr, 1.0, acc);
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -284,6 +369,7 @@ __global__ __launch_bounds__(256) void dbatch_grad_kernel(const DVol* __restrict
                                                           int total_frames,
                                                           const long long* __restrict__ unit_start,
                                                           const double* __restrict__ partial,
+                                                          const double* __restrict__ fgrad,
                                                           double* __restrict__ grad) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total_frames * 6) return;
@@ -291,7 +377,10 @@ __global__ __launch_bounds__(256) void dbatch_grad_kernel(const DVol* __restrict
     const int v = fr[g].volume;
     const int f = g - vols[v].first_frame;
     double s = 0.0;
-    for (long long u = unit_start[v]; u < unit_start[v + 1]; ++u) s += partial[((size_t)u * 64 + f) * 6 + c];
+    if (unit_start)
+        for (long long u = unit_start[v]; u < unit_start[v + 1]; ++u)
+            s += partial[((size_t)u * 64 + f) * 6 + c];
+    if (fgrad) s += fgrad[i];
     grad[i] = s;
 }
 
@@ -459,6 +548,10 @@ struct recon_diff_batch {
     int64_t* lam_ids = nullptr;
     float* lam = nullptr;
     double* loss = nullptr;  // internal per-volume loss of refinement steps
+    int64_t* fr_range = nullptr;  // model-to-frame pairs of each frame
+    double* floss = nullptr;      // per-frame model-to-frame loss and gradient
+    double* fgrad = nullptr;
+    bool frame_term = false;
     uint32_t* owner = nullptr;
     char* merge_scratch = nullptr;
     MergeLayout M;
@@ -501,17 +594,21 @@ int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t 
     if (loss_blocks > 0)
         hipLaunchKernelGGL(dbatch_loss_kernel, dim3(loss_blocks), dim3(256), 0, s, d_vols,
                            d_loss_wg, loss_partials, lam_ids, lam);
+    if (frame_term)
+        hipLaunchKernelGGL(dbatch_frame_loss_kernel, dim3(total_frames), dim3(256), 0, s, d_vols,
+                           d_frames, table, fr_range, floss, fgrad);
     hipLaunchKernelGGL(dbatch_loss_sum_kernel, dim3(n), dim3(256), 0, s, d_vols, loss_partials,
-                       loss_out);
+                       frame_term ? floss : nullptr, loss_out);
     RECON_HIP_TRY(hipGetLastError(), "diff batch loss launch");
     return RECON_OK;
 }
 
 int recon_diff_batch::backward(double* grad_out, hipStream_t s) {
     if (npairs == 0) {
-        hipLaunchKernelGGL(dbatch_zero_f64_kernel, dim3(1), dim3(256), 0, s, grad_out,
-                           (int64_t)total_frames * 6);
-        RECON_HIP_TRY(hipGetLastError(), "diff batch zero launch");
+        hipLaunchKernelGGL(dbatch_grad_kernel, dim3((total_frames * 6 + 255) / 256), dim3(256), 0,
+                           s, d_vols, d_frames, total_frames, nullptr, nullptr,
+                           frame_term ? fgrad : nullptr, grad_out);
+        RECON_HIP_TRY(hipGetLastError(), "diff batch gradient launch");
         return RECON_OK;
     }
     uint32_t* uid = nullptr;
@@ -525,7 +622,8 @@ int recon_diff_batch::backward(double* grad_out, hipStream_t s) {
     hipLaunchKernelGGL(dbatch_adjoint_kernel, dim3(adjoint_grid), dim3(256), 0, s, d_vols, n, table,
                        uid, lam_u, beg, unit_start, partial);
     hipLaunchKernelGGL(dbatch_grad_kernel, dim3((total_frames * 6 + 255) / 256), dim3(256), 0, s,
-                       d_vols, d_frames, total_frames, unit_start, partial, grad_out);
+                       d_vols, d_frames, total_frames, unit_start, partial,
+                       frame_term ? fgrad : nullptr, grad_out);
     RECON_HIP_TRY(hipGetLastError(), "diff batch backward launch");
     return RECON_OK;
 }
@@ -563,6 +661,10 @@ static int check_dvolume(const recon_diff_volume& v, int i, recon_tsdf_info* inf
         return fail(RECON_ERR_NULL, "recon_diff_batch_create: volume %d points", i);
     if (v.num_points >= (int64_t(1) << 32))
         return fail(RECON_ERR_RANGE, "recon_diff_batch_create: volume %d has too many points", i);
+    if (v.num_frame_pairs < 0 || (v.num_frame_pairs > 0 && !v.frame_pairs))
+        return fail(RECON_ERR_NULL, "recon_diff_batch_create: volume %d frame pairs", i);
+    if (v.num_frame_pairs > 0 && !(v.frame_weight >= 0.0 && v.frame_weight < 1e30))
+        return fail(RECON_ERR_CONFIG, "recon_diff_batch_create: volume %d frame weight", i);
     return RECON_OK;
 }
 
@@ -594,6 +696,7 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
     std::vector<BoundsJob> jobs;
     std::vector<int> fill_start;
     std::vector<uint2> cull_wg, loss_wg;
+    std::vector<int64_t> fr_ranges;
     size_t bounds_floats = 0;
     for (int i = 0; i < n; ++i) {
         const recon_diff_volume& v = vols[i];
@@ -651,6 +754,30 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
         dv.nframes = v.num_frames;
         dv.first_loss_block = loss_blocks;
         dv.nloss_blocks = (int)((v.num_points + 255) / 256);
+        dv.fpairs = v.num_frame_pairs > 0 ? v.frame_pairs : nullptr;
+        dv.nfpairs = v.num_frame_pairs;
+        dv.fweight = v.frame_weight;
+        dv.ftrunc = v.frame_trunc > 0.0 ? v.frame_trunc : 0.0;
+        // the model-to-frame pairs of each frame: checked on the host (frames
+        // grouped in increasing order, points in range), ranges per frame
+        std::vector<int64_t> rng(2 * (size_t)v.num_frames, 0);
+        if (v.num_frame_pairs > 0) {
+            std::vector<int32_t> hp(2 * (size_t)v.num_frame_pairs);
+            RECON_HIP_TRY(hipMemcpy(hp.data(), v.frame_pairs, hp.size() * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost), "frame pairs download");
+            int32_t prev = 0;
+            for (int64_t k = 0; k < v.num_frame_pairs; ++k) {
+                const int32_t f = hp[2 * k], q = hp[2 * k + 1];
+                if (f < prev || f >= v.num_frames || q < 0 || q >= v.num_points)
+                    return fail(RECON_ERR_SIZE, "recon_diff_batch_create: volume %d frame pair %lld "
+                                "(%d, %d) is out of range or out of frame order", i, (long long)k, f, q);
+                if (k == 0 || f != prev) rng[2 * f] = k;
+                rng[2 * f + 1] = k + 1;
+                prev = f;
+            }
+            b->frame_term = true;
+        }
+        fr_ranges.insert(fr_ranges.end(), rng.begin(), rng.end());
         for (int f = 0; f < v.num_frames; ++f) {
             DFrame fr;
             fr.depth = reinterpret_cast<const float*>(reinterpret_cast<const char*>(v.depth) +
@@ -705,6 +832,9 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
     if ((rc = b->alloc(&b->lam_ids, b->npairs, "diff batch pairs"))) return rc;
     if ((rc = b->alloc(&b->lam, b->npairs, "diff batch pairs"))) return rc;
     if ((rc = b->alloc(&b->loss, n, "diff batch loss"))) return rc;
+    if ((rc = b->alloc(&b->fr_range, fr_ranges.size(), "diff batch frame pairs"))) return rc;
+    if ((rc = b->alloc(&b->floss, (size_t)frames, "diff batch frame losses"))) return rc;
+    if ((rc = b->alloc(&b->fgrad, (size_t)frames * 6, "diff batch frame gradients"))) return rc;
     if ((rc = b->alloc(&b->owner, arena, "diff batch owner table"))) return rc;
     b->M = merge_layout(b->npairs);
     if ((rc = b->alloc(&b->merge_scratch, b->M.total, "diff batch merge scratch"))) return rc;
@@ -725,6 +855,8 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
     RECON_HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(BoundsJob),
                                  hipMemcpyHostToDevice, s), "diff batch upload");
     RECON_HIP_TRY(hipMemcpyAsync(b->d_fill_start, fill_start.data(), n * sizeof(int),
+                                 hipMemcpyHostToDevice, s), "diff batch upload");
+    RECON_HIP_TRY(hipMemcpyAsync(b->fr_range, fr_ranges.data(), fr_ranges.size() * sizeof(int64_t),
                                  hipMemcpyHostToDevice, s), "diff batch upload");
     RECON_HIP_TRY(hipMemcpyAsync(b->d_cull_wg, cull_wg.data(), cull_wg.size() * sizeof(uint2),
                                  hipMemcpyHostToDevice, s), "diff batch upload");

@@ -262,11 +262,30 @@ __device__ __forceinline__ LaneLayout lane_layout(int F) {
     return ly;
 }
 
+// Depth gradients are summed exactly: each tap's contribution c is added to
+// an int64 accumulator as llrint(c * scale), scale = 2^(41 - e) for the
+// largest |dL/dD| in [2^e, 2^(e+1)) -- a contribution is at most that
+// value (the running-average factors and bilinear weights are <= 1), so
+// 2^20 contributions to one pixel cannot overflow, and the integer sum does
+// not depend on the order of the atomics.
+__device__ __forceinline__ double depth_grad_scale(unsigned maxbits) {
+    const float m = __uint_as_float(maxbits);
+    return m > 0.0f ? ldexp(1.0, 41 - ilogbf(m)) : 1.0;
+}
+
+__device__ __forceinline__ void depth_grad_add(unsigned long long* o, float c, double scale) {
+    const long long q = llrint((double)c * scale);
+    if (q) atomicAdd(o, (unsigned long long)q);
+}
+
 // One pass of the wave: this lane's segment holds voxel v (local index, or
-// -1) with dL/dD = lam; L is the lane's frame.  Every lane must call.
+// -1) with dL/dD = lam; L is the lane's frame.  gd_frame (may be null): the
+// lane frame's depth-gradient accumulators, one per depth element, scaled
+// by gd_scale.  Every lane must call.
 __device__ __forceinline__ void adjoint_lane_pass(const VolumeParams& vp, const LaneLayout& ly,
                                                   const FrameParams& L, int64_t v, float lam,
-                                                  float* gd_frame, double acc[6]) {
+                                                  unsigned long long* gd_frame, double gd_scale,
+                                                  double acc[6]) {
     const int64_t N = (int64_t)vp.gx * vp.gy * vp.gz;
     const bool act = ly.ok && v >= 0 && v < N && lam != 0.0f && isfinite(lam);
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
@@ -327,11 +346,11 @@ __device__ __forceinline__ void adjoint_lane_pass(const VolumeParams& vp, const 
             acc[4] += (double)s.pz * gx - (double)s.px * gz;
             acc[5] += (double)s.px * gy - (double)s.py * gx;
             if (gd_frame) {
-                float* o = gd_frame + s.addr;
-                unsafeAtomicAdd(o, g * (ia * ib));
-                unsafeAtomicAdd(o + 1, g * (s.a * ib));
-                unsafeAtomicAdd(o + L.pitch, g * (ia * s.b));
-                unsafeAtomicAdd(o + L.pitch + 1, g * (s.a * s.b));
+                unsigned long long* o = gd_frame + s.addr;
+                depth_grad_add(o, g * (ia * ib), gd_scale);
+                depth_grad_add(o + 1, g * (s.a * ib), gd_scale);
+                depth_grad_add(o + L.pitch, g * (ia * s.b), gd_scale);
+                depth_grad_add(o + L.pitch + 1, g * (s.a * s.b), gd_scale);
             }
         }
     }

@@ -35,4 +35,8 @@ __device__ __forceinline__ long long block_exclusive_scan(long long v, long long
 int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
                           hipStream_t s);
 
+// One workgroup: tile_off[i] = sum of tile_sum[0..i) for i <= n (int64;
+// tile_off[n] is the total).  Defined in marching_cubes.hip.
+int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStream_t s);
+
 }  // namespace recon

@@ -14,8 +14,9 @@
 //               persistent integrate over the list: each wave-tile loads D
 //               and W once, applies all of its volume's frames in order and
 //               stores once;
-//   marching cubes over every volume (row counts, one scan across all
-//               volumes, emission), then each volume's triangle range.
+//   marching cubes over every volume (chunk counts per tile of every
+//               volume, one scan across all tiles, emission), then each
+//               volume's triangle range.
 // Every voxel sees the same float32 operations, frame by frame, as the
 // single-volume path, so D, W and meshes are identical to looping over
 // volumes with recon_tsdf_integrate / recon_tsdf_extract_mesh.
@@ -55,7 +56,6 @@ constexpr Variant kVariants[] = {{8, 2, 1}, {16, 2, 1}, {8, 1, 1}, {16, 1, 1},
                                  {8, 2, 2}, {16, 2, 2}, {8, 1, 2}, {16, 1, 2}};
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 constexpr int kFillBlock = 1024;  // voxels per fill workgroup
-constexpr int kMcRowsPerBlock = 4;
 
 // ---- kernels -------------------------------------------------------------------
 __global__ __launch_bounds__(256) void batch_fill_kernel(const BatchVolParams* __restrict__ vols,
@@ -135,86 +135,57 @@ __global__ __launch_bounds__(kThreads) void batch_integrate_kernel(
     }
 }
 
-// One wavefront (64-thread workgroup) per work-table entry of kMcRowsPerBlock
-// rows: when the entry's rows lie in one layer they are streamed together
-// (their 2 * (kMcRowsPerBlock + 1) voxel rows read once, one chunk mask for
-// the entry's rows), else row by row (a chunk mask per row).
-__global__ __launch_bounds__(64) void batch_mc_count_kernel(const McParams* __restrict__ mcs,
-                                                            const int2* __restrict__ wg_rows,
-                                                            int32_t* __restrict__ row_count,
-                                                            uint32_t* __restrict__ row_mask) {
-    const int2 e = wg_rows[blockIdx.x];  // (volume, first local row)
-    const McParams& p = mcs[e.x];
-    const int lane = threadIdx.x;
-    const int y = e.y % p.cyr, z = e.y / p.cyr;
-    int32_t* out = row_count + blockIdx.x * kMcRowsPerBlock;
-    uint32_t* out_mask = row_mask + blockIdx.x * kMcRowsPerBlock;
-    if (y + kMcRowsPerBlock <= p.cyr) {
-        CornerStreamK<kMcRowsPerBlock> cs(p, y, z, lane);
-        int n[kMcRowsPerBlock];
-#pragma unroll
-        for (int i = 0; i < kMcRowsPerBlock; ++i) n[i] = 0;
-        uint32_t m = 0;
-        for (int x0 = 0; x0 < p.cx; x0 += 64) {
-            float v[kMcRowsPerBlock][8];
-            cs.step(x0, lane, v);
-            if (x0 + lane < p.cx) {
-                int any = 0;
-#pragma unroll
-                for (int i = 0; i < kMcRowsPerBlock; ++i) {
-                    const int t = class_ntri(cube_class(v[i]));
-                    n[i] += t;
-                    any |= t;
-                }
-                if (__ballot(any != 0)) m |= chunk_bit(x0);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kMcRowsPerBlock; ++i) {
-            const int t = wave_sum(n[i]);
-            if (lane == 0) {
-                out[i] = t;
-                out_mask[i] = m;
-            }
-        }
-    } else {
-        for (int i = 0; i < kMcRowsPerBlock; ++i) {
-            const int r = e.y + i;
-            uint32_t m = 0;
-            const int t = r < p.rows ? mc_row_triangles(p, r, lane, &m) : 0;
-            if (lane == 0) {
-                out[i] = t;
-                out_mask[i] = m;
-            }
-        }
-    }
+// Marching cubes of every volume: one 256-thread workgroup per tile of a
+// volume (mc_count_tile / mc_emit_tile, the single-volume path's code), the
+// tiles of all volumes in one table and one scan, so a volume's triangles
+// are the range of its first tile's offset to its last tile's end.
+struct McBatchVol {
+    McParams p;
+    McTiling t;
+    long long cc_base;  // first chunk count of the volume
+};
+
+__global__ __launch_bounds__(256) void batch_mc_count_kernel(const McBatchVol* __restrict__ vols,
+                                                             const int2* __restrict__ tiles,
+                                                             uint16_t* __restrict__ cc,
+                                                             int32_t* __restrict__ tile_sum) {
+    __shared__ uint8_t ntab[256];
+    __shared__ int wtot[4];
+    ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
+    __syncthreads();
+    const int2 e = tiles[blockIdx.x];  // (volume, tile of the volume)
+    const McBatchVol& v = vols[e.x];
+    mc_count_tile(v.p, v.t, e.y, cc + v.cc_base, ntab, wtot, tile_sum + blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McParams* __restrict__ mcs,
-                                                            const int2* __restrict__ wg_rows,
-                                                            const int32_t* __restrict__ row_count,
-                                                            const uint32_t* __restrict__ row_mask,
-                                                            const long long* __restrict__ row_offset,
+__global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McBatchVol* __restrict__ vols,
+                                                            const int2* __restrict__ tiles,
+                                                            const uint16_t* __restrict__ cc,
+                                                            const long long* __restrict__ tile_off,
                                                             float* __restrict__ verts,
                                                             long long capacity) {
-    const int2 e = wg_rows[blockIdx.x];
-    const McParams& p = mcs[e.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r = e.y + w;
-    const int g = blockIdx.x * kMcRowsPerBlock + w;
-    if (r >= p.rows || row_count[g] == 0) return;
-    mc_emit_row_masked(p, r, lane, row_offset[g], row_mask[g], verts, nullptr, capacity);
+    __shared__ uint8_t ntab[256];
+    __shared__ int lchunk[256];
+    __shared__ int loff[256];
+    __shared__ int wtot[4];
+    const long long base = tile_off[blockIdx.x];
+    if (tile_off[blockIdx.x + 1] == base) return;  // the same for every thread
+    ntab[threadIdx.x] = (uint8_t)class_ntri((int)threadIdx.x);
+    const int2 e = tiles[blockIdx.x];
+    const McBatchVol& v = vols[e.x];
+    mc_emit_tile(v.p, v.t, e.y, cc + v.cc_base, base, ntab, lchunk, loff, wtot, verts, nullptr,
+                 capacity);
 }
 
-// ranges[2v], ranges[2v+1] = first triangle and triangle count of volume v.
-__global__ void batch_mesh_ranges_kernel(const int* __restrict__ first_row, int nvol, int total_rows,
-                                         const long long* __restrict__ row_offset,
-                                         const long long* __restrict__ total,
+// ranges[2v], ranges[2v+1] = first triangle and triangle count of volume v
+// (its tiles are first_tile[v] .. first_tile[v+1]-1).
+__global__ void batch_mesh_ranges_kernel(const int* __restrict__ first_tile, int nvol,
+                                         const long long* __restrict__ tile_off,
                                          long long* __restrict__ ranges) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvol) return;
-    const long long a = first_row[v] < total_rows ? row_offset[first_row[v]] : *total;
-    const long long b = first_row[v + 1] < total_rows ? row_offset[first_row[v + 1]] : *total;
+    const long long a = tile_off[first_tile[v]];
+    const long long b = tile_off[first_tile[v + 1]];
     ranges[2 * v] = a;
     ranges[2 * v + 1] = b - a;
 }
@@ -334,15 +305,13 @@ struct recon_batch {
     };
     std::vector<Group> groups;
     unsigned* d_counts = nullptr;
-    McParams* d_mc = nullptr;
-    int2* d_mc_wg = nullptr;
-    int mc_blocks = 0, total_rows = 0;
-    int32_t* row_count = nullptr;
-    uint32_t* row_mask = nullptr;
-    long long* row_offset = nullptr;
-    long long* sums = nullptr;
-    int nb = 0;
-    int* d_first_row = nullptr;
+    McBatchVol* d_mc = nullptr;
+    int2* d_mc_tiles = nullptr;
+    int mc_tiles = 0;
+    uint16_t* mc_cc = nullptr;
+    int32_t* tile_sum = nullptr;
+    long long* tile_off = nullptr;
+    int* d_first_tile = nullptr;
     long long* d_ranges = nullptr;
     float* verts = nullptr;
     long long capacity = 0;
@@ -389,17 +358,21 @@ int recon_batch::record(hipStream_t s) {
             if (rc) return rc;
         }
     }
-    if (with_mesh && mc_blocks > 0) {
-        hipLaunchKernelGGL(batch_mc_count_kernel, dim3(mc_blocks), dim3(64), 0, s, d_mc, d_mc_wg,
-                           row_count, row_mask);
-        RECON_HIP_TRY(hipGetLastError(), "batch_mc_count_kernel launch");
-        int rc = launch_exclusive_scan(row_count, total_rows, row_offset, sums, s);
+    if (with_mesh) {
+        if (mc_tiles > 0) {
+            hipLaunchKernelGGL(batch_mc_count_kernel, dim3(mc_tiles), dim3(256), 0, s, d_mc,
+                               d_mc_tiles, mc_cc, tile_sum);
+            RECON_HIP_TRY(hipGetLastError(), "batch_mc_count_kernel launch");
+        }
+        int rc = launch_tile_scan(tile_sum, mc_tiles, tile_off, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(batch_mc_emit_kernel, dim3(mc_blocks), dim3(256), 0, s, d_mc, d_mc_wg,
-                           row_count, row_mask, row_offset, verts, capacity);
-        RECON_HIP_TRY(hipGetLastError(), "batch_mc_emit_kernel launch");
-        hipLaunchKernelGGL(batch_mesh_ranges_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_first_row,
-                           n, total_rows, row_offset, sums + nb, d_ranges);
+        if (mc_tiles > 0) {
+            hipLaunchKernelGGL(batch_mc_emit_kernel, dim3(mc_tiles), dim3(256), 0, s, d_mc,
+                               d_mc_tiles, mc_cc, tile_off, verts, capacity);
+            RECON_HIP_TRY(hipGetLastError(), "batch_mc_emit_kernel launch");
+        }
+        hipLaunchKernelGGL(batch_mesh_ranges_kernel, dim3((n + 63) / 64), dim3(64), 0, s,
+                           d_first_tile, n, tile_off, d_ranges);
         RECON_HIP_TRY(hipGetLastError(), "batch_mesh_ranges_kernel launch");
     }
     return RECON_OK;
@@ -573,14 +546,15 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
     }
     if ((rc = b->alloc(&b->d_counts, std::max<size_t>(b->groups.size(), 1), "batch list counters")))
         return fail_out(rc);
-    // marching cubes tables: each volume's cube rows padded to whole blocks
+    // marching cubes tables: every volume's tiles, its chunk counts' place
     if (with_mesh) {
-        std::vector<McParams> mcs(n);
-        std::vector<int2> mc_wg;
-        std::vector<int> first_row(n + 1, 0);
+        std::vector<McBatchVol> mcs(n);
+        std::vector<int2> tiles;
+        std::vector<int> first_tile(n + 1, 0);
+        long long cc_total = 0;
         for (int i = 0; i < n; ++i) {
             const VolumeParams& vp = b->h_vols[i].vp;
-            McParams& p = mcs[i];
+            McParams& p = mcs[i].p;
             p.f = vp.D;
             p.gx = vp.gx;
             p.gy = vp.gy;
@@ -592,21 +566,23 @@ static int plan_batch(const recon_batch_volume* vols, int n, const std::vector<i
                 p.vs[k] = vp.vs[k];
                 p.off[k] = vp.off[k];
             }
-            first_row[i] = (int)mc_wg.size() * kMcRowsPerBlock;
-            for (int r = 0; r < p.rows; r += kMcRowsPerBlock) mc_wg.push_back(make_int2(i, r));
+            mcs[i].t = mc_tiling(vp.gx, vp.gy, vp.gz);
+            mcs[i].cc_base = cc_total;
+            cc_total += (long long)p.rows * mcs[i].t.nch;
+            first_tile[i] = (int)tiles.size();
+            for (int t = 0; t < mcs[i].t.tiles; ++t) tiles.push_back(make_int2(i, t));
         }
-        b->mc_blocks = (int)mc_wg.size();
-        b->total_rows = b->mc_blocks * kMcRowsPerBlock;
-        first_row[n] = b->total_rows;
-        b->nb = (b->total_rows + kScanBlock - 1) / kScanBlock;
+        first_tile[n] = (int)tiles.size();
+        if (tiles.size() >= (size_t(1) << 31) - 1)
+            return fail_out(fail(RECON_ERR_RANGE, "recon_batch_create: too many marching-cubes tiles"));
+        b->mc_tiles = (int)tiles.size();
         b->capacity = capacity;
         if ((rc = b->upload(&b->d_mc, mcs, "batch mc table")) ||
-            (rc = b->upload(&b->d_mc_wg, mc_wg, "batch mc work table")) ||
-            (rc = b->upload(&b->d_first_row, first_row, "batch mc first rows")) ||
-            (rc = b->alloc(&b->row_count, b->total_rows, "batch row counts")) ||
-            (rc = b->alloc(&b->row_mask, b->total_rows, "batch row chunk masks")) ||
-            (rc = b->alloc(&b->row_offset, b->total_rows, "batch row offsets")) ||
-            (rc = b->alloc(&b->sums, b->nb + 1, "batch scan sums")) ||
+            (rc = b->upload(&b->d_mc_tiles, tiles, "batch mc tiles")) ||
+            (rc = b->upload(&b->d_first_tile, first_tile, "batch mc first tiles")) ||
+            (rc = b->alloc(&b->mc_cc, (size_t)cc_total, "batch chunk counts")) ||
+            (rc = b->alloc(&b->tile_sum, (size_t)b->mc_tiles, "batch tile sums")) ||
+            (rc = b->alloc(&b->tile_off, (size_t)b->mc_tiles + 1, "batch tile offsets")) ||
             (rc = b->alloc(&b->d_ranges, 2 * (size_t)n, "batch mesh ranges")) ||
             (rc = b->alloc(&b->verts, 9 * (size_t)std::max<int64_t>(capacity, 1), "batch mesh")))
             return fail_out(rc);

@@ -179,13 +179,15 @@ template <class IdT>
 __global__ __launch_bounds__(256) void diff_adjoint_kernel(
     const VolumeParams vp, const int nframes, const FrameTable ft, const IdT* __restrict__ ids,
     const float* __restrict__ lam_in, const int64_t n_host, const long long* __restrict__ n_dev,
-    double* __restrict__ partial, float* __restrict__ gdepth, const size_t gd_frame_stride) {
+    double* __restrict__ partial, unsigned long long* __restrict__ gdepth,
+    const size_t gd_frame_stride, const unsigned* __restrict__ gd_maxbits) {
     const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const unsigned waves = gridDim.x * 4u;
     const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
     const LaneLayout ly = lane_layout(nframes);
     const FrameParams L = ft.f[ly.ok ? ly.f : 0];  // kept in registers
-    float* gd_frame = gdepth ? gdepth + gd_frame_stride * (ly.ok ? ly.f : 0) : nullptr;
+    unsigned long long* gd_frame = gdepth ? gdepth + gd_frame_stride * (ly.ok ? ly.f : 0) : nullptr;
+    const double gd_scale = gdepth ? depth_grad_scale(*gd_maxbits) : 1.0;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t e0 = (int64_t)wave * ly.S; e0 < n; e0 += (int64_t)waves * ly.S) {
         const int64_t e = e0 + ly.seg;
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(256) void diff_adjoint_kernel(
             v = (int64_t)ids[e];
             lam = lam_in[e];
         }
-        adjoint_lane_pass(vp, ly, L, v, lam, gd_frame, acc);
+        adjoint_lane_pass(vp, ly, L, v, lam, gd_frame, gd_scale, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -212,9 +214,11 @@ template <class IdT>
 __global__ __launch_bounds__(256) void diff_adjoint_long_kernel(
     const VolumeParams vp, const int nframes, const FrameTable ft, const IdT* __restrict__ ids,
     const float* __restrict__ lam_in, const int64_t n_host, const long long* __restrict__ n_dev,
-    double* __restrict__ partial, float* __restrict__ gdepth, const size_t gd_frame_stride) {
+    double* __restrict__ partial, unsigned long long* __restrict__ gdepth,
+    const size_t gd_frame_stride, const unsigned* __restrict__ gd_maxbits) {
     const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int lane = threadIdx.x & 63;
+    const double gd_scale = gdepth ? depth_grad_scale(*gd_maxbits) : 1.0;
     const unsigned waves = gridDim.x * 4u;
     const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
     const int64_t N = (int64_t)vp.gx * vp.gy * vp.gz;
@@ -282,11 +286,11 @@ __global__ __launch_bounds__(256) void diff_adjoint_long_kernel(
                             c[4] = (double)s.pz * gx - (double)s.px * gz;
                             c[5] = (double)s.px * gy - (double)s.py * gx;
                             if (gdepth) {
-                                float* o = gdepth + gd_frame_stride * f + s.addr;
-                                unsafeAtomicAdd(o, g * (ia * ib));
-                                unsafeAtomicAdd(o + 1, g * (s.a * ib));
-                                unsafeAtomicAdd(o + F.pitch, g * (ia * s.b));
-                                unsafeAtomicAdd(o + F.pitch + 1, g * (s.a * s.b));
+                                unsigned long long* o = gdepth + gd_frame_stride * f + s.addr;
+                                depth_grad_add(o, g * (ia * ib), gd_scale);
+                                depth_grad_add(o + 1, g * (s.a * ib), gd_scale);
+                                depth_grad_add(o + F.pitch, g * (ia * s.b), gd_scale);
+                                depth_grad_add(o + F.pitch + 1, g * (s.a * s.b), gd_scale);
                             }
                         }
                         lc = lc * a;
@@ -423,6 +427,42 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p,
                                                        uint32_t value) {
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         p[i] = value;
+}
+
+// Block maxima of |lam| over n values (float bits, which order like the
+// magnitudes), for the depth-gradient scale; n may live on the device.
+__global__ __launch_bounds__(256) void lambda_max_kernel(const float* __restrict__ lam,
+                                                         const int64_t n_host,
+                                                         const long long* __restrict__ n_dev,
+                                                         unsigned* __restrict__ block_max) {
+    __shared__ unsigned s_max[4];
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    unsigned m = 0;
+    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const float l = lam[e];
+        if (isfinite(l)) m = max(m, __float_as_uint(fabsf(l)));
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, d, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) block_max[blockIdx.x] = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
+__global__ __launch_bounds__(256) void zero_u64_kernel(unsigned long long* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) p[i] = 0ull;
+}
+
+// gd (frame y at y * frame_stride floats) <- the accumulators (frame y at
+// y * per_frame) divided by their scale.
+__global__ __launch_bounds__(256) void depth_grad_finish_kernel(
+    const unsigned long long* __restrict__ acc, size_t per_frame, float* __restrict__ gd,
+    size_t frame_stride, const unsigned* __restrict__ maxbits) {
+    const double inv = 1.0 / depth_grad_scale(*maxbits);
+    const unsigned long long* a = acc + per_frame * blockIdx.y;
+    float* out = gd + frame_stride * blockIdx.y;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < per_frame; i += gridDim.x * 256ull)
+        out[i] = (float)((double)(long long)a[i] * inv);
 }
 
 __global__ __launch_bounds__(256) void zero_frames_kernel(float* __restrict__ p, size_t frame_stride,
@@ -740,18 +780,31 @@ extern "C" int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* de
     const int64_t N = h->info.num_voxels;
     const bool merge = num_lambda > 0 && N < 0xFFFFFFFFll && num_lambda < 0x7FFFFFFFll;
     const MergeLayout M = merge_layout(num_lambda);
-    if ((rc = ensure_diff_scratch(h, L.total + (merge ? M.total : 0), s))) return rc;
+    // depth gradients: int64 accumulators (one per depth element of every
+    // frame), the block maxima of |dL/dD| and their maximum
+    const size_t per_frame = (depth0->step / 4) * (size_t)depth0->rows;
+    const int nmax = (int)std::min<int64_t>(std::max<int64_t>((num_lambda + 255) / 256, 1), 1024);
+    const size_t gd_base = L.total + (merge ? M.total : 0);
+    const size_t gd_blocks = gd_base + align256(8 * per_frame * (size_t)num_frames);
+    const size_t gd_maxbits = gd_blocks + align256(4 * (size_t)nmax);
+    const size_t need = grad_depth ? gd_maxbits + 256 : gd_base;
+    if ((rc = ensure_diff_scratch(h, need, s))) return rc;
     if (merge && (rc = ensure_owner(h, s))) return rc;
     char* base = static_cast<char*>(h->diff_scratch);
     launch_setup(vp, depth0, frame_stride_bytes, num_frames, poses, base, L, s);
     RECON_HIP_TRY(hipGetLastError(), "diff_setup_kernel launch");
     const size_t gstride = num_frames > 1 ? frame_stride_bytes / 4 : 0;
+    unsigned long long* gacc = nullptr;
+    unsigned* gmax_blocks = nullptr;
+    unsigned* gmax = nullptr;
     if (grad_depth) {
-        const size_t per_frame = (depth0->step / 4) * (size_t)depth0->rows;
-        const unsigned gx = (unsigned)std::min<size_t>((per_frame + 255) / 256, 1024);
-        hipLaunchKernelGGL(zero_frames_kernel, dim3(gx, num_frames), dim3(256), 0, s, grad_depth,
-                           gstride, per_frame);
-        RECON_HIP_TRY(hipGetLastError(), "zero_frames_kernel launch");
+        gacc = reinterpret_cast<unsigned long long*>(base + gd_base);
+        gmax_blocks = reinterpret_cast<unsigned*>(base + gd_blocks);
+        gmax = reinterpret_cast<unsigned*>(base + gd_maxbits);
+        const size_t n = per_frame * (size_t)num_frames;
+        const unsigned gx = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(gx), dim3(256), 0, s, gacc, n);
+        RECON_HIP_TRY(hipGetLastError(), "zero_u64_kernel launch");
     }
     double* partial = reinterpret_cast<double*>(base + L.partial);
     const FrameTable ft{reinterpret_cast<const FrameParams*>(base + L.table)};
@@ -766,24 +819,42 @@ extern "C" int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* de
                                &lam_u, &count, s)))
             return rc;
         h->owner_dirty = false;
+        if (grad_depth) {
+            hipLaunchKernelGGL(lambda_max_kernel, dim3(nmax), dim3(256), 0, s, lam_u, num_lambda,
+                               count, gmax_blocks);
+            hipLaunchKernelGGL(merge_max_kernel, dim3(1), dim3(256), 0, s, gmax_blocks, nmax, gmax);
+            RECON_HIP_TRY(hipGetLastError(), "lambda_max_kernel launch");
+        }
         if (num_frames <= 64)
             hipLaunchKernelGGL(diff_adjoint_kernel<uint32_t>, grid, dim3(256), 0, s, vp, num_frames,
-                               ft, uid, lam_u, num_lambda, count, partial, grad_depth, gstride);
+                               ft, uid, lam_u, num_lambda, count, partial, gacc, per_frame, gmax);
         else
             hipLaunchKernelGGL(diff_adjoint_long_kernel<uint32_t>, grid, dim3(256), 0, s, vp,
-                               num_frames, ft, uid, lam_u, num_lambda, count, partial, grad_depth,
-                               gstride);
+                               num_frames, ft, uid, lam_u, num_lambda, count, partial, gacc,
+                               per_frame, gmax);
     } else {
         const long long* none = nullptr;
+        if (grad_depth) {
+            hipLaunchKernelGGL(lambda_max_kernel, dim3(nmax), dim3(256), 0, s, lambda, num_lambda,
+                               none, gmax_blocks);
+            hipLaunchKernelGGL(merge_max_kernel, dim3(1), dim3(256), 0, s, gmax_blocks, nmax, gmax);
+            RECON_HIP_TRY(hipGetLastError(), "lambda_max_kernel launch");
+        }
         if (num_frames <= 64)
             hipLaunchKernelGGL(diff_adjoint_kernel<int64_t>, grid, dim3(256), 0, s, vp, num_frames,
-                               ft, lambda_ids, lambda, num_lambda, none, partial, grad_depth,
-                               gstride);
+                               ft, lambda_ids, lambda, num_lambda, none, partial, gacc, per_frame,
+                               gmax);
         else
             hipLaunchKernelGGL(diff_adjoint_long_kernel<int64_t>, grid, dim3(256), 0, s, vp,
-                               num_frames, ft, lambda_ids, lambda, num_lambda, none, partial,
-                               grad_depth, gstride);
+                               num_frames, ft, lambda_ids, lambda, num_lambda, none, partial, gacc,
+                               per_frame, gmax);
     }
     RECON_HIP_TRY(hipGetLastError(), "diff_adjoint_kernel launch");
+    if (grad_depth) {
+        const unsigned gx = (unsigned)std::min<size_t>((per_frame + 255) / 256, 1024);
+        hipLaunchKernelGGL(depth_grad_finish_kernel, dim3(gx, num_frames), dim3(256), 0, s, gacc,
+                           per_frame, grad_depth, gstride, gmax);
+        RECON_HIP_TRY(hipGetLastError(), "depth_grad_finish_kernel launch");
+    }
     return launch_fixed_sum(partial, waves, grad_pose, 6 * num_frames, 1.0, s);
 }

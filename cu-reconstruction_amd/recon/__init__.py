@@ -686,8 +686,11 @@ class Batch:
 class DiffBatch:
     """Pose refinement over many fragments (recon_diff_batch_*).  `volumes` is
     a list of (config, depth tensor (F, H, W) on the device, points (K, 3)
-    float32 device tensor, weights (K,) float32 device tensor or None); the
-    tensors are kept alive here because every step reads them."""
+    float32 device tensor, weights (K,) float32 device tensor or None), each
+    optionally followed by model-to-frame pairs ((P, 2) int32 device tensor
+    of (frame, point), grouped by frame), their weight in the loss and the
+    residual's truncation (metres; 0: none); the tensors are kept alive here
+    because every step reads them."""
 
     def __init__(self, volumes):
         torch = _torch()
@@ -695,7 +698,19 @@ class DiffBatch:
         self._keep = []
         arr = (ffi.DiffVolume * len(volumes))()
         frames = 0
-        for i, (cfg, depth, points, weights) in enumerate(volumes):
+        for i, spec in enumerate(volumes):
+            cfg, depth, points, weights = spec[:4]
+            pairs, pair_weight = (spec[4], float(spec[5])) if len(spec) > 4 else (None, 0.0)
+            pair_trunc = float(spec[6]) if len(spec) > 6 else 0.0
+            if pairs is not None:
+                _device_tensor(pairs, "frame pairs", torch.int32)
+                if pairs.dim() != 2 or pairs.shape[1] != 2 or not pairs.is_contiguous():
+                    raise ValueError("frame pairs must be a contiguous (P, 2) int32 tensor")
+                self._keep.append(pairs)
+                arr[i].frame_pairs = pairs.data_ptr()
+                arr[i].num_frame_pairs = int(pairs.shape[0])
+                arr[i].frame_weight = pair_weight
+                arr[i].frame_trunc = pair_trunc
             _frames_view(depth, "depth")
             _device_tensor(points, "points", torch.float32)
             if not points.is_contiguous() or points.dim() != 2 or points.shape[1] != 3:

@@ -71,7 +71,9 @@ class DiffVolume(C.Structure):
     _fields_ = [("config", TsdfConfig), ("num_frames", C.c_int32), ("rows", C.c_int32),
                 ("cols", C.c_int32), ("depth_step", C.c_size_t), ("frame_stride", C.c_size_t),
                 ("depth", C.c_void_p), ("points", C.c_void_p), ("weights", C.c_void_p),
-                ("num_points", C.c_int64)]
+                ("num_points", C.c_int64), ("frame_pairs", C.c_void_p),
+                ("num_frame_pairs", C.c_int64), ("frame_weight", C.c_double),
+                ("frame_trunc", C.c_double)]
 
 
 class AdamParams(C.Structure):

@@ -513,17 +513,53 @@ def visible_points(w: Workload, depth: np.ndarray, poses, points: np.ndarray, to
     return (seen >= min_frames).astype(np.float32)
 
 
-def refinement_points(w: Workload, depth: np.ndarray, n: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+def refinement_points(w: Workload, depth: np.ndarray, n: int, seed: int,
+                      with_normals: bool = False):
     """Surface points for pose refinement and their float32 weights: n points
     uniform on the scene's surfaces, weight 1 where at least 3 frames see the
     point at its initial pose (meta["init"]) away from silhouettes (5x5
     depth window within 3 cm) and not at grazing angles (|n . view| >= 0.4);
     elsewhere the fused TSDF is not locally linear and its residuals would
-    pull the poses."""
+    pull the poses.  With `with_normals`, also the points' normals."""
     pts, nrm = surface_points(w, n, seed=seed, with_normals=True)
     wts = visible_points(w, depth, w.meta.get("init", w.poses), pts, tol=0.03, min_frames=3,
                          normals=nrm, min_cos=0.4, window=2)
-    return pts, wts
+    return (pts, wts, nrm) if with_normals else (pts, wts)
+
+
+def frame_pairs(w: Workload, depth: np.ndarray, points: np.ndarray, normals: np.ndarray,
+                poses=None, tol: float = 0.03, window: int = 2, min_cos: float = 0.4) -> np.ndarray:
+    """(P, 2) int32 (frame, point) pairs, grouped by frame, for the
+    model-to-frame term of pose refinement: frame f sees point p at pose
+    poses[f] (default meta["init"]) away from silhouettes (every depth of the
+    (2 window + 1)^2 pixels around the projection valid and within `tol` of
+    the point's camera z) and not at a grazing angle (|n . view| >= min_cos)."""
+    fx, fy, cx, cy = w.intrinsics
+    cols, rows = w.image
+    poses = np.asarray(w.meta.get("init", w.poses) if poses is None else poses)
+    P = np.asarray(points, np.float64)
+    out = []
+    for f, pose in enumerate(poses):
+        R, t = pose_matrix(pose)
+        pc = P @ R.T + t
+        z = pc[:, 2]
+        with np.errstate(all="ignore"):
+            u = np.floor(pc[:, 0] * fx / z + cx + 0.5).astype(np.int64)
+            v = np.floor(pc[:, 1] * fy / z + cy + 0.5).astype(np.int64)
+        r = window
+        ok = (z > 0) & (u >= r) & (u < cols - r) & (v >= r) & (v < rows - r)
+        good = ok.copy()
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                d = np.zeros(len(P))
+                d[ok] = depth[f][v[ok] + dy, u[ok] + dx]
+                good &= (d > 0) & (np.abs(d - z) < tol)
+        view = pc / np.maximum(np.linalg.norm(pc, axis=1, keepdims=True), 1e-12)
+        nc = np.asarray(normals, np.float64) @ R.T
+        good &= np.abs(np.sum(nc * view, axis=1)) >= min_cos
+        idx = np.nonzero(good)[0]
+        out.append(np.stack([np.full(len(idx), f), idx], axis=1))
+    return np.ascontiguousarray(np.concatenate(out).astype(np.int32))
 
 
 def mesh_rms(w: Workload, verts: np.ndarray) -> float:

@@ -371,7 +371,9 @@ RECON_API int recon_tsdf_surface_loss(recon_tsdf* h, const float* points, const 
  * rho + omega x p); grad_depth (device, may be NULL; the depth frames'
  * layout) <- dL/ddepth.  The +trunc clamp and the skip below -trunc are
  * taken as constant (their subgradient 0).  Pose sums are reduced in a fixed
- * order (run-to-run identical); depth sums use float atomics. */
+ * order; depth sums are exact int64 fixed-point sums (each contribution
+ * rounded to 2^-41 of the largest |dL/dD|, then added in any order): both
+ * are run-to-run identical. */
 RECON_API int recon_tsdf_diff_adjoint(recon_tsdf* h, const recon_depth_view* depth0,
                                       size_t frame_stride_bytes, int32_t num_frames,
                                       const float* poses_rt, const int64_t* lambda_ids,
@@ -396,6 +398,20 @@ typedef struct recon_diff_volume {
     const float* points;       /* device: 3 floats per surface point (world) */
     const float* weights;      /* device: one per point, or NULL (all 1) */
     int64_t num_points;
+    /* Model-to-frame term, optional (num_frame_pairs 0): pair k = (frame
+     * frame_pairs[2k], point frame_pairs[2k+1]) scores that frame's depth
+     * against that point, r = d(s) - z: d(s) the bilinear depth at the
+     * point's projection s (the diff_integrate sampling rule), z its camera
+     * z; r = 0 when the sample is invalid.  The volume's loss adds
+     * frame_weight * sum_k min(r_k^2, frame_trunc^2) / num_frame_pairs
+     * (frame_trunc <= 0: not truncated; a truncated pair has no gradient),
+     * which observes every frame's own pose (the fused TSDF averages the
+     * frames).  Device int32 pairs, grouped by frame in increasing frame
+     * order. */
+    const int32_t* frame_pairs;
+    int64_t num_frame_pairs;
+    double frame_weight;
+    double frame_trunc;
 } recon_diff_volume;
 
 typedef struct recon_adam_params {

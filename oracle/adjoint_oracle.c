@@ -413,3 +413,50 @@ void oracle_sample_colors_f32(const oracle_volume* vol, const float* C, const fl
         out[3 * k + 2] = q[2];
     }
 }
+
+/* Model-to-frame loss of pose refinement (no counterpart in the reference):
+ * each pair scores one frame's depth at the projection of one model point,
+ * as a quadratic truncated at tau (tau <= 0: not truncated).
+ * With g = 2 w r / npairs and the bilinear slopes of sample_depth,
+ *   dr/dp = (dd/du fx/z, dd/dv fy/z, -(dd/du fx x + dd/dv fy y)/z^2 - 1),
+ *   dL/drho += g dr/dp,  dL/domega += p x (g dr/dp)
+ * (a camera point moves by rho + omega x p under the left perturbation). */
+double oracle_frame_loss_f64(const oracle_volume* vol, const float* depth, int32_t rows,
+                             int32_t cols, const double* Rt, int32_t nframes, const double* points,
+                             const int32_t* pairs, const double* weights, int64_t npairs,
+                             double tau, double* grad_pose, double* residuals) {
+    double loss = 0.0;
+    for (int64_t k = 0; k < npairs; ++k) {
+        const int32_t f = pairs[2 * k], q = pairs[2 * k + 1];
+        if (residuals) residuals[k] = 0.0;
+        if (f < 0 || f >= nframes) continue;
+        sample s;
+        sample_depth(vol, Rt + 12 * f, depth + (int64_t)f * rows * cols, rows, cols,
+                     points + 3 * (int64_t)q, &s);
+        if (!s.valid) continue;
+        const double r = s.d - s.p[2];
+        const double w = weights ? weights[k] : 1.0;
+        if (residuals) residuals[k] = r;
+        if (tau > 0.0 && !(fabs(r) < tau)) {
+            loss += w * tau * tau;
+            continue;
+        }
+        loss += w * r * r;
+        if (!grad_pose) continue;
+        const double g = 2.0 * w * r / (double)npairs;
+        const double dd_du = (1 - s.b) * (s.d10 - s.d00) + s.b * (s.d11 - s.d01);
+        const double dd_dv = (1 - s.a) * (s.d01 - s.d00) + s.a * (s.d11 - s.d10);
+        const double iz = 1.0 / s.p[2];
+        const double gx = g * dd_du * vol->fx * iz;
+        const double gy = g * dd_dv * vol->fy * iz;
+        const double gz = -(gx * s.p[0] + gy * s.p[1]) * iz - g;
+        double* o = grad_pose + 6 * f;
+        o[0] += gx;
+        o[1] += gy;
+        o[2] += gz;
+        o[3] += s.p[1] * gz - s.p[2] * gy;
+        o[4] += s.p[2] * gx - s.p[0] * gz;
+        o[5] += s.p[0] * gy - s.p[1] * gx;
+    }
+    return npairs > 0 ? loss / (double)npairs : 0.0;
+}

@@ -195,6 +195,20 @@ double oracle_surface_loss_f64(const oracle_volume* vol, const double* D, const 
 void oracle_diff_adjoint_f64(const oracle_volume* vol, const float* depth, int32_t rows,
                              int32_t cols, const double* Rt, int32_t nframes, const double* lambda,
                              double* grad_pose, double* grad_depth);
+/* Per-frame (model-to-frame) loss of pose refinement: pair k = (frame
+ * pairs[2k], point pairs[2k+1]) has residual r = d(s) - z, the frame's
+ * bilinear depth at the projection s of world point p = points[3*point]
+ * minus the point's camera z (0 when the sample is invalid: z <= 0, the 2x2
+ * footprint leaves the image or a tap is not a valid depth).  With tau > 0
+ * a residual of |r| >= tau counts tau^2 and has no gradient (a truncated
+ * quadratic).  Returns L = sum_k w_k min(r_k^2, tau^2) / npairs (w =
+ * weights, NULL for all 1) and adds dL/dxi (6 per frame, left perturbation,
+ * the adjoint's convention) into grad_pose (may be NULL); residuals (may be
+ * NULL) receives r. */
+double oracle_frame_loss_f64(const oracle_volume* vol, const float* depth, int32_t rows,
+                             int32_t cols, const double* Rt, int32_t nframes, const double* points,
+                             const int32_t* pairs, const double* weights, int64_t npairs,
+                             double tau, double* grad_pose, double* residuals);
 /* Per-point weights: 1 when no decision of the point's support voxels (frame
  * validity, footprint, the -trunc skip, the +trunc clamp) can change under a
  * perturbation of size <= h of one pose coordinate or one depth tap, and the

@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
             "oracle_diff_forward_f32": (None, [vp, vp, vp, vp, i32, i32, vp, i32, C.c_int]),
             "oracle_surface_loss_f64": (C.c_double, [vp, vp, vp, vp, i64, vp]),
             "oracle_diff_adjoint_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp]),
+            "oracle_frame_loss_f64": (C.c_double, [vp, vp, i32, i32, vp, i32, vp, vp, vp, i64,
+                                                   C.c_double, vp, vp]),
             "oracle_stable_points": (None, [vp, vp, i32, i32, vp, i32, vp, i64, C.c_double,
                                             C.c_double, vp]),
             "oracle_render_points": (None, [vp, vp, vp, i64, vp, C.c_float, C.c_float, C.c_float,
@@ -379,6 +381,23 @@ def diff_adjoint(vol, depth, Rt, lam, want_depth=True):
                                   _p(np.ascontiguousarray(Rt, np.float64)), F,
                                   _p(np.ascontiguousarray(lam, np.float64)), _p(gp), _p(gd))
     return gp, gd
+
+
+def frame_loss(vol, depth, Rt, points, pairs, weights=None, want_grad=True, tau=0.0):
+    """Model-to-frame loss (float64): (L, dL/dxi (F, 6) or None, residuals
+    (K,)) for pairs (K, 2) int32 of (frame, point); residuals of |r| >= tau
+    (tau > 0) count tau^2 without gradient."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    pts = np.ascontiguousarray(points, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    gp = np.zeros((F, 6), np.float64) if want_grad else None
+    res = np.empty(pr.shape[0], np.float64)
+    L = lib().oracle_frame_loss_f64(C.byref(vol), _p(depth), rows, cols,
+                                    _p(np.ascontiguousarray(Rt, np.float64)), F, _p(pts), _p(pr),
+                                    _p(w), pr.shape[0], float(tau), _p(gp), _p(res))
+    return L, gp, res
 
 
 def stable_points(vol, depth, Rt, points, h=1e-4, kink=1e-5):

@@ -177,7 +177,8 @@ def test_adjoint_pairs_semantics_and_determinism(cuda):
     np.testing.assert_allclose(g3.cpu().numpy(), g1.cpu().numpy(), rtol=1e-5,
                                atol=1e-6 * g1.abs().max().item())
     assert (g4 == 0).all() and (d4 == 0).all()
-    assert np.allclose(d1.cpu().numpy(), d2.cpu().numpy(), rtol=1e-5, atol=1e-9)
+    # depth gradients are exact fixed-point sums: identical run to run
+    assert cuda.equal(d1, d2) and d1.abs().max().item() > 0
     v.close()
 
 

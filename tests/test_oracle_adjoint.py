@@ -294,3 +294,105 @@ def test_float32_forward_restatement_agrees_with_float64():
     assert (err < 1e-6).mean() > 0.995 and err.max() < 3e-4
     # untouched voxels keep the initial value
     assert (D32[W32 == 0] == np.float32(vol.truncation)).all()
+
+
+# ---- model-to-frame loss of pose refinement --------------------------------------
+
+def test_frame_loss_closed_form_on_a_linear_depth_map():
+    """A depth map affine in pixel coordinates, d(i, j) = al i + be j + ga, is
+    sampled exactly by bilinear interpolation (depth[j][i] at pixel (i, j)),
+    so r = d(u, v) - z for the continuous projection (u, v) of each point;
+    the gradient is the analytic one of that residual; invalid samples give
+    r = 0 and no gradient."""
+    rows, cols = 48, 64
+    al, be, ga = 0.004, -0.003, 1.2
+    jj, ii = np.mgrid[0:rows, 0:cols]
+    depth = (al * ii + be * jj + ga).astype(np.float32)
+    vol = O.volume((8, 8, 8), [0.1] * 3, (0.0, 0.0, 0.0), 0.3, 128.0, (50.0, 55.0, 31.5, 23.5))
+    Rt = O.rt_from_poses(np.array([[1.0, 0, 0, 0, 0, 0, 0]]))
+    rng = np.random.default_rng(31)
+    pts = np.c_[rng.uniform(-0.4, 0.4, 50), rng.uniform(-0.3, 0.3, 50), rng.uniform(1.0, 1.5, 50)]
+    pts = np.r_[pts, [[0.0, 0.0, -1.0], [50.0, 0.0, 1.0]]]       # behind, off the image
+    pairs = np.c_[np.zeros(len(pts)), np.arange(len(pts))].astype(np.int32)
+    L, g, res = O.frame_loss(vol, depth[None], Rt, pts, pairs)
+    u = pts[:50, 0] * 50.0 / pts[:50, 2] + 31.5
+    v = pts[:50, 1] * 55.0 / pts[:50, 2] + 23.5
+    dd = al * u + be * v + ga
+    # the map holds float32 values of the affine function
+    np.testing.assert_allclose(res[:50], dd - pts[:50, 2], atol=2e-7)
+    assert res[50] == 0.0 and res[51] == 0.0
+    assert abs(L - np.sum(res ** 2) / len(pts)) < 1e-15
+    # analytic gradient of sum r^2 / K with r = al u + be v + ga - z
+    K = len(pts)
+    x, y, z = pts[:50].T
+    gr = 2 * res[:50] / K
+    drdp = np.c_[al * 50.0 / z, be * 55.0 / z, -(al * 50.0 * x + be * 55.0 * y) / z ** 2 - 1]
+    want = np.r_[(gr[:, None] * drdp).sum(0), (gr[:, None] * np.cross(pts[:50], drdp)).sum(0)]
+    # (slopes from float32 depths: relative error ~ ulp(d) / (0.004 px) = 3e-5)
+    np.testing.assert_allclose(g[0], want, rtol=1e-4, atol=1e-12)
+    # truncated at tau: residuals of |r| >= tau count tau^2 and drop out of
+    # the gradient
+    tau = float(np.median(np.abs(res[:50])))
+    Lt, gt, _ = O.frame_loss(vol, depth[None], Rt, pts, pairs, tau=tau)
+    keep = np.abs(res[:50]) < tau
+    assert 10 < keep.sum() < 40
+    assert abs(Lt - (np.sum(res[:50][keep] ** 2) + tau ** 2 * (~keep).sum()) / K) < 1e-15
+    gk = gr * keep
+    want_t = np.r_[(gk[:, None] * drdp).sum(0), (gk[:, None] * np.cross(pts[:50], drdp)).sum(0)]
+    np.testing.assert_allclose(gt[0], want_t, rtol=1e-4, atol=1e-12)
+
+
+def _frame_pair_setup(F=4, npts=3000):
+    w = scenes.plain_small(num_frames=F, grid=64, voxel_size=0.016)
+    depth = w.render(noise=False)
+    vol = O.volume_from_config(w.config())
+    Rt = O.rt_from_poses(w.poses)
+    # poses a little off, so that residuals are not zero
+    rng = np.random.default_rng(41)
+    Rt = np.stack([O.se3_exp_left(Rt[f], rng.normal(0, [2e-3] * 3 + [3e-3] * 3)) for f in range(F)])
+    pts, nrm = scenes.surface_points(w, npts, seed=42, with_normals=True)
+    pairs = scenes.frame_pairs(w, depth, pts, nrm, poses=w.poses)
+    return w, depth, vol, Rt, pts, pairs
+
+
+def _stable_pairs(w, Rt, pts, pairs, margin=0.02):
+    """1 for pairs whose projection stays inside one bilinear cell under the
+    finite-difference steps (fractional parts of u and v away from 0 and 1)."""
+    fx, fy, cx, cy = w.intrinsics
+    f, q = pairs[:, 0], pairs[:, 1]
+    R = Rt[f, :9].reshape(-1, 3, 3)
+    pc = np.einsum("kij,kj->ki", R, pts[q]) + Rt[f, 9:]
+    u = pc[:, 0] * fx / pc[:, 2] + cx
+    v = pc[:, 1] * fy / pc[:, 2] + cy
+    fu, fv = u - np.floor(u), v - np.floor(v)
+    return ((fu > margin) & (fu < 1 - margin) & (fv > margin) & (fv < 1 - margin)).astype(np.float64)
+
+
+def test_frame_loss_gradient_matches_central_differences():
+    w, depth, vol, Rt, pts, pairs = _frame_pair_setup()
+    assert len(np.unique(pairs[:, 0])) == Rt.shape[0] and len(pairs) > 1000
+    wts = _stable_pairs(w, Rt, pts, pairs)
+    assert wts.mean() > 0.8
+    L, g, res = O.frame_loss(vol, depth, Rt, pts, pairs, wts)
+    assert L > 0 and (np.abs(g).max(axis=1) > 0).all()
+    h = 1e-6
+    fd = np.zeros_like(g)
+    for f in range(Rt.shape[0]):
+        for i in range(6):
+            e = np.zeros(6)
+            e[i] = h
+            Rp, Rm = Rt.copy(), Rt.copy()
+            Rp[f] = O.se3_exp_left(Rt[f], e)
+            Rm[f] = O.se3_exp_left(Rt[f], -e)
+            fd[f, i] = (O.frame_loss(vol, depth, Rp, pts, pairs, wts, want_grad=False)[0] -
+                        O.frame_loss(vol, depth, Rm, pts, pairs, wts, want_grad=False)[0]) / (2 * h)
+    err = np.abs(fd - g).max() / np.abs(g).max()
+    assert err < 1e-5, (err, g, fd)
+
+
+def test_frame_pairs_are_grouped_by_frame_and_seen():
+    w, depth, vol, Rt, pts, pairs = _frame_pair_setup()
+    assert (np.diff(pairs[:, 0]) >= 0).all()
+    # at the true poses the noise-free residuals of the pairs are small
+    _, _, res = O.frame_loss(vol, depth, O.rt_from_poses(w.poses), pts, pairs, want_grad=False)
+    assert np.percentile(np.abs(res), 95) < 2e-3
