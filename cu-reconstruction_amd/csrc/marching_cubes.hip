@@ -372,6 +372,60 @@ extern "C" int recon_mc_extract(const float* field, const int32_t grid[3],
     return rc;
 }
 
+// ---- marching cubes without allocation ---------------------------------------
+// Scratch of recon_mc_extract_noalloc: the chunk counts (2 B per 64-cube
+// chunk of every cube row) and the tile sums and offsets (12 B per 16 cube
+// rows) -- mc_scratch_need bytes -- in one static array that each GPU holds
+// for the life of the process.  48 MiB covers a 1024^3 grid (1023^2 cube
+// rows of 16 chunks: 34.6 MB).
+constexpr size_t kMcNoAllocScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kMcNoAllocScratchBytes];
+
+// The array's device address on the current device, looked up once per
+// device (hipGetSymbolAddress; the host-side name is not a device address).
+static int noalloc_scratch(void** out) {
+    static std::mutex mu;
+    static void* addr[64] = {nullptr};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(RECON_ERR_DEVICE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(marching-cubes scratch)");
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return RECON_OK;
+}
+
+extern "C" int recon_mc_extract_noalloc(const float* field, const int32_t grid[3],
+                                        const float voxel_size[3], const float offset[3],
+                                        float* verts, int32_t* tris, int64_t capacity_triangles,
+                                        int64_t* num_triangles, recon_stream_t stream) {
+    clear_error();
+    const char* fn = "recon_mc_extract_noalloc";
+    if (!num_triangles) return fail(RECON_ERR_NULL, "%s: num_triangles is NULL", fn);
+    *num_triangles = 0;
+    int rc = check_grid(field, grid, voxel_size, offset, fn);
+    if (rc) return rc;
+    if (mc_scratch_need(grid) > kMcNoAllocScratchBytes)
+        return fail(RECON_ERR_RANGE,
+                    "%s: a %d x %d x %d grid needs %zu bytes of scratch, above the static maximum "
+                    "of %zu (2 B per 64-cube chunk of each of the (gy-1)*(gz-1) cube rows)",
+                    fn, grid[0], grid[1], grid[2], mc_scratch_need(grid), kMcNoAllocScratchBytes);
+    rc = require_gfx950();
+    if (rc) return rc;
+    void* sp = nullptr;
+    rc = noalloc_scratch(&sp);
+    if (rc) return rc;
+    size_t bytes = kMcNoAllocScratchBytes;
+    return mc_entry(field, grid, voxel_size, offset, verts, tris, capacity_triangles, num_triangles,
+                    false, &sp, &bytes, (hipStream_t)stream, fn, false);
+}
+
 extern "C" int recon_mc_class_histogram(const float* field, const int32_t grid[3],
                                         unsigned long long* hist_device, recon_stream_t stream) {
     clear_error();

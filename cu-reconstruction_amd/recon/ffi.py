@@ -117,6 +117,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_float), _pi64, _vp]),
     "recon_mc_extract": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
                                    C.POINTER(C.c_float), _vp, _vp, _i64, _pi64, _vp]),
+    "recon_mc_extract_noalloc": (C.c_int, [_vp, C.POINTER(_i32), C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float), _vp, _vp, _i64, _pi64, _vp]),
     "recon_mc_class_histogram": (C.c_int, [_vp, C.POINTER(_i32), _vp, _vp]),
     "recon_tsdf_enable_color": (C.c_int, [_vp, C.c_int, _vp]),
     "recon_tsdf_colors": (C.c_void_p, [_vp]),

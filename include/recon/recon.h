@@ -193,6 +193,20 @@ RECON_API int recon_mc_extract(const float* field, const int32_t grid[3], const 
                      const float offset[3], float* verts, int32_t* tris,
                      int64_t capacity_triangles, int64_t* num_triangles,
                      recon_stream_t stream);
+/* No counterpart in the reference: recon_mc_extract for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and status
+ * codes as recon_mc_extract, but it calls no hipMalloc / hipFree: its
+ * scratch (2 B per 64-cube chunk of each of the (gy-1)*(gz-1) cube rows,
+ * plus 12 B per 16 cube rows) lives in one static 48 MiB device array per
+ * GPU, held until the process ends; a 1024^3 grid fits, and grids that need
+ * more fail with RECON_ERR_RANGE.  Runs on the current device.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other host
+ * threads, or other streams with work in flight) race.  sync. */
+RECON_API int recon_mc_extract_noalloc(const float* field, const int32_t grid[3],
+                                       const float voxel_size[3], const float offset[3],
+                                       float* verts, int32_t* tris, int64_t capacity_triangles,
+                                       int64_t* num_triangles, recon_stream_t stream);
 /* async.  hist_device (256 uint64, device memory) <- number of cubes of each
  * class, bit i of a class set iff corner i is < 0 (marching_cubes.cu:76-84).
  * The reference has no such call; it exposes the classification that the

@@ -240,3 +240,27 @@ def test_transformation_voxels_rejects_null_before_any_launch():
     lib = ffi.load()
     assert lib.recon_tsdf_transformation_voxels(None, C.c_void_p(4096), None) == ffi.ERR_NULL
 
+
+def _gpu_present() -> bool:
+    import torch
+    return torch.cuda.device_count() > 0
+
+
+@pytest.mark.skipif(_gpu_present(), reason="host-only check: it passes a fake field address, which "
+                                           "must never reach a kernel")
+def test_noalloc_mc_refuses_grids_above_its_static_scratch():
+    """recon_mc_extract_noalloc checks its 48 MiB static scratch before any
+    device call: a 65537 x 160 x 160 grid needs 2 B for each of the 1024
+    64-cube chunks of each of its 159^2 cube rows (51.8 MB) and is refused
+    with RECON_ERR_RANGE; the message names the limit."""
+    lib = ffi.load()
+    n = C.c_int64(-1)
+    st = lib.recon_mc_extract_noalloc(C.c_void_p(1024), ffi.i32x3((65537, 160, 160)),
+                                      ffi.f32x3((1, 1, 1)), ffi.f32x3((0, 0, 0)), None, None, 0,
+                                      C.byref(n), None)
+    assert st == ffi.ERR_RANGE
+    assert b"static maximum" in lib.recon_last_error()
+    assert n.value == 0
+    assert lib.recon_mc_extract_noalloc(None, ffi.i32x3((4, 4, 4)), ffi.f32x3((1, 1, 1)),
+                                        ffi.f32x3((0, 0, 0)), None, None, 0, C.byref(n),
+                                        None) == ffi.ERR_NULL
