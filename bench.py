@@ -362,16 +362,21 @@ def bench_render(w, depth, steps: int, frames: int = 30, cpu: bool = True) -> di
     return out
 
 
-def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
-                          frames: int = 10, points: int = 20000, frame_weight: float = 1.0) -> dict:
+def bench_pose_refinement(steps: int = 20, fragments: int = 32, grid: int = 128,
+                          frames: int = 10, points: int = 20000, frame_weight: float = 100.0,
+                          frame_trunc: float = 0.010, trunc_ratio: float = 0.3,
+                          gn_step: float = 0.7, damping: float = 1e-3) -> dict:
     """Batched pose refinement: `fragments` fragments (128^3, 8 mm, 10 frames
-    of 640x480 of a distinct seeded box/sphere scene), initial poses perturbed
-    by up to 1 degree and 1 cm, `steps` on-device steps (one graph launch per
-    step: forward, fused-TSDF surface loss plus the model-to-frame term with
-    weight `frame_weight`, adjoint, Adam) over all fragments.  Reports the time
-    per step, pose errors, the loss and, as the quality measure, the RMS distance to the
-    analytic surface of the plain integrate's mesh with initial, refined and
-    ground-truth poses."""
+    of 640x480 of a distinct seeded scene of boxes and spheres on a floor),
+    initial poses perturbed by up to 1 degree and 1 cm, `steps` on-device
+    damped Gauss-Newton steps (one graph launch per step: forward, fused-TSDF
+    surface loss plus the model-to-frame term with weight `frame_weight` and
+    its point-to-plane normal matrices, adjoint, a 6x6 solve per frame) over
+    all fragments; the model-to-frame truncation shrinks from `frame_trunc`
+    by `trunc_ratio` over the first half of the steps.  Reports the time per
+    step, per-frame pose errors, the loss (at `frame_trunc`) and, as the
+    quality measure, the RMS distance to the analytic surface of the plain
+    integrate's mesh with initial, refined and ground-truth poses."""
     import numpy as np
     import torch
 
@@ -391,22 +396,24 @@ def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
         depths_np.append(depth)
         spec.append((w.config(), torch.from_numpy(depth).cuda(),
                      torch.from_numpy(pts.astype(np.float32)).cuda(),
-                     torch.from_numpy(wts).cuda(), torch.from_numpy(pairs).cuda(), frame_weight))
+                     torch.from_numpy(wts).cuda(), torch.from_numpy(pairs).cuda(), frame_weight,
+                     frame_trunc, torch.from_numpy(nrm.astype(np.float32)).cuda()))
     log(f"pose refinement: {fragments} fragments prepared in {time.perf_counter() - t0:.1f} s")
     init = np.concatenate([recon.pose_to_matrix(w.meta["init"]) for w in ws])
     gt = np.concatenate([recon.pose_to_matrix(w.poses) for w in ws])
     b = recon.DiffBatch(spec)
     loss_gt = b.forward(torch.from_numpy(gt).cuda()).sum().item()
+    loss_init = b.forward(torch.from_numpy(init).cuda()).sum().item()
     rt = torch.from_numpy(init.copy()).cuda()
-    b.refine(rt, 1)  # warm-up: capture and first replay
+    b.refine_gn(rt, 1)  # warm-up: capture and first replay
     rt = torch.from_numpy(init.copy()).cuda()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    hist = b.refine(rt, steps, lr_translation=5e-4, lr_rotation=1e-3, decay=0.95)
+    b.refine_gn(rt, steps, step=gn_step, damping=damping, trunc_ratio=trunc_ratio)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
     refined = rt.cpu().numpy()
-    h = hist.cpu().numpy()
+    loss_final = b.forward(rt).sum().item()
     b.close()
     a0, c0 = scenes.pose_errors(init, gt)
     a1, c1 = scenes.pose_errors(refined, gt)
@@ -446,17 +453,24 @@ def bench_pose_refinement(steps: int = 50, fragments: int = 32, grid: int = 128,
 
     return {"fragments": fragments, "grid": grid, "frames": frames, "points_per_fragment": points,
             "frame_pairs": int(npairs), "frame_weight": frame_weight,
+            "optimizer": {"method": "damped Gauss-Newton, point-to-plane normal matrix",
+                          "step": gn_step, "damping": damping, "frame_trunc_m": frame_trunc,
+                          "trunc_ratio": round(trunc_ratio, 6)},
             "steps": steps, "ms_per_step": round(1000.0 * dt / max(steps, 1), 3),
-            "loss_initial": float(h[0].sum()), "loss_final": float(h[-1].sum()),
+            "loss_initial": float(loss_init), "loss_final": float(loss_final),
             "loss_ground_truth": float(loss_gt),
             "rotation_error_deg": {"initial_mean": round(float(a0.mean()), 4),
                                    "final_mean": round(float(a1.mean()), 4),
                                    "initial_max": round(float(a0.max()), 4),
-                                   "final_max": round(float(a1.max()), 4)},
+                                   "final_p99": round(float(np.percentile(a1, 99)), 4),
+                                   "final_max": round(float(a1.max()), 4),
+                                   "frames_above_initial_max": int((a1 > a0.max()).sum())},
             "centre_error_mm": {"initial_mean": round(float(c0.mean()) * 1e3, 3),
                                 "final_mean": round(float(c1.mean()) * 1e3, 3),
                                 "initial_max": round(float(c0.max()) * 1e3, 3),
-                                "final_max": round(float(c1.max()) * 1e3, 3)},
+                                "final_p99": round(float(np.percentile(c1, 99)) * 1e3, 3),
+                                "final_max": round(float(c1.max()) * 1e3, 3),
+                                "frames_above_initial_max": int((c1 > c0.max()).sum())},
             "common_mode_error": {"initial_mm_deg": common_mode(init),
                                   "final_mm_deg": common_mode(refined)},
             "mesh_rms_mm": {"initial": round(mesh_error(init) * 1e3, 4),

@@ -21,10 +21,13 @@
 //             partials summed per (frame, component) over the volume's units
 //             in order;
 //   update    Adam on xi = (rho, omega) per frame with second moments shared
-//             per volume (translations, rotations), T <- Exp(step) T in
-//             float64, R re-orthonormalised, stored as float32.
-// recon_diff_batch_refine captures forward + backward + update once and
-// replays the graph per step.
+//             per volume (translations, rotations), or a damped Gauss-Newton
+//             step per frame from the model-to-frame term's 6x6 normal
+//             matrix (accumulated by the forward's frame-term kernel);
+//             T <- Exp(step) T in float64, R re-orthonormalised, stored as
+//             float32.
+// recon_diff_batch_refine and recon_diff_batch_refine_gn capture forward +
+// backward + update once and replay the graph per step.
 #pragma clang fp contract(off)
 
 #include "diff_device.hpp"
@@ -55,6 +58,7 @@ struct DVol {
     int64_t nfpairs;
     double fweight;
     double ftrunc;           // residuals of |r| >= ftrunc count ftrunc^2 (0: off)
+    const float* fnormals;   // unit world normals of the points, or null
 };
 
 struct DFrame {
@@ -220,22 +224,43 @@ __global__ __launch_bounds__(256) void dbatch_loss_sum_kernel(const DVol* __rest
 
 // Model-to-frame term: one workgroup per frame g over its pairs
 // [fr_range[2g], fr_range[2g+1]) of its volume, in float64 (the frame table's
-// float32 R, t widened): the weighted sum of min(r^2, trunc^2) / K into
+// float32 R, t widened): the weighted sum of min(r^2, tau^2) / K into
 // floss[g] (a truncated residual has no gradient) and its
 // gradient w.r.t. the frame's left perturbation into fgrad[6g..6g+5] (the
-// adjoint's formula at the point), each summed in a fixed order.
+// adjoint's formula at the point), each summed in a fixed order.  tau is the
+// volume's truncation, times ratio^min(1, step / ka) when `anneal` is given
+// (Gauss-Newton refinement).  With kHess, the frame's Gauss-Newton system
+// instead: fgrad = fweight * 2 sum r J / K and the upper triangle of
+// fweight * 2 sum J J^T / K into fhess[21g..21g+20] (row-major), over the
+// pairs with |r| < tau, J = (j, p x j): j = -(z / (n . p)) n with n the
+// point's normal in the camera frame (point-to-plane; |n . p| < 0.1 |n| |p|
+// adds nothing) when the volume has normals, else the bilinear slopes'
+// dr/dp (then fgrad is the exact gradient, as without kHess).
+struct FrameAnneal {
+    const int* step;  // null: fixed truncation
+    double ratio, ka;
+};
+
+template <bool kHess>
 __global__ __launch_bounds__(256) void dbatch_frame_loss_kernel(
     const DVol* __restrict__ vols, const DFrame* __restrict__ fr,
     const FrameParams* __restrict__ table, const int64_t* __restrict__ fr_range,
-    double* __restrict__ floss, double* __restrict__ fgrad) {
-    __shared__ double s_red[7][4];
+    const FrameAnneal anneal, double* __restrict__ floss, double* __restrict__ fgrad,
+    double* __restrict__ fhess) {
+    constexpr int kAcc = kHess ? 28 : 7;
+    __shared__ double s_red[kAcc][4];
     const int g = blockIdx.x;
     const DVol& dv = vols[fr[g].volume];
     const VolumeParams& vp = dv.vp;
     const FrameParams& L = table[g];
     const int64_t k0 = fr_range[2 * g], k1 = fr_range[2 * g + 1];
     const double scale = dv.nfpairs > 0 ? dv.fweight / (double)dv.nfpairs : 0.0;
-    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double tau = dv.ftrunc;
+    if (anneal.step) tau *= pow(anneal.ratio, fmin(1.0, (double)*anneal.step / anneal.ka));
+    const float* nw = kHess ? dv.fnormals : nullptr;
+    double acc[kAcc];
+#pragma unroll
+    for (int c = 0; c < kAcc; ++c) acc[c] = 0.0;
     for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
         const int64_t q = dv.fpairs[2 * k + 1];
         const double c[3] = {(double)dv.points[3 * q], (double)dv.points[3 * q + 1],
@@ -260,8 +285,8 @@ __global__ __launch_bounds__(256) void dbatch_frame_loss_kernel(
             continue;
         const double d = (1 - a) * (1 - b) * d00 + a * (1 - b) * d10 + (1 - a) * b * d01 + a * b * d11;
         const double res = d - pc[2];
-        if (dv.ftrunc > 0.0 && !(fabs(res) < dv.ftrunc)) {
-            acc[0] += dv.ftrunc * dv.ftrunc;
+        if (tau > 0.0 && !(fabs(res) < tau)) {
+            acc[0] += tau * tau;
             continue;
         }
         acc[0] += res * res;
@@ -269,27 +294,66 @@ __global__ __launch_bounds__(256) void dbatch_frame_loss_kernel(
         const double dd_du = (1 - b) * (d10 - d00) + b * (d11 - d01);
         const double dd_dv = (1 - a) * (d01 - d00) + a * (d11 - d10);
         const double iz = 1.0 / pc[2];
-        const double gx = gs * dd_du * (double)vp.fx * iz;
-        const double gy = gs * dd_dv * (double)vp.fy * iz;
-        const double gz = -(gx * pc[0] + gy * pc[1]) * iz - gs;
-        acc[1] += gx;
-        acc[2] += gy;
-        acc[3] += gz;
-        acc[4] += pc[1] * gz - pc[2] * gy;
-        acc[5] += pc[2] * gx - pc[0] * gz;
-        acc[6] += pc[0] * gy - pc[1] * gx;
+        if constexpr (!kHess) {
+            const double gx = gs * dd_du * (double)vp.fx * iz;
+            const double gy = gs * dd_dv * (double)vp.fy * iz;
+            const double gz = -(gx * pc[0] + gy * pc[1]) * iz - gs;
+            acc[1] += gx;
+            acc[2] += gy;
+            acc[3] += gz;
+            acc[4] += pc[1] * gz - pc[2] * gy;
+            acc[5] += pc[2] * gx - pc[0] * gz;
+            acc[6] += pc[0] * gy - pc[1] * gx;
+        } else {
+            // the Gauss-Newton system of one linearisation: J from the
+            // normal (point-to-plane) when the volume has normals, else
+            // from the slopes
+            double j[3];
+            bool use = true;
+            if (nw) {
+                const double nwx = nw[3 * q], nwy = nw[3 * q + 1], nwz = nw[3 * q + 2];
+                double n[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    n[r] = (double)L.R[3 * r] * nwx + (double)L.R[3 * r + 1] * nwy +
+                           (double)L.R[3 * r + 2] * nwz;
+                const double np = n[0] * pc[0] + n[1] * pc[1] + n[2] * pc[2];
+                const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                const double pn = sqrt(pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
+                use = fabs(np) >= 0.1 * nn * pn;
+                const double cj = use ? -pc[2] / np : 0.0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) j[r] = cj * n[r];
+            } else {
+                j[0] = dd_du * (double)vp.fx * iz;
+                j[1] = dd_dv * (double)vp.fy * iz;
+                j[2] = -(j[0] * pc[0] + j[1] * pc[1]) * iz - 1.0;
+            }
+            if (use) {
+                const double J[6] = {j[0], j[1], j[2], pc[1] * j[2] - pc[2] * j[1],
+                                     pc[2] * j[0] - pc[0] * j[2], pc[0] * j[1] - pc[1] * j[0]};
+#pragma unroll
+                for (int r = 0; r < 6; ++r) acc[1 + r] += gs * J[r];
+                int e = 7;
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+#pragma unroll
+                    for (int cc = r; cc < 6; ++cc) acc[e++] += J[r] * J[cc];
+            }
+        }
     }
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
+    for (int c = 0; c < kAcc; ++c) {
         const double w = wave_sum_f64(acc[c]);
         if ((threadIdx.x & 63) == 0) s_red[c][threadIdx.x >> 6] = w;
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < kAcc) {
         const int c = threadIdx.x;
         const double sum = ((s_red[c][0] + s_red[c][1]) + s_red[c][2]) + s_red[c][3];
         if (c == 0) floss[g] = sum * scale;
-        else fgrad[6 * g + c - 1] = sum;
+        else if (c < 7) fgrad[6 * g + c - 1] = sum;
+        else fhess[21 * g + c - 7] = 2.0 * scale * sum;
     }
 }
 
@@ -416,6 +480,35 @@ __device__ void se3_exp_f64(const double xi[6], double R[9], double t[3]) {
     for (int r = 0; r < 3; ++r) t[r] = V[3 * r] * xi[0] + V[3 * r + 1] * xi[1] + V[3 * r + 2] * xi[2];
 }
 
+// T <- Exp(xi) T in float64 for a float32 [R | t] (12 floats), R's rows
+// re-orthonormalised (Gram-Schmidt, third row from the cross product).
+__device__ void left_update(float* __restrict__ pose, const double xi[6]) {
+    double Re[9], te[3];
+    se3_exp_f64(xi, Re, te);
+    double R0[9], t0[3], R[9], tt[3];
+    for (int k = 0; k < 9; ++k) R0[k] = pose[k];
+    for (int k = 0; k < 3; ++k) t0[k] = pose[9 + k];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            R[3 * r + c] = Re[3 * r] * R0[c] + Re[3 * r + 1] * R0[3 + c] + Re[3 * r + 2] * R0[6 + c];
+        tt[r] = Re[3 * r] * t0[0] + Re[3 * r + 1] * t0[1] + Re[3 * r + 2] * t0[2] + te[r];
+    }
+    double* a = R;
+    double* b = R + 3;
+    double* c = R + 6;
+    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; ++k) a[k] /= na;
+    const double d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    for (int k = 0; k < 3; ++k) b[k] -= d * a[k];
+    const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    for (int k = 0; k < 3; ++k) b[k] /= nb;
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+    for (int k = 0; k < 9; ++k) pose[k] = (float)R[k];
+    for (int k = 0; k < 3; ++k) pose[9 + k] = (float)tt[k];
+}
+
 // One workgroup per volume, one thread per frame.  Adam with a first moment
 // per coordinate and a second moment shared by the volume's translations and
 // one shared by its rotations (the mean square over the volume's frames): a
@@ -473,31 +566,66 @@ __global__ __launch_bounds__(64) void dbatch_adam_kernel(const DVol* __restrict_
         const double vk = k < 3 ? vt : vr;
         xi[k] = -lr * (mk / c1) / (sqrt(vk / c2) + opt.eps);
     }
-    double Re[9], te[3];
-    se3_exp_f64(xi, Re, te);
-    double R0[9], t0[3], R[9], tt[3];
-    for (int k = 0; k < 9; ++k) R0[k] = poses[12 * g + k];
-    for (int k = 0; k < 3; ++k) t0[k] = poses[12 * g + 9 + k];
-    for (int r = 0; r < 3; ++r) {
-        for (int c = 0; c < 3; ++c)
-            R[3 * r + c] = Re[3 * r] * R0[c] + Re[3 * r + 1] * R0[3 + c] + Re[3 * r + 2] * R0[6 + c];
-        tt[r] = Re[3 * r] * t0[0] + Re[3 * r + 1] * t0[1] + Re[3 * r + 2] * t0[2] + te[r];
+    left_update(poses + 12 * (size_t)g, xi);
+}
+
+// One thread per frame: a damped Gauss-Newton step.  M = H + mu diag(H) +
+// 1e-12 max(diag H) I from the frame's normal matrix (upper triangle, 21
+// doubles), factored M = L L^T column by column; a column whose pivot is at
+// most 1e-9 max(diag M) (a coordinate the pairs do not fix beyond the ones
+// before it) is skipped: its row and column of L are zero and its coordinate
+// of x is 0.  x solves the rest; T <- Exp(-alpha x) T.  A frame without
+// pairs (H = 0) keeps its pose.
+struct GnState {
+    double alpha, mu;
+};
+
+__global__ __launch_bounds__(64) void dbatch_gn_kernel(int nframes, float* __restrict__ poses,
+                                                       const double* __restrict__ grad,
+                                                       const double* __restrict__ hess,
+                                                       const GnState opt) {
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    if (g >= nframes) return;
+    double M[6][6];
+    int e = 0;
+    for (int r = 0; r < 6; ++r)
+        for (int c = r; c < 6; ++c) {
+            M[r][c] = hess[21 * (size_t)g + e++];
+            M[c][r] = M[r][c];
+        }
+    double dmax = 0.0;
+    for (int r = 0; r < 6; ++r) dmax = fmax(dmax, M[r][r]);
+    if (!(dmax > 0.0)) return;
+    for (int r = 0; r < 6; ++r) M[r][r] = M[r][r] * (1.0 + opt.mu) + 1e-12 * dmax;
+    double mmax = 0.0;
+    for (int r = 0; r < 6; ++r) mmax = fmax(mmax, M[r][r]);
+    bool live[6];
+    for (int c = 0; c < 6; ++c) {
+        double d = M[c][c];
+        for (int k = 0; k < c; ++k) d -= M[c][k] * M[c][k];
+        live[c] = d > 1e-9 * mmax;
+        const double l = live[c] ? sqrt(d) : 0.0;
+        M[c][c] = l;
+        for (int r = c + 1; r < 6; ++r) {
+            double v = M[r][c];
+            for (int k = 0; k < c; ++k) v -= M[r][k] * M[c][k];
+            M[r][c] = live[c] ? v / l : 0.0;
+        }
     }
-    // rows re-orthonormalised (Gram-Schmidt), third row from the cross product
-    double* a = R;
-    double* b = R + 3;
-    double* c = R + 6;
-    const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    for (int k = 0; k < 3; ++k) a[k] /= na;
-    const double d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    for (int k = 0; k < 3; ++k) b[k] -= d * a[k];
-    const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    for (int k = 0; k < 3; ++k) b[k] /= nb;
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
-    for (int k = 0; k < 9; ++k) poses[12 * g + k] = (float)R[k];
-    for (int k = 0; k < 3; ++k) poses[12 * g + 9 + k] = (float)tt[k];
+    double y[6], x[6];
+    for (int r = 0; r < 6; ++r) {
+        double v = grad[6 * (size_t)g + r];
+        for (int k = 0; k < r; ++k) v -= M[r][k] * y[k];
+        y[r] = live[r] ? v / M[r][r] : 0.0;
+    }
+    for (int r = 5; r >= 0; --r) {
+        double v = y[r];
+        for (int k = r + 1; k < 6; ++k) v -= M[k][r] * x[k];
+        x[r] = live[r] ? v / M[r][r] : 0.0;
+    }
+    double xi[6];
+    for (int k = 0; k < 6; ++k) xi[k] = -opt.alpha * x[k];
+    left_update(poses + 12 * (size_t)g, xi);
 }
 
 // Records the step's losses and advances the step counter.
@@ -551,7 +679,13 @@ struct recon_diff_batch {
     int64_t* fr_range = nullptr;  // model-to-frame pairs of each frame
     double* floss = nullptr;      // per-frame model-to-frame loss and gradient
     double* fgrad = nullptr;
+    double* fhess = nullptr;      // per-frame normal matrices (Gauss-Newton refinement)
     bool frame_term = false;
+    bool all_frame_terms = false; // every volume has pairs
+    // set while a Gauss-Newton step is captured: the annealed truncation and
+    // the normal matrices of the frame term
+    FrameAnneal anneal{nullptr, 1.0, 1.0};
+    bool want_hess = false;
     uint32_t* owner = nullptr;
     char* merge_scratch = nullptr;
     MergeLayout M;
@@ -579,6 +713,7 @@ struct recon_diff_batch {
     int forward(const float* poses, double* loss_out, hipStream_t s);
     int backward(double* grad_out, hipStream_t s);
     int update(float* poses, const AdamState& opt, double* history, hipStream_t s);
+    int update_gn(float* poses, const GnState& opt, double* history, hipStream_t s);
 };
 
 int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t s) {
@@ -594,9 +729,12 @@ int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t 
     if (loss_blocks > 0)
         hipLaunchKernelGGL(dbatch_loss_kernel, dim3(loss_blocks), dim3(256), 0, s, d_vols,
                            d_loss_wg, loss_partials, lam_ids, lam);
-    if (frame_term)
-        hipLaunchKernelGGL(dbatch_frame_loss_kernel, dim3(total_frames), dim3(256), 0, s, d_vols,
-                           d_frames, table, fr_range, floss, fgrad);
+    if (frame_term && want_hess)
+        hipLaunchKernelGGL(dbatch_frame_loss_kernel<true>, dim3(total_frames), dim3(256), 0, s,
+                           d_vols, d_frames, table, fr_range, anneal, floss, fgrad, fhess);
+    else if (frame_term)
+        hipLaunchKernelGGL(dbatch_frame_loss_kernel<false>, dim3(total_frames), dim3(256), 0, s,
+                           d_vols, d_frames, table, fr_range, anneal, floss, fgrad, nullptr);
     hipLaunchKernelGGL(dbatch_loss_sum_kernel, dim3(n), dim3(256), 0, s, d_vols, loss_partials,
                        frame_term ? floss : nullptr, loss_out);
     RECON_HIP_TRY(hipGetLastError(), "diff batch loss launch");
@@ -633,6 +771,14 @@ int recon_diff_batch::update(float* poses, const AdamState& opt, double* history
                        adam_v, step, opt);
     hipLaunchKernelGGL(dbatch_step_kernel, dim3(1), dim3(256), 0, s, step, loss, n, history);
     RECON_HIP_TRY(hipGetLastError(), "diff batch update launch");
+    return RECON_OK;
+}
+
+int recon_diff_batch::update_gn(float* poses, const GnState& opt, double* history, hipStream_t s) {
+    hipLaunchKernelGGL(dbatch_gn_kernel, dim3((total_frames + 63) / 64), dim3(64), 0, s,
+                       total_frames, poses, grad, fhess, opt);
+    hipLaunchKernelGGL(dbatch_step_kernel, dim3(1), dim3(256), 0, s, step, loss, n, history);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch Gauss-Newton update launch");
     return RECON_OK;
 }
 
@@ -686,6 +832,7 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
         return fail(RECON_ERR_RANGE, "recon_diff_batch_create: %lld points exceed 2^28",
                     (long long)points);
     b->arena_voxels = arena;
+    b->all_frame_terms = n > 0;
     b->total_frames = (int)frames;
     b->total_points = points;
     b->npairs = 8 * points;
@@ -758,6 +905,8 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
         dv.nfpairs = v.num_frame_pairs;
         dv.fweight = v.frame_weight;
         dv.ftrunc = v.frame_trunc > 0.0 ? v.frame_trunc : 0.0;
+        dv.fnormals = v.num_frame_pairs > 0 ? v.normals : nullptr;
+        if (v.num_frame_pairs == 0) b->all_frame_terms = false;
         // the model-to-frame pairs of each frame: checked on the host (frames
         // grouped in increasing order, points in range), ranges per frame
         std::vector<int64_t> rng(2 * (size_t)v.num_frames, 0);
@@ -835,6 +984,8 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
     if ((rc = b->alloc(&b->fr_range, fr_ranges.size(), "diff batch frame pairs"))) return rc;
     if ((rc = b->alloc(&b->floss, (size_t)frames, "diff batch frame losses"))) return rc;
     if ((rc = b->alloc(&b->fgrad, (size_t)frames * 6, "diff batch frame gradients"))) return rc;
+    if ((rc = b->alloc(&b->fhess, (size_t)frames * 21, "diff batch frame normal matrices")))
+        return rc;
     if ((rc = b->alloc(&b->owner, arena, "diff batch owner table"))) return rc;
     b->M = merge_layout(b->npairs);
     if ((rc = b->alloc(&b->merge_scratch, b->M.total, "diff batch merge scratch"))) return rc;
@@ -942,6 +1093,47 @@ extern "C" int recon_diff_batch_backward(recon_diff_batch* b, double* grad_pose,
     return b->backward(grad_pose, (hipStream_t)stream);
 }
 
+// One step (forward, backward, `update`) captured once and replayed `steps`
+// times on s.
+template <class Update>
+static int replay_steps(recon_diff_batch* b, float* poses_rt, int steps, hipStream_t s,
+                        Update update) {
+    hipStream_t cs = nullptr;
+    RECON_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    int rc = RECON_OK;
+    const hipError_t e0 = hipStreamSynchronize(s);
+    if (e0 != hipSuccess) rc = hip_fail(e0, "refine synchronize");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (!rc) {
+        const hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = b->forward(poses_rt, b->loss, cs);
+            if (!rc) rc = b->backward(b->grad, cs);
+            if (!rc) rc = update(cs);
+            const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+            if (!rc && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+            if (!rc) {
+                const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e3 != hipSuccess) rc = hip_fail(e3, "hipGraphInstantiate");
+            }
+        } else {
+            rc = hip_fail(e, "hipStreamBeginCapture");
+        }
+    }
+    for (int k = 0; !rc && k < steps; ++k) {
+        const hipError_t e4 = hipGraphLaunch(exec, s);
+        if (e4 != hipSuccess) rc = hip_fail(e4, "hipGraphLaunch");
+    }
+    // the graph may still run: wait before releasing it
+    const hipError_t e5 = hipStreamSynchronize(s);
+    if (!rc && e5 != hipSuccess) rc = hip_fail(e5, "refine synchronize");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    return rc;
+}
+
 extern "C" int recon_diff_batch_refine(recon_diff_batch* b, float* poses_rt, int32_t steps,
                                        const recon_adam_params* opt, double* loss_history,
                                        recon_stream_t stream) {
@@ -961,36 +1153,35 @@ extern "C" int recon_diff_batch_refine(recon_diff_batch* b, float* poses_rt, int
     hipLaunchKernelGGL(dbatch_reset_step_kernel, dim3(1), dim3(64), 0, s, b->step);
     RECON_HIP_TRY(hipGetLastError(), "diff batch refine reset");
     if (steps == 0) return RECON_OK;
-    // one step, captured once and replayed
-    hipStream_t cs = nullptr;
-    RECON_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
-    RECON_HIP_TRY(hipStreamSynchronize(s), "refine synchronize");
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int rc = RECON_OK;
-    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        rc = b->forward(poses_rt, b->loss, cs);
-        if (!rc) rc = b->backward(b->grad, cs);
-        if (!rc) rc = b->update(poses_rt, st, loss_history, cs);
-        const hipError_t e2 = hipStreamEndCapture(cs, &graph);
-        if (!rc && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
-        if (!rc) {
-            const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (e3 != hipSuccess) rc = hip_fail(e3, "hipGraphInstantiate");
-        }
-    } else {
-        rc = hip_fail(e, "hipStreamBeginCapture");
-    }
-    for (int k = 0; !rc && k < steps; ++k) {
-        const hipError_t e4 = hipGraphLaunch(exec, s);
-        if (e4 != hipSuccess) rc = hip_fail(e4, "hipGraphLaunch");
-    }
-    // the graph may still run: wait before releasing it
-    const hipError_t e5 = hipStreamSynchronize(s);
-    if (!rc && e5 != hipSuccess) rc = hip_fail(e5, "refine synchronize");
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipStreamDestroy(cs);
+    return replay_steps(b, poses_rt, steps, s, [&](hipStream_t cs) {
+        return b->update(poses_rt, st, loss_history, cs);
+    });
+}
+
+extern "C" int recon_diff_batch_refine_gn(recon_diff_batch* b, float* poses_rt, int32_t steps,
+                                          const recon_gn_params* opt, double* loss_history,
+                                          recon_stream_t stream) {
+    clear_error();
+    const char* fn = "recon_diff_batch_refine_gn";
+    if (!b || !poses_rt || !opt) return fail(RECON_ERR_NULL, "%s: NULL", fn);
+    if (steps < 0) return fail(RECON_ERR_SIZE, "%s: steps < 0", fn);
+    if (!(opt->step > 0.0 && opt->step <= 1.0 && opt->damping >= 0.0 && opt->damping < 1e30 &&
+          opt->trunc_ratio > 0.0 && opt->trunc_ratio <= 1.0))
+        return fail(RECON_ERR_CONFIG, "%s: step must be in (0, 1], damping >= 0 and trunc_ratio "
+                    "in (0, 1]", fn);
+    if (!b->all_frame_terms)
+        return fail(RECON_ERR_CONFIG, "%s: every volume needs model-to-frame pairs", fn);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(dbatch_reset_step_kernel, dim3(1), dim3(64), 0, s, b->step);
+    RECON_HIP_TRY(hipGetLastError(), "diff batch refine reset");
+    if (steps == 0) return RECON_OK;
+    const GnState st{opt->step, opt->damping};
+    b->anneal = FrameAnneal{b->step, opt->trunc_ratio, (double)std::max(1, steps / 2 - 1)};
+    b->want_hess = true;
+    const int rc = replay_steps(b, poses_rt, steps, s, [&](hipStream_t cs) {
+        return b->update_gn(poses_rt, st, loss_history, cs);
+    });
+    b->anneal = FrameAnneal{nullptr, 1.0, 1.0};
+    b->want_hess = false;
     return rc;
 }

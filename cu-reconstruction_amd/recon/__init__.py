@@ -688,9 +688,10 @@ class DiffBatch:
     a list of (config, depth tensor (F, H, W) on the device, points (K, 3)
     float32 device tensor, weights (K,) float32 device tensor or None), each
     optionally followed by model-to-frame pairs ((P, 2) int32 device tensor
-    of (frame, point), grouped by frame), their weight in the loss and the
-    residual's truncation (metres; 0: none); the tensors are kept alive here
-    because every step reads them."""
+    of (frame, point), grouped by frame), their weight in the loss, the
+    residual's truncation (metres; 0: none) and the points' unit normals
+    ((K, 3) float32 device tensor or None, read by refine_gn); the tensors are
+    kept alive here because every step reads them."""
 
     def __init__(self, volumes):
         torch = _torch()
@@ -702,6 +703,7 @@ class DiffBatch:
             cfg, depth, points, weights = spec[:4]
             pairs, pair_weight = (spec[4], float(spec[5])) if len(spec) > 4 else (None, 0.0)
             pair_trunc = float(spec[6]) if len(spec) > 6 else 0.0
+            normals = spec[7] if len(spec) > 7 else None
             if pairs is not None:
                 _device_tensor(pairs, "frame pairs", torch.int32)
                 if pairs.dim() != 2 or pairs.shape[1] != 2 or not pairs.is_contiguous():
@@ -719,7 +721,12 @@ class DiffBatch:
                 _device_tensor(weights, "weights", torch.float32)
                 if not weights.is_contiguous() or weights.numel() != points.shape[0]:
                     raise ValueError("weights must be a contiguous (K,) float32 tensor")
-            self._keep.append((depth, points, weights))
+            if normals is not None:
+                _device_tensor(normals, "normals", torch.float32)
+                if not normals.is_contiguous() or tuple(normals.shape) != tuple(points.shape):
+                    raise ValueError("normals must be a contiguous (K, 3) float32 tensor")
+                arr[i].normals = normals.data_ptr()
+            self._keep.append((depth, points, weights, normals))
             v = arr[i]
             v.config = cfg
             v.num_frames = int(depth.shape[0])
@@ -777,6 +784,19 @@ class DiffBatch:
         opt = ffi.AdamParams(lr_translation, lr_rotation, beta1, beta2, eps, decay)
         check(self._lib.recon_diff_batch_refine(self._h, poses_rt.data_ptr(), int(steps),
                                                 C.byref(opt), hist.data_ptr(), _stream(stream)))
+        return hist[:steps]
+
+    def refine_gn(self, poses_rt, steps, step=0.7, damping=1e-3, trunc_ratio=1.0, stream=None):
+        """Damped Gauss-Newton refinement of poses_rt in place (every volume
+        needs frame pairs); the model-to-frame truncation shrinks by
+        trunc_ratio over the first half of the steps.  Returns the (steps, n)
+        loss history."""
+        torch = _torch()
+        self._check_rt(poses_rt)
+        hist = torch.empty((max(steps, 1), self.n), dtype=torch.float64, device=poses_rt.device)
+        opt = ffi.GnParams(step, damping, trunc_ratio)
+        check(self._lib.recon_diff_batch_refine_gn(self._h, poses_rt.data_ptr(), int(steps),
+                                                   C.byref(opt), hist.data_ptr(), _stream(stream)))
         return hist[:steps]
 
     def volume(self, i):

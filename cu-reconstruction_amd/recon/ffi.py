@@ -73,13 +73,17 @@ class DiffVolume(C.Structure):
                 ("depth", C.c_void_p), ("points", C.c_void_p), ("weights", C.c_void_p),
                 ("num_points", C.c_int64), ("frame_pairs", C.c_void_p),
                 ("num_frame_pairs", C.c_int64), ("frame_weight", C.c_double),
-                ("frame_trunc", C.c_double)]
+                ("frame_trunc", C.c_double), ("normals", C.c_void_p)]
 
 
 class AdamParams(C.Structure):
     _fields_ = [("lr_translation", C.c_double), ("lr_rotation", C.c_double),
                 ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
                 ("decay", C.c_double)]
+
+
+class GnParams(C.Structure):
+    _fields_ = [("step", C.c_double), ("damping", C.c_double), ("trunc_ratio", C.c_double)]
 
 
 class DepthView(C.Structure):
@@ -163,6 +167,7 @@ SIGNATURES = {
     "recon_diff_batch_forward": (C.c_int, [_vp, _vp, _vp, _vp]),
     "recon_diff_batch_backward": (C.c_int, [_vp, _vp, _vp]),
     "recon_diff_batch_refine": (C.c_int, [_vp, _vp, _i32, C.POINTER(AdamParams), _vp, _vp]),
+    "recon_diff_batch_refine_gn": (C.c_int, [_vp, _vp, _i32, C.POINTER(GnParams), _vp, _vp]),
     "recon_tsdf_diff_integrate": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp]),
     "recon_tsdf_surface_loss": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "recon_tsdf_diff_adjoint": (C.c_int, [_vp, C.POINTER(DepthView), C.c_size_t, _i32, _vp, _vp,

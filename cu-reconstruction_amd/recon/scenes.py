@@ -403,18 +403,21 @@ def perturb_pose(pose, rot_rad: float, trans_m: float, rng: Rng) -> np.ndarray:
 def pose_refinement(num_fragments: int = 32, frames: int = 10, seed: int = 0x5EED0004,
                     grid: int = 128, voxel_size: float = 0.008) -> list[Workload]:
     """Fragments for batched pose refinement: each a 128^3 volume (8 mm) of a
-    distinct seeded scene of 2-5 spheres and boxes (at least one box, so no
-    rotation leaves the scene unchanged), seen by `frames` 640x480 cameras on
-    an arc 10 degrees apart.  `poses` are the ground truth; meta["init"] are
-    the poses perturbed by up to 1 degree and 1 cm."""
+    distinct seeded scene on a floor (z = -0.3 m): one box, two spheres and
+    up to two more boxes or spheres, seen by `frames` 640x480 cameras on an
+    arc 10 degrees apart.  Every frame then sees surfaces whose normals fix
+    all six pose coordinates (a box or the floor alone leaves in-plane shifts
+    free, a sphere alone rotations about its centre).  `poses` are the
+    ground truth; meta["init"] are the poses perturbed by up to 1 degree and
+    1 cm."""
     rng = Rng(seed)
     out = []
     half = grid * voxel_size / 2
     for v in range(num_fragments):
         prims = []
-        nprim = rng.integer(2, 5)
+        nprim = rng.integer(3, 5)
         for k in range(nprim):
-            if k == 0 or rng.uniform() < 0.5:
+            if k == 0 or (k >= 3 and rng.uniform() < 0.5):
                 c = [rng.uniform(-0.2, 0.2) for _ in range(3)]
                 h = [rng.uniform(0.05, 0.18) for _ in range(3)]
                 prims.append((PRIM_BOX, tuple(c[i] - h[i] for i in range(3)) +
@@ -422,6 +425,7 @@ def pose_refinement(num_fragments: int = 32, frames: int = 10, seed: int = 0x5EE
             else:
                 prims.append((PRIM_SPHERE, (rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2),
                                             rng.uniform(-0.2, 0.2), rng.uniform(0.06, 0.2))))
+        prims.append((PRIM_PLANE_Z, (-0.3,)))
         poses = arc_poses(frames, step_deg=10.0, radius=1.1, height=rng.uniform(0.1, 0.6),
                           start_deg=rng.uniform(0.0, 360.0))
         init = np.stack([perturb_pose(p, math.radians(1.0), 0.01, rng) for p in poses])

@@ -426,6 +426,11 @@ typedef struct recon_diff_volume {
     int64_t num_frame_pairs;
     double frame_weight;
     double frame_trunc;
+    /* Optional (NULL): device, 3 floats per point, the points' unit surface
+     * normals (world).  Used only by recon_diff_batch_refine_gn, whose
+     * Gauss-Newton system then takes the point-to-plane linearisation of
+     * each pair's residual instead of the bilinear depth slopes. */
+    const float* normals;
 } recon_diff_volume;
 
 typedef struct recon_adam_params {
@@ -462,6 +467,34 @@ RECON_API int recon_diff_batch_backward(recon_diff_batch* b, double* grad_pose,
 RECON_API int recon_diff_batch_refine(recon_diff_batch* b, float* poses_rt, int32_t steps,
                                       const recon_adam_params* opt, double* loss_history,
                                       recon_stream_t stream);
+
+typedef struct recon_gn_params {
+    double step;         /* fraction of the Gauss-Newton step taken, (0, 1] */
+    double damping;      /* Levenberg-Marquardt: solve (H + damping diag(H)) x = g */
+    double trunc_ratio;  /* (0, 1]: at step k every volume's frame_trunc is scaled
+                          * by trunc_ratio^min(1, k / max(1, steps / 2 - 1))
+                          * (integer division); 1: fixed */
+} recon_gn_params;
+
+/* sync.  `steps` damped Gauss-Newton steps on every frame's xi: forward,
+ * backward, then per frame x = M^-1 g, M = H + damping diag(H) +
+ * 1e-12 max(diag H) I factored by Cholesky (a coordinate whose pivot is at
+ * most 1e-9 max(diag M), one the pairs do not fix, takes no step).  H and
+ * the model-to-frame part of g are the Gauss-Newton system of that term,
+ * frame_weight * sum 2 J J^T / num_frame_pairs and frame_weight *
+ * sum 2 r J / num_frame_pairs over the pairs with |r| < trunc, J = (j, p x j)
+ * the residual's derivative with respect to xi linearised as
+ * j = -(z / (n . p)) n from the point's normal n (camera frame) when the
+ * volume has normals (point-to-plane; pairs seen at |n . p| < 0.1 |n| |p| add
+ * nothing), else j = dr/dp from the bilinear depth slopes (the exact
+ * gradient); the surface loss adds its exact gradient to g.
+ * T <- Exp(-step x) T in float64, R re-orthonormalised, poses_rt updated in
+ * place; a frame without pairs (H = 0) keeps its pose.  Every volume needs
+ * frame pairs.  loss_history as recon_diff_batch_refine (each step's loss at
+ * its own truncation).  No counterpart in the reference. */
+RECON_API int recon_diff_batch_refine_gn(recon_diff_batch* b, float* poses_rt, int32_t steps,
+                                         const recon_gn_params* opt, double* loss_history,
+                                         recon_stream_t stream);
 
 /* ---- batched volumes ----------------------------------------------------- */
 /* No counterpart in the reference, which builds one TSDFVolume per scene and

@@ -460,3 +460,65 @@ double oracle_frame_loss_f64(const oracle_volume* vol, const float* depth, int32
     }
     return npairs > 0 ? loss / (double)npairs : 0.0;
 }
+
+/* Gauss-Newton system of the model-to-frame term (no counterpart in the
+ * reference): per frame H = sum 2 w J J^T / npairs and g = sum 2 w r J /
+ * npairs over the valid pairs with |r| < tau (every valid pair when
+ * tau <= 0), J = (j, p x j) the residual's derivative with respect to xi.
+ * With normals (world, one per point), j is the point-to-plane
+ * linearisation of r = d(s) - z: the frame sees the plane through the point
+ * with camera-frame normal n = R n_w, and moving the point by dp changes r by
+ * -(z / (n . p)) n . dp, so j = -(z / (n . p)) n; pairs whose |n . p| is
+ * below 0.1 |n| |p| (grazing views) add nothing.  Without normals j is dr/dp
+ * of oracle_frame_loss_f64 (the bilinear depth slopes), and g is that
+ * function's gradient.  H is (nframes, 6, 6), row-major, and g (nframes, 6)
+ * (may be NULL), both added to. */
+void oracle_frame_normal_matrix_f64(const oracle_volume* vol, const float* depth, int32_t rows,
+                                    int32_t cols, const double* Rt, int32_t nframes,
+                                    const double* points, const double* normals,
+                                    const int32_t* pairs, const double* weights, int64_t npairs,
+                                    double tau, double* H, double* grad) {
+    for (int64_t k = 0; k < npairs; ++k) {
+        const int32_t f = pairs[2 * k], q = pairs[2 * k + 1];
+        if (f < 0 || f >= nframes) continue;
+        sample s;
+        const double* T = Rt + 12 * f;
+        sample_depth(vol, T, depth + (int64_t)f * rows * cols, rows, cols,
+                     points + 3 * (int64_t)q, &s);
+        if (!s.valid) continue;
+        const double r = s.d - s.p[2];
+        if (tau > 0.0 && !(fabs(r) < tau)) continue;
+        const double w = weights ? weights[k] : 1.0;
+        double j[3];
+        if (normals) {
+            const double* nw = normals + 3 * (int64_t)q;
+            double n[3];
+            for (int i = 0; i < 3; ++i) n[i] = T[3 * i] * nw[0] + T[3 * i + 1] * nw[1] + T[3 * i + 2] * nw[2];
+            const double np = n[0] * s.p[0] + n[1] * s.p[1] + n[2] * s.p[2];
+            const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            const double pn = sqrt(s.p[0] * s.p[0] + s.p[1] * s.p[1] + s.p[2] * s.p[2]);
+            if (!(fabs(np) >= 0.1 * nn * pn)) continue;
+            const double c = -s.p[2] / np;
+            for (int i = 0; i < 3; ++i) j[i] = c * n[i];
+        } else {
+            const double dd_du = (1 - s.b) * (s.d10 - s.d00) + s.b * (s.d11 - s.d01);
+            const double dd_dv = (1 - s.a) * (s.d01 - s.d00) + s.a * (s.d11 - s.d10);
+            const double iz = 1.0 / s.p[2];
+            j[0] = dd_du * vol->fx * iz;
+            j[1] = dd_dv * vol->fy * iz;
+            j[2] = -(j[0] * s.p[0] + j[1] * s.p[1]) * iz - 1.0;
+        }
+        const double J[6] = {j[0], j[1], j[2], s.p[1] * j[2] - s.p[2] * j[1],
+                             s.p[2] * j[0] - s.p[0] * j[2], s.p[0] * j[1] - s.p[1] * j[0]};
+        const double c = 2.0 * w / (double)npairs;
+        if (grad)
+            for (int a = 0; a < 6; ++a) grad[6 * (int64_t)f + a] += c * r * J[a];
+        double* o = H + 36 * (int64_t)f;
+        for (int a = 0; a < 6; ++a)
+            for (int b = a; b < 6; ++b) {
+                const double e = c * J[a] * J[b];
+                o[6 * a + b] += e;
+                if (b != a) o[6 * b + a] += e;
+            }
+    }
+}

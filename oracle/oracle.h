@@ -209,6 +209,15 @@ double oracle_frame_loss_f64(const oracle_volume* vol, const float* depth, int32
                              int32_t cols, const double* Rt, int32_t nframes, const double* points,
                              const int32_t* pairs, const double* weights, int64_t npairs,
                              double tau, double* grad_pose, double* residuals);
+/* Gauss-Newton system of that term, (nframes, 6, 6) added to H and
+ * (nframes, 6) to grad (may be NULL): the point-to-plane linearisation with
+ * world normals (one per point), or the bilinear depth slopes when normals is
+ * NULL. */
+void oracle_frame_normal_matrix_f64(const oracle_volume* vol, const float* depth, int32_t rows,
+                                    int32_t cols, const double* Rt, int32_t nframes,
+                                    const double* points, const double* normals,
+                                    const int32_t* pairs, const double* weights, int64_t npairs,
+                                    double tau, double* H, double* grad);
 /* Per-point weights: 1 when no decision of the point's support voxels (frame
  * validity, footprint, the -trunc skip, the +trunc clamp) can change under a
  * perturbation of size <= h of one pose coordinate or one depth tap, and the

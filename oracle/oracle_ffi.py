@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
             "oracle_diff_adjoint_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp]),
             "oracle_frame_loss_f64": (C.c_double, [vp, vp, i32, i32, vp, i32, vp, vp, vp, i64,
                                                    C.c_double, vp, vp]),
+            "oracle_frame_normal_matrix_f64": (None, [vp, vp, i32, i32, vp, i32, vp, vp, vp, vp,
+                                                      i64, C.c_double, vp, vp]),
             "oracle_stable_points": (None, [vp, vp, i32, i32, vp, i32, vp, i64, C.c_double,
                                             C.c_double, vp]),
             "oracle_render_points": (None, [vp, vp, vp, i64, vp, C.c_float, C.c_float, C.c_float,
@@ -398,6 +400,72 @@ def frame_loss(vol, depth, Rt, points, pairs, weights=None, want_grad=True, tau=
                                     _p(np.ascontiguousarray(Rt, np.float64)), F, _p(pts), _p(pr),
                                     _p(w), pr.shape[0], float(tau), _p(gp), _p(res))
     return L, gp, res
+
+
+def frame_normal_matrix(vol, depth, Rt, points, pairs, normals=None, weights=None, tau=0.0,
+                        want_grad=False):
+    """(F, 6, 6) Gauss-Newton normal matrix of the model-to-frame term (and,
+    with want_grad, the (F, 6) gradient of the same linearisation): the
+    point-to-plane linearisation with world `normals`, or the bilinear depth
+    slopes without."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    F, rows, cols = depth.shape
+    pts = np.ascontiguousarray(points, np.float64)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    H = np.zeros((F, 6, 6), np.float64)
+    g = np.zeros((F, 6), np.float64) if want_grad else None
+    lib().oracle_frame_normal_matrix_f64(C.byref(vol), _p(depth), rows, cols,
+                                         _p(np.ascontiguousarray(Rt, np.float64)), F, _p(pts),
+                                         _p(nrm), _p(pr), _p(w), pr.shape[0], float(tau), _p(H),
+                                         _p(g))
+    return (H, g) if want_grad else H
+
+
+def gn_update(Rt, grad, H, step=1.0, damping=0.0):
+    """float32 (F, 12) poses after one damped Gauss-Newton step per frame:
+    M = H + damping diag(H) + 1e-12 max(diag H) I factored M = L L^T column
+    by column, a column whose pivot is at most 1e-9 max(diag M) skipped (its
+    coordinate of x is 0), x = M^-1 grad over the rest, T <- Exp(-step x) T in
+    float64, R's rows re-orthonormalised (Gram-Schmidt, third row from the
+    cross product).  Frames with H = 0 keep their pose.  H is (F, 6, 6)."""
+    Rt = np.asarray(Rt, np.float32)
+    out = Rt.copy()
+    for f in range(Rt.shape[0]):
+        M = np.array(H[f], np.float64)
+        dmax = np.max(np.diag(M))
+        if not dmax > 0.0:
+            continue
+        M[np.diag_indices(6)] = np.diag(M) * (1.0 + damping) + 1e-12 * dmax
+        mmax = np.max(np.diag(M))
+        Lc = np.zeros((6, 6))
+        live = np.zeros(6, bool)
+        for c in range(6):
+            d = M[c, c] - np.dot(Lc[c, :c], Lc[c, :c])
+            live[c] = d > 1e-9 * mmax
+            if live[c]:
+                Lc[c, c] = np.sqrt(d)
+                for r in range(c + 1, 6):
+                    Lc[r, c] = (M[r, c] - np.dot(Lc[r, :c], Lc[c, :c])) / Lc[c, c]
+        g = np.asarray(grad[f], np.float64)
+        y = np.zeros(6)
+        for r in range(6):
+            if live[r]:
+                y[r] = (g[r] - np.dot(Lc[r, :r], y[:r])) / Lc[r, r]
+        x = np.zeros(6)
+        for r in range(5, -1, -1):
+            if live[r]:
+                x[r] = (y[r] - np.dot(Lc[r + 1:, r], x[r + 1:])) / Lc[r, r]
+        T = se3_exp_left(Rt[f].astype(np.float64), -step * x)
+        R = T[:9].reshape(3, 3).copy()
+        R[0] /= np.linalg.norm(R[0])
+        R[1] -= np.dot(R[0], R[1]) * R[0]
+        R[1] /= np.linalg.norm(R[1])
+        R[2] = np.cross(R[0], R[1])
+        out[f, :9] = R.ravel().astype(np.float32)
+        out[f, 9:] = T[9:].astype(np.float32)
+    return out
 
 
 def stable_points(vol, depth, Rt, points, h=1e-4, kink=1e-5):

@@ -396,3 +396,145 @@ def test_frame_pairs_are_grouped_by_frame_and_seen():
     # at the true poses the noise-free residuals of the pairs are small
     _, _, res = O.frame_loss(vol, depth, O.rt_from_poses(w.poses), pts, pairs, want_grad=False)
     assert np.percentile(np.abs(res), 95) < 2e-3
+
+
+# ---- Gauss-Newton refinement: normal matrix and step -------------------------------
+
+def test_frame_normal_matrix_slopes_on_a_linear_depth_map():
+    """Without normals the normal matrix is sum 2 J J^T / K over the valid
+    pairs with |r| < tau, J = (dr/dp, p x dr/dp) from the bilinear slopes; on
+    a depth map affine in pixel coordinates dr/dp has the closed form of the
+    frame-loss test above."""
+    rows, cols = 48, 64
+    al, be, ga = 0.004, -0.003, 1.2
+    jj, ii = np.mgrid[0:rows, 0:cols]
+    depth = (al * ii + be * jj + ga).astype(np.float32)
+    vol = O.volume((8, 8, 8), [0.1] * 3, (0.0, 0.0, 0.0), 0.3, 128.0, (50.0, 55.0, 31.5, 23.5))
+    Rt = O.rt_from_poses(np.array([[1.0, 0, 0, 0, 0, 0, 0]]))
+    rng = np.random.default_rng(32)
+    pts = np.c_[rng.uniform(-0.4, 0.4, 60), rng.uniform(-0.3, 0.3, 60), rng.uniform(1.0, 1.5, 60)]
+    pts = np.r_[pts, [[0.0, 0.0, -1.0]]]                         # behind: adds nothing
+    pairs = np.c_[np.zeros(len(pts)), np.arange(len(pts))].astype(np.int32)
+    _, _, res = O.frame_loss(vol, depth[None], Rt, pts, pairs, want_grad=False)
+    x, y, z = pts[:60].T
+    drdp = np.c_[al * 50.0 / z, be * 55.0 / z, -(al * 50.0 * x + be * 55.0 * y) / z ** 2 - 1]
+    J = np.c_[drdp, np.cross(pts[:60], drdp)]
+    K = len(pts)
+    for tau in (0.0, float(np.median(np.abs(res[:60])))):
+        keep = np.ones(60, bool) if tau == 0.0 else np.abs(res[:60]) < tau
+        assert keep.sum() >= 20
+        want = 2.0 * (J[keep].T @ J[keep]) / K
+        H = O.frame_normal_matrix(vol, depth[None], Rt, pts, pairs, tau=tau)
+        assert H.shape == (1, 6, 6) and np.array_equal(H[0], H[0].T)
+        np.testing.assert_allclose(H[0], want, rtol=1e-4, atol=1e-12 * np.abs(want).max())
+
+
+def _plane_setup(seed=33, n=400):
+    """A plane seen by a rotated, shifted camera: its depth map rendered
+    exactly at the pixel centres, points on the plane (world), its normal."""
+    rows, cols = 120, 160
+    fx, fy, cx, cy = 150.0, 140.0, 79.5, 59.5
+    nc = np.array([0.25, -0.15, -1.0])
+    nc /= np.linalg.norm(nc)
+    c = float(nc @ np.array([0.05, -0.02, 1.3]))          # plane nc . X = c (camera)
+    jj, ii = np.mgrid[0:rows, 0:cols]
+    ray = np.stack([(ii - cx) / fx, (jj - cy) / fy, np.ones_like(ii, float)], -1)
+    depth = (c / (ray @ nc)).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    u, v = rng.uniform(5, cols - 6, n), rng.uniform(5, rows - 6, n)
+    rc = np.c_[(u - cx) / fx, (v - cy) / fy, np.ones(n)]
+    Xc = rc * (c / (rc @ nc))[:, None]
+    R = Rotation.from_rotvec([0.3, -0.2, 0.5]).as_matrix()
+    t = np.array([0.1, -0.05, 0.2])
+    Xw = (Xc - t) @ R                                       # camera -> world
+    nw = R.T @ nc
+    Rt = np.r_[R.ravel(), t][None]
+    vol = O.volume((8, 8, 8), [0.1] * 3, (0.0, 0.0, 0.0), 0.3, 128.0, (fx, fy, cx, cy))
+    pairs = np.c_[np.zeros(n), np.arange(n)].astype(np.int32)
+
+    def residual(T, i):
+        """r of point i under pose T against the exact plane (no sampling)."""
+        p = T[:9].reshape(3, 3) @ Xw[i] + T[9:]
+        uu, vv = p[0] * fx / p[2] + cx, p[1] * fy / p[2] + cy
+        r_ = np.array([(uu - cx) / fx, (vv - cy) / fy, 1.0])
+        return c / (r_ @ nc) - p[2]
+
+    return vol, depth, Rt, Xw, np.tile(nw, (n, 1)), pairs, residual
+
+
+def test_frame_normal_matrix_point_to_plane_on_a_plane():
+    """With normals, J = (j, p x j), j = -(z / (n . p)) n: on a plane this is
+    the exact derivative of r = d(s) - z for points on it, so the normal
+    matrix equals sum 2 J J^T / K with J from central differences of the
+    exact plane residual (world normals rotated into the camera)."""
+    vol, depth, Rt, Xw, nw, pairs, residual = _plane_setup()
+    h = 1e-6
+    J = np.zeros((len(Xw), 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        Tp, Tm = O.se3_exp_left(Rt[0], e), O.se3_exp_left(Rt[0], -e)
+        J[:, k] = [(residual(Tp, i) - residual(Tm, i)) / (2 * h) for i in range(len(Xw))]
+    want = 2.0 * (J.T @ J) / len(Xw)
+    H = O.frame_normal_matrix(vol, depth[None], Rt, Xw, pairs, normals=nw)
+    np.testing.assert_allclose(H[0], want, rtol=1e-6, atol=1e-8 * np.abs(want).max())
+    # its gradient is sum 2 r J / K with the sampled residuals, at a pose off
+    # the plane's fit; without normals it is the exact (slope) gradient
+    off = O.se3_exp_left(Rt[0], np.array([0.002, -0.001, 0.002, 1e-3, -1.5e-3, 5e-4]))[None]
+    _, g_exact, r = O.frame_loss(vol, depth[None], off, Xw, pairs)
+    Hn, gn = O.frame_normal_matrix(vol, depth[None], off, Xw, pairs, normals=nw, want_grad=True)
+    pc = Xw @ off[0, :9].reshape(3, 3).T + off[0, 9:]
+    nc = nw @ off[0, :9].reshape(3, 3).T
+    j = -(pc[:, 2] / np.sum(nc * pc, 1))[:, None] * nc
+    Jo = np.c_[j, np.cross(pc, j)]
+    np.testing.assert_allclose(gn[0], 2.0 * (r[:, None] * Jo).sum(0) / len(Xw), rtol=1e-9,
+                               atol=1e-12 * np.abs(gn).max())
+    np.testing.assert_allclose(Hn[0], 2.0 * Jo.T @ Jo / len(Xw), rtol=1e-9,
+                               atol=1e-12 * np.abs(Hn).max())
+    _, gs = O.frame_normal_matrix(vol, depth[None], off, Xw, pairs, want_grad=True)
+    np.testing.assert_allclose(gs[0], g_exact[0], rtol=1e-12, atol=1e-15 * np.abs(g_exact).max())
+    # on a plane the two linearisations agree to first order in the
+    # residual (millimetres against a 1.3 m depth)
+    np.testing.assert_allclose(gn[0], g_exact[0], rtol=0, atol=1e-2 * np.abs(g_exact).max())
+    # a plane fixes three coordinates (the shift along n, two tilts)
+    ev = np.linalg.eigvalsh(H[0])
+    assert ev[3] > 1e-3 * ev[5] and ev[2] < 1e-12 * ev[5]
+    # a flipped normal gives the same matrix; grazing pairs add nothing
+    np.testing.assert_allclose(O.frame_normal_matrix(vol, depth[None], Rt, Xw, pairs, normals=-nw)[0],
+                               H[0], rtol=1e-12, atol=0)
+    side = np.cross(nw[0], [1.0, 0.0, 0.0])
+    side = np.tile(side / np.linalg.norm(side), (len(Xw), 1))
+    Hs = O.frame_normal_matrix(vol, depth[None], Rt, Xw, pairs, normals=side)
+    pc = Xw @ Rt[0, :9].reshape(3, 3).T + Rt[0, 9:]
+    ns = side @ Rt[0, :9].reshape(3, 3).T
+    seen = np.abs(np.sum(ns * pc, 1)) >= 0.1 * np.linalg.norm(pc, axis=1)
+    j = -(pc[:, 2] / np.sum(ns * pc, 1))[:, None] * ns
+    Js = np.c_[j, np.cross(pc, j)][seen]
+    np.testing.assert_allclose(Hs[0], 2.0 * Js.T @ Js / len(Xw), rtol=1e-9,
+                               atol=1e-12 * np.abs(Hs).max())
+    assert 0 < seen.sum() < len(Xw)
+
+
+def test_gn_update_fits_a_plane_in_one_step():
+    """One full Gauss-Newton step (step 1, no damping) from a pose 3 mm and
+    0.1 degree off brings the residuals of points on a plane to nearly zero
+    (the residual is linear in xi up to second order) without moving the
+    pose much further than its observable error (the coordinates a plane
+    leaves free take no step), and H = 0 leaves a pose unchanged."""
+    vol, depth, Rt, Xw, nw, pairs, _ = _plane_setup(seed=34)
+    off = O.se3_exp_left(Rt[0], np.array([0.002, -0.001, 0.002, 1e-3, -1.5e-3, 5e-4]))[None]
+    L0, g, r0 = O.frame_loss(vol, depth[None], off, Xw, pairs)
+    H = O.frame_normal_matrix(vol, depth[None], off, Xw, pairs, normals=nw)
+    new = O.gn_update(off, g, H, step=1.0, damping=0.0)
+    L1, _, r1 = O.frame_loss(vol, depth[None], new.astype(np.float64), Xw, pairs)
+    assert np.abs(r0).max() > 2e-3
+    assert np.abs(r1).max() < 2e-2 * np.abs(r0).max(), (np.abs(r0).max(), np.abs(r1).max())
+    # the pose moved by about the size of the observable error, not more
+    a, c = scenes.pose_errors(new.astype(np.float64), off)
+    assert a[0] < 0.3 and c[0] < 0.02
+    # half a step halves the residuals' observable part
+    half = O.gn_update(off, g, H, step=0.5, damping=0.0)
+    _, _, rh = O.frame_loss(vol, depth[None], half.astype(np.float64), Xw, pairs)
+    np.testing.assert_allclose(rh, 0.5 * r0, atol=0.03 * np.abs(r0).max())
+    same = O.gn_update(off, g, np.zeros((1, 6, 6)))
+    assert np.array_equal(same, off.astype(np.float32))
