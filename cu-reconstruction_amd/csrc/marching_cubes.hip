@@ -11,15 +11,21 @@
 //                    the corners' sign bits, its triangle count from a
 //                    table, and per 64-cube chunk of each row the chunk's
 //                    triangle count; per workgroup ("tile", 16 cube rows)
-//                    the tile's total;
-//   2. tile scan   - one workgroup: exclusive scan of the tile totals, each
-//                    tile's first triangle and the volume's total;
-//   3. emit        - one workgroup per tile: the tile's chunks with
-//                    triangles are compacted in LDS and taken four at a
-//                    time by a wavefront (16 lanes per chunk, 4 cubes per
-//                    lane), whose triangles are spread over its 64 lanes
-//                    to compute and write the vertices
-//                    (marching_cubes.cu:23-74).
+//                    the tile's total and its number of chunks with
+//                    triangles;
+//   2. tile scan   - one workgroup: exclusive scans of the tile totals
+//                    (each tile's first triangle and the volume's total)
+//                    and of the tiles' groups of four chunks with
+//                    triangles;
+//   3. emit        - a grid sized to the resident workgroups: each
+//                    wavefront takes groups gi = wave, wave + waves, ...,
+//                    finds each group's tile and chunks, and spreads their
+//                    triangles (16 lanes per chunk, 4 cubes per lane) over
+//                    its 64 lanes to compute the vertices
+//                    (marching_cubes.cu:23-74), staged in LDS and written in
+//                    consecutive dwords.  A tile dense with triangles (a
+//                    floor along a cube layer) is thus shared by many
+//                    wavefronts.
 // Output order is cube z, y, x, table slot -- the order the reference's host
 // loop produces (marching_cubes.cu:163-188 inside :213-225) -- as a triangle
 // soup: triangle t owns vertices 3t, 3t+1, 3t+2 (marching_cubes.cu:171).
@@ -47,45 +53,117 @@ __device__ __forceinline__ long long wave_inclusive_scan64(long long v, int lane
     return v;
 }
 
-// One workgroup of 1024 threads: tile_off[i] = sum of tile_sum[0..i), and
-// tile_off[n] = the total.
+// One workgroup of 1024 threads, 16 consecutive tiles per thread (16-byte
+// loads and stores): tile_off[i] = sum of tile_sum[0..i) and tile_off[n] =
+// the total; when tile_nne is not null also group_off[i] = sum over tiles
+// before i of ceil(tile_nne / 4) (the emit's four-chunk groups) and
+// group_off[n] = their total.  All four arrays are 16-byte aligned.
 __global__ __launch_bounds__(1024) void mc_tile_scan_kernel(const int32_t* __restrict__ tile_sum,
-                                                            int n, long long* __restrict__ tile_off) {
+                                                            const int32_t* __restrict__ tile_nne,
+                                                            int n, long long* __restrict__ tile_off,
+                                                            int32_t* __restrict__ group_off) {
     __shared__ long long wtot[16];
+    __shared__ int gtot[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     long long carry = 0;
+    int gcarry = 0;
     for (int base = 0; base < n; base += 1024 * kTileScanItems) {
         const int i0 = base + (int)threadIdx.x * kTileScanItems;
-        int v[kTileScanItems];
+        const bool whole = i0 + kTileScanItems <= n;
+        int v[kTileScanItems], g[kTileScanItems];
+        if (whole) {
+            const int4* s4 = reinterpret_cast<const int4*>(tile_sum + i0);
+#pragma unroll
+            for (int q = 0; q < kTileScanItems / 4; ++q) {
+                const int4 a = s4[q];
+                v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+            }
+            if (tile_nne) {
+                const int4* n4 = reinterpret_cast<const int4*>(tile_nne + i0);
+#pragma unroll
+                for (int q = 0; q < kTileScanItems / 4; ++q) {
+                    const int4 a = n4[q];
+                    g[4 * q] = (a.x + 3) >> 2; g[4 * q + 1] = (a.y + 3) >> 2;
+                    g[4 * q + 2] = (a.z + 3) >> 2; g[4 * q + 3] = (a.w + 3) >> 2;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kTileScanItems; ++j) {
+                v[j] = i0 + j < n ? tile_sum[i0 + j] : 0;
+                g[j] = (tile_nne && i0 + j < n) ? (tile_nne[i0 + j] + 3) >> 2 : 0;
+            }
+        }
+        if (!tile_nne) {
+#pragma unroll
+            for (int j = 0; j < kTileScanItems; ++j) g[j] = 0;
+        }
         long long sum = 0;
+        int gsum = 0;
 #pragma unroll
         for (int j = 0; j < kTileScanItems; ++j) {
-            v[j] = i0 + j < n ? tile_sum[i0 + j] : 0;
             sum += v[j];
+            gsum += g[j];
         }
         const long long incl = wave_inclusive_scan64(sum, lane);
-        if (lane == 63) wtot[wave] = incl;
+        const int ginc = wave_inclusive_scan(gsum, lane);
+        if (lane == 63) {
+            wtot[wave] = incl;
+            gtot[wave] = ginc;
+        }
         __syncthreads();
         long long before = carry, all = 0;
+        int gbefore = gcarry, gall = 0;
 #pragma unroll
         for (int w = 0; w < 16; ++w) {
             before += w < wave ? wtot[w] : 0;
             all += wtot[w];
+            gbefore += w < wave ? gtot[w] : 0;
+            gall += gtot[w];
         }
         long long ex = before + incl - sum;
+        int gex = gbefore + ginc - gsum;
+        long long o[kTileScanItems];
+        int go[kTileScanItems];
 #pragma unroll
         for (int j = 0; j < kTileScanItems; ++j) {
-            if (i0 + j < n) tile_off[i0 + j] = ex;
+            o[j] = ex;
             ex += v[j];
+            go[j] = gex;
+            gex += g[j];
+        }
+        if (whole) {
+            longlong2* d2 = reinterpret_cast<longlong2*>(tile_off + i0);
+#pragma unroll
+            for (int q = 0; q < kTileScanItems / 2; ++q) d2[q] = make_longlong2(o[2 * q], o[2 * q + 1]);
+            if (group_off) {
+                int4* g4 = reinterpret_cast<int4*>(group_off + i0);
+#pragma unroll
+                for (int q = 0; q < kTileScanItems / 4; ++q)
+                    g4[q] = make_int4(go[4 * q], go[4 * q + 1], go[4 * q + 2], go[4 * q + 3]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kTileScanItems; ++j)
+                if (i0 + j < n) {
+                    tile_off[i0 + j] = o[j];
+                    if (group_off) group_off[i0 + j] = go[j];
+                }
         }
         carry += all;
+        gcarry += gall;
         __syncthreads();
     }
-    if (threadIdx.x == 0) tile_off[n] = carry;
+    if (threadIdx.x == 0) {
+        tile_off[n] = carry;
+        if (group_off) group_off[n] = gcarry;
+    }
 }
 
-int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStream_t s) {
-    hipLaunchKernelGGL(mc_tile_scan_kernel, dim3(1), dim3(1024), 0, s, tile_sum, n, tile_off);
+int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStream_t s,
+                     const int32_t* tile_nne, int32_t* group_off) {
+    hipLaunchKernelGGL(mc_tile_scan_kernel, dim3(1), dim3(1024), 0, s, tile_sum, tile_nne, n,
+                       tile_off, group_off);
     RECON_HIP_TRY(hipGetLastError(), "mc_tile_scan_kernel launch");
     return RECON_OK;
 }
@@ -93,29 +171,36 @@ int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStr
 // ---- single-volume kernels ------------------------------------------------------
 __global__ __launch_bounds__(256) void mc_chunk_count_kernel(const McParams p, const McTiling tl,
                                                              uint16_t* __restrict__ cc,
-                                                             int32_t* __restrict__ tile_sum) {
+                                                             int32_t* __restrict__ tile_sum,
+                                                             int32_t* __restrict__ tile_nne) {
     __shared__ uint8_t ntab[256];
-    __shared__ int wtot[4];
+    __shared__ int wtot[8];
     ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
     __syncthreads();
-    mc_count_tile(p, tl, blockIdx.x, cc, ntab, wtot, tile_sum + blockIdx.x);
+    mc_count_tile(p, tl, blockIdx.x, cc, ntab, wtot, tile_sum + blockIdx.x, tile_nne + blockIdx.x);
 }
 
+// A grid sized to the resident workgroups walks the volume's groups of four
+// chunks with triangles, wavefront by wavefront (gi = wave, wave + waves,
+// ...): the groups of a tile dense with triangles (a floor along a cube
+// layer) go to as many wavefronts.
 __global__ __launch_bounds__(256) void mc_chunk_emit_kernel(const McParams p, const McTiling tl,
                                                             const uint16_t* __restrict__ cc,
                                                             const long long* __restrict__ tile_off,
+                                                            const int32_t* __restrict__ tile_nne,
+                                                            const int32_t* __restrict__ group_off,
                                                             float* __restrict__ verts,
                                                             int32_t* __restrict__ tris,
                                                             long long capacity) {
-    __shared__ uint8_t ntab[256];
-    __shared__ int lchunk[256];
-    __shared__ int loff[256];
-    __shared__ int wtot[4];
-    const int b = blockIdx.x;
-    const long long base = tile_off[b];
-    if (tile_off[b + 1] == base) return;  // the same for every thread
-    ntab[threadIdx.x] = (uint8_t)class_ntri((int)threadIdx.x);
-    mc_emit_tile(p, tl, b, cc, base, ntab, lchunk, loff, wtot, verts, tris, capacity);
+    __shared__ McEmitLds lds;
+    mc_emit_lds_init(lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ngroups = group_off[tl.tiles];
+    const int waves = (int)gridDim.x * 4;
+    for (int gi = (int)blockIdx.x * 4 + wave; gi < ngroups; gi += waves)
+        mc_emit_group_at(p, tl, gi, cc, tile_off, tile_nne, group_off, lds, wave, lane, verts,
+                         tris, capacity);
 }
 
 __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
@@ -235,7 +320,7 @@ static int check_grid(const float* field, const int32_t grid[3], const float vs[
 struct McScratchLayout {
     McTiling t{0, 0, 0, 0};
     int gpl = 0, groups = 0, tiles = 0, nch = 0;
-    size_t off_sum = 0, off_off = 0, need = 0;
+    size_t off_sum = 0, off_nne = 0, off_off = 0, off_goff = 0, need = 0;
     explicit McScratchLayout(const int32_t grid[3]) {
         if (grid[0] < 2 || grid[1] < 2 || grid[2] < 2) return;
         t = mc_tiling(grid[0], grid[1], grid[2]);
@@ -244,11 +329,36 @@ struct McScratchLayout {
         tiles = t.tiles;
         nch = t.nch;
         const size_t rows = (size_t)(grid[1] - 1) * (size_t)(grid[2] - 1);
-        off_sum = ((2 * rows * (size_t)nch + 255) / 256) * 256;
-        off_off = off_sum + ((4 * (size_t)tiles + 255) / 256) * 256;
-        need = off_off + 8 * ((size_t)tiles + 1);
+        auto a256 = [](size_t b) { return (b + 255) / 256 * 256; };
+        off_sum = a256(2 * rows * (size_t)nch);
+        off_nne = off_sum + a256(4 * (size_t)tiles);
+        off_off = off_nne + a256(4 * (size_t)tiles);
+        off_goff = off_off + a256(8 * ((size_t)tiles + 1));
+        need = off_goff + 4 * ((size_t)tiles + 1);
     }
 };
+
+// Workgroups of the emit pass: its resident slots on every CU of the current
+// device (looked up once per device).
+static int emit_grid() {
+    static std::mutex mu;
+    static int grid[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!grid[dev]) {
+        int cus = 0, occ = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus < 1)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_chunk_emit_kernel, 256, 0) !=
+                hipSuccess ||
+            occ < 1)
+            occ = 1;
+        grid[dev] = cus * occ;
+    }
+    return grid[dev];
+}
 
 size_t mc_scratch_need(const int32_t grid[3]) { return McScratchLayout(grid).need; }
 
@@ -274,18 +384,21 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     char* sb = static_cast<char*>(*scratch);
     uint16_t* cc = reinterpret_cast<uint16_t*>(sb);
     int32_t* tile_sum = reinterpret_cast<int32_t*>(sb + L.off_sum);
+    int32_t* tile_nne = reinterpret_cast<int32_t*>(sb + L.off_nne);
     long long* tile_off = reinterpret_cast<long long*>(sb + L.off_off);
-    hipLaunchKernelGGL(mc_chunk_count_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc, tile_sum);
+    int32_t* group_off = reinterpret_cast<int32_t*>(sb + L.off_goff);
+    hipLaunchKernelGGL(mc_chunk_count_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc, tile_sum,
+                       tile_nne);
     RECON_HIP_TRY(hipGetLastError(), "mc_chunk_count_kernel launch");
-    int rc = launch_tile_scan(tile_sum, L.tiles, tile_off, s);
+    int rc = launch_tile_scan(tile_sum, L.tiles, tile_off, s, tile_nne, group_off);
     if (rc) return rc;
     long long host_total = 0;
     RECON_HIP_TRY(hipMemcpyAsync(&host_total, tile_off + L.tiles, sizeof(long long),
                                  hipMemcpyDeviceToHost, s),
                   "marching cubes total");
     if (!count_only && capacity > 0 && verts) {
-        hipLaunchKernelGGL(mc_chunk_emit_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc,
-                           tile_off, verts, tris, (long long)capacity);
+        hipLaunchKernelGGL(mc_chunk_emit_kernel, dim3(emit_grid()), dim3(256), 0, s, p, L.t, cc,
+                           tile_off, tile_nne, group_off, verts, tris, (long long)capacity);
         RECON_HIP_TRY(hipGetLastError(), "mc_chunk_emit_kernel launch");
     }
     RECON_HIP_TRY(hipStreamSynchronize(s), "marching cubes synchronize");
@@ -374,8 +487,9 @@ extern "C" int recon_mc_extract(const float* field, const int32_t grid[3],
 
 // ---- marching cubes without allocation ---------------------------------------
 // Scratch of recon_mc_extract_noalloc: the chunk counts (2 B per 64-cube
-// chunk of every cube row) and the tile sums and offsets (12 B per 16 cube
-// rows) -- mc_scratch_need bytes -- in one static array that each GPU holds
+// chunk of every cube row) and the tile sums, chunk counts, triangle and
+// group offsets (20 B per 16 cube rows) -- mc_scratch_need bytes -- in one
+// static array that each GPU holds
 // for the life of the process.  48 MiB covers a 1024^3 grid (1023^2 cube
 // rows of 16 chunks: 34.6 MB).
 constexpr size_t kMcNoAllocScratchBytes = size_t(48) << 20;

@@ -15,6 +15,16 @@ namespace recon {
 static __constant__ uint64_t c_tri[256] = RECON_MC_TRI_PACKED_INIT;
 static __constant__ int c_edge[12][2] = RECON_MC_EDGE_CORNERS_INIT;
 
+// The edge table's corners packed 4 bits per edge into two immediates, so
+// that a lane's edge lookup is a shift, not a load.
+constexpr int kEdgeCorners[12][2] = RECON_MC_EDGE_CORNERS_INIT;
+constexpr uint64_t pack_edge_corner(int w) {
+    uint64_t m = 0;
+    for (int e = 0; e < 12; ++e) m |= (uint64_t)kEdgeCorners[e][w] << (4 * e);
+    return m;
+}
+constexpr uint64_t kEdgeA = pack_edge_corner(0), kEdgeB = pack_edge_corner(1);
+
 struct McParams {
     const float* f;
     int32_t gx, gy, gz;
@@ -240,7 +250,7 @@ __device__ __forceinline__ int mc_row_triangles(const McParams& p, int r, int la
 // p = s + ((-ws) / (we - ws)) * (e - s), component by component.
 __device__ __forceinline__ void edge_vertex(const McParams& p, int x, int y, int z,
                                             const float v[8], int e, float out[3]) {
-    int a = c_edge[e][0], b = c_edge[e][1];
+    int a = (int)((kEdgeA >> (4 * e)) & 0xFull), b = (int)((kEdgeB >> (4 * e)) & 0xFull);
     float wa = v[a], wb = v[b];
     if ((wa > 0.0f) && (wb < 0.0f)) {
         const int ti = a; a = b; b = ti;
@@ -401,23 +411,26 @@ __device__ __forceinline__ uint32_t mc_sign_bits(const float v[kVoxelRows]) {
 }
 
 // Triangles of each 64-cube chunk of the tile's cube rows into cc (chunk c
-// of row r at r*nch + c), and the tile's total into *tile_sum_out.  The chunk's voxel
+// of row r at r*nch + c), the tile's total into *tile_sum_out and, when
+// tile_nne_out is not null, its number of chunks with triangles into
+// *tile_nne_out.  The chunk's voxel
 // rows are loaded two chunks ahead; a cube's x+1 corners are the next lane's
 // sign bits (a DPP wave shift), lane 63's from the following chunk.  Loads
 // past the row end read its last voxel, and rows past the layer's last
 // voxel row read that row (both in bounds); their cubes are not counted.
 // Tile b of a volume (a 256-thread workgroup; every thread calls).  ntab:
 // the workgroup's LDS table of triangle counts by packed-word byte
-// (mc_ntab_qbyte, filled before the call); wtot: 4 ints of LDS.
+// (mc_ntab_qbyte, filled before the call); wtot: 8 ints of LDS.
 __device__ __forceinline__ void mc_count_tile(const McParams& p, const McTiling& tl, const int b,
                                               uint16_t* __restrict__ cc,
                                               const uint8_t* __restrict__ ntab, int* wtot,
-                                              int32_t* __restrict__ tile_sum_out) {
+                                              int32_t* __restrict__ tile_sum_out,
+                                              int32_t* __restrict__ tile_nne_out) {
     const int gpl = tl.gpl, groups = tl.groups, nch = tl.nch;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int q = b * kTileGroups + wave;
-    int total = 0;
+    int total = 0, nne = 0;
     if (q < groups) {
         int y, z, k;
         mc_group(p, gpl, q, y, z, k);
@@ -463,7 +476,9 @@ __device__ __forceinline__ void mc_count_tile(const McParams& p, const McTiling&
             }
             if (lane < k) {
                 const int w = lane < 2 ? cnt01 : cnt23;
-                ccr[(size_t)lane * nch + c] = (uint16_t)((lane & 1) ? (w >> 16) : (w & 0xFFFF));
+                const int rowc = (lane & 1) ? (w >> 16) : (w & 0xFFFF);
+                ccr[(size_t)lane * nch + c] = (uint16_t)rowc;
+                nne += rowc > 0 ? 1 : 0;
             }
         };
         // two buffers, each chunk's loads issued two chunks ahead
@@ -485,22 +500,49 @@ __device__ __forceinline__ void mc_count_tile(const McParams& p, const McTiling&
         }
     }
     total = wave_sum(total);
-    if (lane == 0) wtot[threadIdx.x >> 6] = total;
+    nne = wave_sum(nne);
+    if (lane == 0) {
+        wtot[threadIdx.x >> 6] = total;
+        wtot[4 + (threadIdx.x >> 6)] = nne;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) *tile_sum_out = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    if (threadIdx.x == 0) {
+        *tile_sum_out = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        if (tile_nne_out) *tile_nne_out = wtot[4] + wtot[5] + wtot[6] + wtot[7];
+    }
 }
 
 // ---- emit -----------------------------------------------------------------------
+// Per-workgroup LDS of the emit pass.
+struct McEmitLds {
+    uint64_t tri[256];              // c_tri
+    int lchunk[256], loff[256];     // compacted chunks with triangles, their first triangle
+    int wtot[4];
+    uint8_t ntab[256];              // triangles by class
+    float stage[4][64 * 9];         // per wavefront: 64 triangles' vertices
+    int gchunk[4][4], goff[4][4];   // per wavefront: its group's chunks, first triangles
+};
+
+// Fills the tables of an emit workgroup (256 threads); the first barrier of
+// mc_emit_tile orders them before their use.
+__device__ __forceinline__ void mc_emit_lds_init(McEmitLds& l) {
+    l.tri[threadIdx.x] = c_tri[threadIdx.x];
+    l.ntab[threadIdx.x] = (uint8_t)class_ntri((int)threadIdx.x);
+}
+
 // Writes the triangles of up to four chunks (list entries e0 .. e0+ne-1 of
 // the tile's compacted list, consecutive in the output from triangle t0) by
 // one wavefront: lane l takes cubes 4(l%16) .. +3 of chunk l/16, so the
 // lanes' cubes are in output order; a wave-level prefix sum places each
-// lane's triangles, and triangle k of the group is written by lane k % 64,
-// which finds its cube (wave_owner, then among the owner's four), reloads
-// the cube's corners and computes the vertices (edge_vertex).
+// lane's triangles, and triangle k of the group is computed by lane k % 64,
+// which finds its cube (wave_owner, then among the owner's four), takes the
+// owner's 20 voxel values (wave shuffles; the cube's 8 corners are among
+// them) and computes the vertices (edge_vertex) into an LDS stage; each 64
+// triangles' vertices and indices then leave in consecutive dwords.
 __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const int* lchunk,
                                               int ne, long long t0, int lane,
-                                              const uint8_t* ntab, float* __restrict__ verts,
+                                              const McEmitLds& l, float* lstage,
+                                              float* __restrict__ verts,
                                               int32_t* __restrict__ tris, long long capacity) {
     const int slot4 = lane >> 4;
     const bool act = slot4 < ne;
@@ -525,7 +567,7 @@ __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const 
         const float cv[8] = {v[0][j], v[0][j + 1], v[1][j + 1], v[1][j],
                              v[2][j], v[2][j + 1], v[3][j + 1], v[3][j]};
         cls[j] = cube_class(cv);
-        nt[j] = (act && x + j < p.cx) ? (int)ntab[cls[j]] : 0;
+        nt[j] = (act && x + j < p.cx) ? (int)l.ntab[cls[j]] : 0;
     }
     const int c1 = nt[0], c2 = c1 + nt[1], c3 = c2 + nt[2];
     const int lt = c3 + nt[3];
@@ -534,6 +576,7 @@ __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const 
     const int gtotal = __builtin_amdgcn_readlane(incl, 63);
     const int cw = cls[0] | (cls[1] << 8) | (cls[2] << 16) | (cls[3] << 24);
     const int cp = c1 | (c2 << 8) | (c3 << 16);  // each <= 15
+    __builtin_amdgcn_wave_barrier();
     for (int k0 = 0; k0 < gtotal; k0 += 64) {
         const int k = k0 + lane;
         const int owner = wave_owner(incl, k);
@@ -548,37 +591,60 @@ __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const 
         const int first = j == 0 ? 0 : (j == 1 ? o1 : (j == 2 ? o2 : o3));
         const int slot = local - first;
         const int oc = (ocw >> (8 * j)) & 0xFF;
-        const long long t = t0 + k;
-        if (k < gtotal && t < capacity) {
+        // the owner's voxel values: its cube j's corners are ov[..][j], ov[..][j+1]
+        float ov[4][5];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) ov[rr][q] = __shfl(v[rr][q], owner, 64);
+        if (k < gtotal) {
             const int cx = ox + j, cy = orow % p.cyr, cz = orow / p.cyr;
-            float cv[8];
-            load_corners(p, cx, cy, cz, cv);
-            const uint64_t tw = c_tri[oc];
+            // load_corners' values (corners 0..7 as in the classification)
+            float lo[4], hi[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                lo[rr] = j == 0 ? ov[rr][0] : (j == 1 ? ov[rr][1] : (j == 2 ? ov[rr][2] : ov[rr][3]));
+                hi[rr] = j == 0 ? ov[rr][1] : (j == 1 ? ov[rr][2] : (j == 2 ? ov[rr][3] : ov[rr][4]));
+            }
+            const float cv[8] = {lo[0], hi[0], hi[1], lo[1], lo[2], hi[2], hi[3], lo[3]};
+            const uint64_t tw = l.tri[oc];
 #pragma unroll
             for (int qv = 0; qv < 3; ++qv) {
                 const int e = (int)((tw >> (4 + 4 * (3 * slot + qv))) & 0xFull);
                 float pt[3];
                 edge_vertex(p, cx, cy, cz, cv, e, pt);
-                float* dst = verts + 9 * t + 3 * qv;
-                dst[0] = pt[0];
-                dst[1] = pt[1];
-                dst[2] = pt[2];
-                if (tris) tris[3 * t + qv] = (int32_t)(3 * t + qv);
+                lstage[9 * lane + 3 * qv + 0] = pt[0];
+                lstage[9 * lane + 3 * qv + 1] = pt[1];
+                lstage[9 * lane + 3 * qv + 2] = pt[2];
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        // triangles t0+k0 .. : 9 floats and 3 indices each, below capacity
+        const long long tb = t0 + k0;
+        const long long n = min((long long)min(64, gtotal - k0), capacity - tb);
+        if (n > 0) {
+            const int nw = 9 * (int)n;
+            float* __restrict__ dst = verts + 9 * tb;
+            for (int w = lane; w < nw; w += 64) dst[w] = lstage[w];
+            if (tris) {
+                int32_t* __restrict__ td = tris + 3 * tb;
+                const int n3 = 3 * (int)n;
+                for (int w = lane; w < n3; w += 64) td[w] = (int32_t)(3 * tb + w);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 // Tile b of a volume, whose triangles start at `base` (a 256-thread
 // workgroup; every thread calls): its chunks with triangles, 256 at a time,
 // are compacted in LDS (chunk index and first triangle in the tile, in
-// order), then emitted four per wavefront.  ntab: LDS triangle counts by
-// class, filled by the caller (no barrier needed before the call: the first
-// use follows one); lchunk, loff: 256 ints of LDS each; wtot: 4 ints.
+// order), then emitted four per wavefront.  l: the workgroup's LDS, its
+// tables filled by mc_emit_lds_init (no barrier needed before the call: the
+// first use follows one).
 __device__ __forceinline__ void mc_emit_tile(const McParams& p, const McTiling& tl, const int b,
                                              const uint16_t* __restrict__ cc, const long long base,
-                                             const uint8_t* ntab, int* lchunk, int* loff,
-                                             int* wtot, float* __restrict__ verts,
+                                             McEmitLds& l, float* __restrict__ verts,
                                              int32_t* __restrict__ tris, long long capacity) {
     const int gpl = tl.gpl, groups = tl.groups, nch = tl.nch;
     const int lane = threadIdx.x & 63;
@@ -596,25 +662,88 @@ __device__ __forceinline__ void mc_emit_tile(const McParams& p, const McTiling& 
         // packed: triangles (<= 256 * 320 < 2^20) and chunks with triangles
         const int pk = cnt | (cnt > 0 ? (1 << 20) : 0);
         const int incl = wave_inclusive_scan(pk, lane);
-        if (lane == 63) wtot[wave] = incl;
+        if (lane == 63) l.wtot[wave] = incl;
         __syncthreads();
         int before = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) before += w < wave ? wtot[w] : 0;
-        const int all = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        for (int w = 0; w < 4; ++w) before += w < wave ? l.wtot[w] : 0;
+        const int all = l.wtot[0] + l.wtot[1] + l.wtot[2] + l.wtot[3];
         const int ex = before + incl - pk;
         if (cnt > 0) {
-            lchunk[ex >> 20] = ci;
-            loff[ex >> 20] = woff + (ex & 0xFFFFF);
+            l.lchunk[ex >> 20] = ci;
+            l.loff[ex >> 20] = woff + (ex & 0xFFFFF);
         }
         __syncthreads();
         const int nne = all >> 20;
         for (int g = 4 * wave; g < nne; g += 16)
-            mc_emit_group(p, nch, lchunk + g, min(4, nne - g), base + loff[g], lane, ntab, verts,
-                          tris, capacity);
+            mc_emit_group(p, nch, l.lchunk + g, min(4, nne - g), base + l.loff[g], lane, l,
+                          l.stage[wave], verts, tris, capacity);
         woff += all & 0xFFFFF;
         __syncthreads();
     }
+}
+
+// The last tile b in [lo, hi] with group_off[b] <= gi (group_off[lo] <= gi
+// given), for one wavefront: each round probes 64 evenly spaced tiles at
+// once, so 16384 tiles take three rounds of loads.
+__device__ __forceinline__ int mc_find_tile(const int32_t* __restrict__ group_off, int lo, int hi,
+                                            int gi, int lane) {
+    while (hi > lo) {
+        const int step = (hi - lo + 64) / 64;  // ceil((hi - lo + 1) / 64)
+        const int idx = lo + lane * step;
+        const bool le = idx <= hi && group_off[idx] <= gi;
+        const unsigned long long m = __ballot(le);  // bit 0 is set
+        const int last = 63 - __builtin_clzll(m);
+        lo = lo + last * step;
+        hi = min(hi, lo + step - 1);
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// Group gi of the volume for one wavefront (gi the same for every lane).
+// Groups are runs of four consecutive chunks with triangles within one tile,
+// tile by tile: group_off[b] is tile b's first group, group_off[ntiles]
+// their total, tile_nne[b] its chunks with triangles.  The tile comes from
+// mc_find_tile; the group's chunks and first triangles from a walk over the
+// tile's chunk counts, 64 at a time (ballot of the chunks with triangles,
+// prefix sums of their counts); then mc_emit_group.
+__device__ __forceinline__ void mc_emit_group_at(const McParams& p, const McTiling& tl, int gi,
+                                                 const uint16_t* __restrict__ cc,
+                                                 const long long* __restrict__ tile_off,
+                                                 const int32_t* __restrict__ tile_nne,
+                                                 const int32_t* __restrict__ group_off,
+                                                 McEmitLds& l, int wave, int lane,
+                                                 float* __restrict__ verts,
+                                                 int32_t* __restrict__ tris, long long capacity) {
+    const int b = mc_find_tile(group_off, 0, tl.tiles - 1, gi, lane);
+    const int qa = b * kTileGroups, qb = min(qa + kTileGroups, tl.groups) - 1;
+    int ya, za, ka, yb, zb, kb;
+    mc_group(p, tl.gpl, qa, ya, za, ka);
+    mc_group(p, tl.gpl, qb, yb, zb, kb);
+    const int c_begin = (ya + p.cyr * za) * tl.nch;
+    const int c_end = (yb + p.cyr * zb + kb) * tl.nch;
+    const int first = 4 * (gi - __builtin_amdgcn_readfirstlane(group_off[b]));
+    const int want = min(4, __builtin_amdgcn_readfirstlane(tile_nne[b]) - first);
+    int found = 0, ne_before = 0, tri_before = 0;
+    for (int w0 = c_begin; w0 < c_end && found < want; w0 += 64) {
+        const int ci = w0 + lane;
+        const int cnt = ci < c_end ? (int)cc[ci] : 0;
+        const unsigned long long m = __ballot(cnt > 0);
+        const int rank = ne_before + (int)__popcll(m & ((1ull << lane) - 1ull));
+        const int incl = wave_inclusive_scan(cnt, lane);
+        if (cnt > 0 && rank >= first && rank < first + 4) {
+            l.gchunk[wave][rank - first] = ci;
+            l.goff[wave][rank - first] = tri_before + incl - cnt;
+        }
+        const int ne_here = (int)__popcll(m);
+        found += max(0, min(first + 4, ne_before + ne_here) - max(first, ne_before));
+        ne_before += ne_here;
+        tri_before += __builtin_amdgcn_readlane(incl, 63);
+    }
+    __builtin_amdgcn_wave_barrier();
+    mc_emit_group(p, tl.nch, l.gchunk[wave], want, tile_off[b] + l.goff[wave][0], lane, l,
+                  l.stage[wave], verts, tris, capacity);
+    __builtin_amdgcn_wave_barrier();
 }
 
 }  // namespace recon

@@ -76,124 +76,110 @@ __device__ __forceinline__ ChunkCorners chunk_corners(const WeldParams& w, int c
     return k;
 }
 
-// Edge masks and triangle count of chunk j of a voxel row (cube corners k of
-// lane `lane`).  Lane j % 64 keeps the chunk's results in km / kv / kt; every
-// 64 chunks and at the row's end the kept chunks are stored together.
-// Every lane must call.
-__device__ __forceinline__ void weld_chunk_out(const WeldParams& w, const ChunkCorners& k, int lane,
-                                               int j, int64_t row_chunk0,
-                                               unsigned long long km[3], int& kv, int& kt,
-                                               unsigned long long* __restrict__ masks,
-                                               int32_t* __restrict__ vcount,
-                                               int32_t* __restrict__ tcount) {
-    const McParams& p = w.mc;
-    bool bx = false, by = false, bz = false;
-    int nt = 0;
-    if (k.in) {
-        // corners 1, 4, 3 are the +x, +y, +z neighbours (marching_cubes.cu:118-127)
-        const bool n0 = k.v[0] < 0.0f;
-        bx = k.x + 1 < p.gx && (k.v[1] < 0.0f) != n0;
-        by = k.y + 1 < p.gy && (k.v[4] < 0.0f) != n0;
-        bz = k.z + 1 < p.gz && (k.v[3] < 0.0f) != n0;
-        if (k.cube) nt = class_ntri(cube_class(k.v));
-    }
-    const unsigned long long mx = __ballot(bx), my = __ballot(by), mz = __ballot(bz);
-    // the chunk's triangles (nt <= 5 per cube) from ballots of nt's bits
-    const int ntot = __popcll(__ballot(nt & 1)) + 2 * __popcll(__ballot(nt & 2)) +
-                     4 * __popcll(__ballot(nt & 4));
-    if (lane == (j & 63)) {
-        km[0] = mx;
-        km[1] = my;
-        km[2] = mz;
-        kv = __popcll(mx) + __popcll(my) + __popcll(mz);
-        kt = ntot;
-    }
-    if ((j & 63) == 63 || j == w.cpr - 1) {
-        const int64_t cc = row_chunk0 + (j & ~63) + lane;
-        if (lane <= (j & 63)) {
-            masks[3 * cc + 0] = km[0];
-            masks[3 * cc + 1] = km[1];
-            masks[3 * cc + 2] = km[2];
-            vcount[cc] = kv;
-            tcount[cc] = kt;
-        }
-    }
-}
-
-// One voxel row (y, z): rows with a +y and a +z neighbour row stream their
-// corners, the grid's last y and z rows read them per chunk.
-__device__ __forceinline__ void weld_count_row(const WeldParams& w, int y, int z, int lane,
-                                               unsigned long long* __restrict__ masks,
-                                               int32_t* __restrict__ vcount,
-                                               int32_t* __restrict__ tcount) {
-    const McParams& p = w.mc;
-    const int row = y + p.gy * z;
-    const bool interior = y < p.gy - 1 && z < p.gz - 1;
-    CornerStream cs(p, interior ? y : 0, interior ? z : 0, lane);
-    unsigned long long km[3] = {0ull, 0ull, 0ull};
-    int kv = 0, kt = 0;
-    for (int j = 0; j < w.cpr; ++j) {
-        const int c = row * w.cpr + j;
-        ChunkCorners k;
-        if (interior) {
-            k.x = j * 64 + lane;
-            k.y = y;
-            k.z = z;
-            k.in = k.x < p.gx;
-            k.cube = k.x < p.gx - 1;
-            cs.step(j * 64, lane, k.v);
-        } else {
-            k = chunk_corners(w, c, lane);
-        }
-        weld_chunk_out(w, k, lane, j, (int64_t)row * w.cpr, km, kv, kt, masks, vcount, tcount);
-    }
-}
-
-// Voxel rows per wavefront in the count pass: 2 interior rows of a layer
-// stream their 6 shared voxel rows once (8 if each read its own 4): 427 ->
-// 352 us at 512^3.  4 rows held 130+ VGPRs (3 waves/SIMD) and took 496 us.
-constexpr int kWeldCountRows = 2;
+// Count pass: one wavefront per four voxel rows (y..y+3, z), streaming the
+// ten voxel rows (y+i, z) and (y+i, z+1), i = 0..4, 64 voxels a step (the
+// marching-cubes count's walk): each voxel's sign bit (v < 0) for the ten
+// rows packs into one word, bit 2j the lane's voxel of row j and bit 2j+1
+// the next lane's (a DPP wave shift; lane 63 takes the next step's lane 0).
+// Per row i and chunk: the x, y and z edge masks are ballots of bit 4i
+// against bits 4i+1, 4i+4 and 4i+2; the cube byte of cube row i is bits 4i
+// .. 4i+7 (corner order 0, 1, 3, 2, 4, 5, 7, 6, the count table
+// mc_ntab_qbyte); lane i of the wave stores row i's masks and counts.
+// Loads past a row's end read its last voxel, and rows past the grid (y+i
+// >= gy, z+1 >= gz) read the last row (in bounds); their edges and cubes
+// do not count.
+constexpr int kWeldRows = 4;
+constexpr int kWeldLoadRows = 2 * (kWeldRows + 1);
 
 __global__ __launch_bounds__(256) void weld_count_kernel(const WeldParams w,
                                                          unsigned long long* __restrict__ masks,
                                                          int32_t* __restrict__ vcount,
                                                          int32_t* __restrict__ tcount) {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint8_t ntab[256];
+    ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
+    __syncthreads();
     const McParams& p = w.mc;
-    const int groups = (p.gy + kWeldCountRows - 1) / kWeldCountRows;
-    const int q = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
-    const int z = q / groups, y = kWeldCountRows * (q % groups);
+    const int lane = threadIdx.x & 63;
+    const int gpl = (p.gy + kWeldRows - 1) / kWeldRows;
+    const int q = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWeldWaves + (threadIdx.x >> 6)));
+    const int z = q / gpl, y = kWeldRows * (q - z * gpl);
     if (z >= p.gz) return;
-    if (!(y + kWeldCountRows <= p.gy - 1 && z < p.gz - 1)) {
-        // a group touching the grid's last y or z row: one row at a time
-        for (int i = 0; i < kWeldCountRows && y + i < p.gy; ++i)
-            weld_count_row(w, y + i, z, lane, masks, vcount, tcount);
-        return;
+    const int k = min(kWeldRows, p.gy - y);         // voxel rows of the wave
+    const int z1 = min(z + 1, p.gz - 1);
+    const bool zin = z + 1 < p.gz;
+    const char* rowp[kWeldLoadRows];
+#pragma unroll
+    for (int i = 0; i <= kWeldRows; ++i) {
+        const int yy = min(y + i, p.gy - 1);
+        rowp[2 * i] = reinterpret_cast<const char*>(p.f + (size_t)p.gx * ((size_t)yy + (size_t)p.gy * z));
+        rowp[2 * i + 1] =
+            reinterpret_cast<const char*>(p.f + (size_t)p.gx * ((size_t)yy + (size_t)p.gy * z1));
     }
-    CornerStreamK<kWeldCountRows> cs(p, y, z, lane);
-    unsigned long long km[kWeldCountRows][3];
-    int kv[kWeldCountRows], kt[kWeldCountRows];
+    const uint32_t xlast = (uint32_t)(p.gx - 1);
+    auto load = [&](int chunk, float v[kWeldLoadRows]) {
+        const uint32_t off = min((uint32_t)(64 * chunk + lane), xlast) << 2;
 #pragma unroll
-    for (int i = 0; i < kWeldCountRows; ++i) {
-        km[i][0] = km[i][1] = km[i][2] = 0ull;
-        kv[i] = kt[i] = 0;
-    }
-    for (int j = 0; j < w.cpr; ++j) {
-        float v[kWeldCountRows][8];
-        cs.step(j * 64, lane, v);
+        for (int j = 0; j < kWeldLoadRows; ++j) v[j] = *reinterpret_cast<const float*>(rowp[j] + off);
+    };
+    auto signs = [&](const float v[kWeldLoadRows]) {
+        uint32_t m = 0;
 #pragma unroll
-        for (int i = 0; i < kWeldCountRows; ++i) {
-            ChunkCorners k;
-            k.x = j * 64 + lane;
-            k.y = y + i;
-            k.z = z;
-            k.in = k.x < p.gx;
-            k.cube = k.x < p.gx - 1;
+        for (int j = 0; j < kWeldLoadRows; ++j) m |= (v[j] < 0.0f ? 1u : 0u) << (2 * j);
+        return m;
+    };
+    const int cpr = w.cpr;
+    auto emit = [&](int c, uint32_t pc, uint32_t pn) {
+        const uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pc, 0x130, 0xF, 0xF, false);
+        const uint32_t first = (uint32_t)__builtin_amdgcn_readlane((int)pn, 0);
+        const uint32_t word = pc | ((lane == 63 ? first : sh) << 1);
+        const int x = 64 * c + lane;
+        const bool xin = x < p.gx, xin1 = x + 1 < p.gx;
+        int n[kWeldRows];
+        unsigned long long mx[kWeldRows], my[kWeldRows], mz[kWeldRows];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) k.v[m] = v[i][m];
-            weld_chunk_out(w, k, lane, j, (int64_t)(y + i + p.gy * z) * w.cpr, km[i], kv[i], kt[i],
-                           masks, vcount, tcount);
+        for (int i = 0; i < kWeldRows; ++i) {
+            const uint32_t s0 = (word >> (4 * i)) & 1u;
+            const bool bx = xin1 && s0 != ((word >> (4 * i + 1)) & 1u);
+            const bool by = xin && y + i + 1 < p.gy && s0 != ((word >> (4 * i + 4)) & 1u);
+            const bool bz = xin && zin && s0 != ((word >> (4 * i + 2)) & 1u);
+            mx[i] = __ballot(bx);
+            my[i] = __ballot(by);
+            mz[i] = __ballot(bz);
+            const bool cube = xin1 && zin && y + i + 1 < p.gy;
+            n[i] = cube ? (int)ntab[(word >> (4 * i)) & 0xFFu] : 0;
         }
+        const int s01 = wave_inclusive_scan(n[0] | (n[1] << 16), lane);
+        const int s23 = wave_inclusive_scan(n[2] | (n[3] << 16), lane);
+        const int t01 = __builtin_amdgcn_readlane(s01, 63), t23 = __builtin_amdgcn_readlane(s23, 63);
+        if (lane < k) {
+            const int i = lane;
+            const unsigned long long ax = i == 0 ? mx[0] : (i == 1 ? mx[1] : (i == 2 ? mx[2] : mx[3]));
+            const unsigned long long ay = i == 0 ? my[0] : (i == 1 ? my[1] : (i == 2 ? my[2] : my[3]));
+            const unsigned long long az = i == 0 ? mz[0] : (i == 1 ? mz[1] : (i == 2 ? mz[2] : mz[3]));
+            const int tw = i < 2 ? t01 : t23;
+            const int64_t cc = (int64_t)(y + i + p.gy * z) * cpr + c;
+            masks[3 * cc + 0] = ax;
+            masks[3 * cc + 1] = ay;
+            masks[3 * cc + 2] = az;
+            vcount[cc] = __popcll(ax) + __popcll(ay) + __popcll(az);
+            tcount[cc] = (i & 1) ? (tw >> 16) : (tw & 0xFFFF);
+        }
+    };
+    float va[kWeldLoadRows], vb[kWeldLoadRows];
+    load(0, va);
+    load(1, vb);
+    uint32_t pc = signs(va);
+    load(2, va);
+    for (int c = 0; c < cpr; c += 2) {
+        uint32_t pn = signs(vb);  // chunk c+1
+        load(c + 3, vb);
+        emit(c, pc, pn);
+        if (c + 1 >= cpr) break;
+        pc = pn;
+        pn = signs(va);  // chunk c+2
+        load(c + 4, va);
+        emit(c + 1, pc, pn);
+        pc = pn;
     }
 }
 
@@ -306,11 +292,57 @@ __device__ __forceinline__ int32_t weld_index(const WeldParams& w,
     return (int32_t)idx;
 }
 
-// One wavefront per voxel row, chunk by chunk: the chunk's vertices, then
-// its cubes' triangles in the soup's order (chunks ascend in z, y, x like
-// the soup's cubes).
+// The chunk's vertices, then its cubes' triangles in the soup's order (one
+// wavefront; every lane calls).
+__device__ __forceinline__ void weld_write_chunk(const WeldParams& w, int c, int lane,
+                                                 const unsigned long long* __restrict__ masks,
+                                                 const long long* __restrict__ vbase,
+                                                 const long long* __restrict__ tbase, int nt_chunk,
+                                                 float* __restrict__ verts,
+                                                 float* __restrict__ normals,
+                                                 int32_t* __restrict__ tris) {
+    if (verts || normals) {
+        const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
+                                         masks[3 * (int64_t)c + 2]};
+        if (m[0] | m[1] | m[2]) weld_chunk_vertices(w, c, lane, m, vbase[c], verts, normals);
+    }
+    if (!tris || nt_chunk == 0) return;
+    const ChunkCorners k = chunk_corners(w, c, lane);
+    int cls = 0, nt = 0;
+    if (k.cube) {
+        cls = cube_class(k.v);
+        nt = class_ntri(cls);
+    }
+    const int incl = wave_inclusive_scan(nt, lane);
+    const int excl = incl - nt;
+    const int total = __shfl(incl, 63, 64);
+    const long long t0 = tbase[c];
+    const int x0 = k.x - lane;
+    // triangle kk of the chunk is written by lane kk % 64 (as in mc_emit_chunk)
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        const int kk = k0 + lane;
+        const int owner = wave_owner(incl, kk);
+        const int ocls = __shfl(cls, owner, 64);
+        const int slot = kk - __shfl(excl, owner, 64);
+        if (kk < total) {
+            const long long t = t0 + kk;
+            const uint64_t word = c_tri[ocls];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
+                tris[3 * t + q] = weld_index(w, masks, vbase, x0 + owner, k.y, k.z, e);
+            }
+        }
+    }
+}
+
+// One wavefront per voxel row: one load of the vertex and triangle counts
+// of 64 of the row's chunks at a time, a ballot of the chunks with either,
+// and weld_write_chunk for each of those (most of a volume's chunks hold
+// neither, and an empty row costs one load).
 __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
                                                          const unsigned long long* __restrict__ masks,
+                                                         const int32_t* __restrict__ vcount,
                                                          const long long* __restrict__ vbase,
                                                          const int32_t* __restrict__ tcount,
                                                          const long long* __restrict__ tbase,
@@ -320,44 +352,24 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
     if (row >= w.mc.gy * w.mc.gz) return;
-    for (int c = row * w.cpr; c < (row + 1) * w.cpr; ++c) {
-        if (verts || normals) {
-            const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
-                                             masks[3 * (int64_t)c + 2]};
-            if (m[0] | m[1] | m[2]) weld_chunk_vertices(w, c, lane, m, vbase[c], verts, normals);
-        }
-        if (!tris || tcount[c] == 0) continue;
-        const ChunkCorners k = chunk_corners(w, c, lane);
-        int cls = 0, nt = 0;
-        if (k.cube) {
-            cls = cube_class(k.v);
-            nt = class_ntri(cls);
-        }
-        const int incl = wave_inclusive_scan(nt, lane);
-        const int excl = incl - nt;
-        const int total = __shfl(incl, 63, 64);
-        const long long t0 = tbase[c];
-        const int x0 = k.x - lane;
-        // triangle kk of the chunk is written by lane kk % 64 (as in mc_emit_chunk)
-        for (int k0 = 0; k0 < total; k0 += 64) {
-            const int kk = k0 + lane;
-            const int owner = wave_owner(incl, kk);
-            const int ocls = __shfl(cls, owner, 64);
-            const int slot = kk - __shfl(excl, owner, 64);
-            if (kk < total) {
-                const long long t = t0 + kk;
-                const uint64_t word = c_tri[ocls];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
-                    tris[3 * t + q] = weld_index(w, masks, vbase, x0 + owner, k.y, k.z, e);
-                }
-            }
+    const bool want_v = verts || normals;
+    const int c0 = row * w.cpr, c1 = c0 + w.cpr;
+    for (int cw = c0; cw < c1; cw += 64) {
+        const int c = cw + lane;
+        const int nv = c < c1 && want_v ? vcount[c] : 0;
+        const int nt = c < c1 && tris ? tcount[c] : 0;
+        unsigned long long busy = __ballot(nv > 0 || nt > 0);
+        while (busy) {
+            const int l = __builtin_ctzll(busy);
+            busy &= busy - 1;
+            weld_write_chunk(w, cw + l, lane, masks, vbase, tbase, __shfl(nt, l, 64), verts,
+                             normals, tris);
         }
     }
 }
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
 
 // Counts (always), then writes what is asked for when it fits.
 static int weld_run(const float* field, const int32_t grid[3], const float vs[3], const float off[3],
@@ -406,9 +418,8 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     auto* tbase = reinterpret_cast<long long*>(sb + o_tbase);
     auto* vsums = reinterpret_cast<long long*>(sb + o_vsums);
     auto* tsums = reinterpret_cast<long long*>(sb + o_tsums);
-    const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
     const int count_blocks =
-        (((grid[1] + kWeldCountRows - 1) / kWeldCountRows) * grid[2] + kWeldWaves - 1) / kWeldWaves;
+        (((grid[1] + kWeldRows - 1) / kWeldRows) * grid[2] + kWeldWaves - 1) / kWeldWaves;
     hipLaunchKernelGGL(weld_count_kernel, dim3(count_blocks), dim3(256), 0, s, w, masks, vcount,
                        tcount);
     RECON_HIP_TRY(hipGetLastError(), "weld_count_kernel launch");
@@ -432,8 +443,9 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
         return fail(RECON_ERR_CAPACITY, "%s: %lld triangles do not fit capacity %lld", fn, totals[1],
                     (long long)tcap);
     if (totals[0] > 0 || totals[1] > 0) {
-        hipLaunchKernelGGL(weld_write_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vbase, tcount,
-                           tbase, verts, normals, tris);
+        const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
+        hipLaunchKernelGGL(weld_write_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vcount,
+                           vbase, tcount, tbase, verts, normals, tris);
         RECON_HIP_TRY(hipGetLastError(), "weld_write_kernel launch");
     }
     RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes");

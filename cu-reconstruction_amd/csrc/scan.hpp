@@ -36,7 +36,10 @@ int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long
                           hipStream_t s);
 
 // One workgroup: tile_off[i] = sum of tile_sum[0..i) for i <= n (int64;
-// tile_off[n] is the total).  Defined in marching_cubes.hip.
-int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStream_t s);
+// tile_off[n] is the total); with tile_nne, also group_off[i] = sum of
+// ceil(tile_nne / 4) over the tiles before i (group_off[n] the total).  The
+// arrays must be 16-byte aligned.  Defined in marching_cubes.hip.
+int launch_tile_scan(const int32_t* tile_sum, int n, long long* tile_off, hipStream_t s,
+                     const int32_t* tile_nne = nullptr, int32_t* group_off = nullptr);
 
 }  // namespace recon

@@ -150,12 +150,12 @@ __global__ __launch_bounds__(256) void batch_mc_count_kernel(const McBatchVol* _
                                                              uint16_t* __restrict__ cc,
                                                              int32_t* __restrict__ tile_sum) {
     __shared__ uint8_t ntab[256];
-    __shared__ int wtot[4];
+    __shared__ int wtot[8];
     ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
     __syncthreads();
     const int2 e = tiles[blockIdx.x];  // (volume, tile of the volume)
     const McBatchVol& v = vols[e.x];
-    mc_count_tile(v.p, v.t, e.y, cc + v.cc_base, ntab, wtot, tile_sum + blockIdx.x);
+    mc_count_tile(v.p, v.t, e.y, cc + v.cc_base, ntab, wtot, tile_sum + blockIdx.x, nullptr);
 }
 
 __global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McBatchVol* __restrict__ vols,
@@ -164,17 +164,13 @@ __global__ __launch_bounds__(256) void batch_mc_emit_kernel(const McBatchVol* __
                                                             const long long* __restrict__ tile_off,
                                                             float* __restrict__ verts,
                                                             long long capacity) {
-    __shared__ uint8_t ntab[256];
-    __shared__ int lchunk[256];
-    __shared__ int loff[256];
-    __shared__ int wtot[4];
+    __shared__ McEmitLds lds;
     const long long base = tile_off[blockIdx.x];
     if (tile_off[blockIdx.x + 1] == base) return;  // the same for every thread
-    ntab[threadIdx.x] = (uint8_t)class_ntri((int)threadIdx.x);
+    mc_emit_lds_init(lds);
     const int2 e = tiles[blockIdx.x];
     const McBatchVol& v = vols[e.x];
-    mc_emit_tile(v.p, v.t, e.y, cc + v.cc_base, base, ntab, lchunk, loff, wtot, verts, nullptr,
-                 capacity);
+    mc_emit_tile(v.p, v.t, e.y, cc + v.cc_base, base, lds, verts, nullptr, capacity);
 }
 
 // ranges[2v], ranges[2v+1] = first triangle and triangle count of volume v

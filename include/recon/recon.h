@@ -198,7 +198,7 @@ RECON_API int recon_mc_extract(const float* field, const int32_t grid[3], const 
  * harnesses that forbid hipMalloc).  Same arguments, results and status
  * codes as recon_mc_extract, but it calls no hipMalloc / hipFree: its
  * scratch (2 B per 64-cube chunk of each of the (gy-1)*(gz-1) cube rows,
- * plus 12 B per 16 cube rows) lives in one static 48 MiB device array per
+ * plus 20 B per 16 cube rows) lives in one static 48 MiB device array per
  * GPU, held until the process ends; a 1024^3 grid fits, and grids that need
  * more fail with RECON_ERR_RANGE.  Runs on the current device.  Every call on a GPU
  * shares that GPU's array: calls that overlap on one GPU (other host
