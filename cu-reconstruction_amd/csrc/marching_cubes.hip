@@ -3,7 +3,7 @@
 // Replaces generate_triangular_surface (cu-Reconstruction
 // modules/src/cuda/marching_cubes.cu:102-236).  The reference launches one
 // kernel per z layer, synchronises, and compacts each layer on the host from
-// managed memory.  Here the whole volume is done on the device in three
+// managed memory.  Here the whole volume is done on the device in four
 // launches with a fixed, deterministic output order:
 //   1. chunk count - one wavefront per 4 cube rows (y..y+3, z), streaming
 //                    their 10 voxel rows once, 64 cubes a step: every
@@ -17,15 +17,20 @@
 //                    (each tile's first triangle and the volume's total)
 //                    and of the tiles' groups of four chunks with
 //                    triangles;
-//   3. emit        - a grid sized to the resident workgroups: each
+//   3. group prep  - one wavefront per tile: a 16-byte record per group of
+//                    four chunks with triangles (its tile, first triangle
+//                    and chunks);
+//   4. emit        - a grid sized to the resident workgroups: each
 //                    wavefront takes groups gi = wave, wave + waves, ...,
-//                    finds each group's tile and chunks, and spreads their
-//                    triangles (16 lanes per chunk, 4 cubes per lane) over
-//                    its 64 lanes to compute the vertices
-//                    (marching_cubes.cu:23-74), staged in LDS and written in
-//                    consecutive dwords.  A tile dense with triangles (a
-//                    floor along a cube layer) is thus shared by many
-//                    wavefronts.
+//                    reads each group's record and spreads its triangles
+//                    (16 lanes per chunk, 4 cubes per lane) over its 64
+//                    lanes to compute the vertices (marching_cubes.cu:23-74),
+//                    staged in LDS and written in consecutive dwords.  A tile
+//                    dense with triangles (a floor along a cube layer) is
+//                    thus shared by many wavefronts.  The allocation-free
+//                    entry skips step 3 (its static scratch has no room for
+//                    the records) and finds each group's tile and chunks by
+//                    a search and a walk instead.
 // Output order is cube z, y, x, table slot -- the order the reference's host
 // loop produces (marching_cubes.cu:163-188 inside :213-225) -- as a triangle
 // soup: triangle t owns vertices 3t, 3t+1, 3t+2 (marching_cubes.cu:171).
@@ -180,11 +185,45 @@ __global__ __launch_bounds__(256) void mc_chunk_count_kernel(const McParams p, c
     mc_count_tile(p, tl, blockIdx.x, cc, ntab, wtot, tile_sum + blockIdx.x, tile_nne + blockIdx.x);
 }
 
+// One wavefront per tile: the records of the tile's groups (mc_group_prep).
+__global__ __launch_bounds__(256) void mc_group_prep_kernel(const McParams p, const McTiling tl,
+                                                            const uint16_t* __restrict__ cc,
+                                                            const long long* __restrict__ tile_off,
+                                                            const int32_t* __restrict__ tile_nne,
+                                                            const int32_t* __restrict__ group_off,
+                                                            McGroupRec* __restrict__ rec) {
+    const int b = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (b < tl.tiles)
+        mc_group_prep(p, tl, b, cc, tile_off, tile_nne, group_off, rec, (int)(threadIdx.x & 63));
+}
+
 // A grid sized to the resident workgroups walks the volume's groups of four
 // chunks with triangles, wavefront by wavefront (gi = wave, wave + waves,
-// ...): the groups of a tile dense with triangles (a floor along a cube
-// layer) go to as many wavefronts.
+// ...), each from its record (one scalar load, which unlike a vector load
+// does not wait for the wavefront's earlier stores): the groups of a tile
+// dense with triangles (a floor along a cube layer) go to as many
+// wavefronts.
 __global__ __launch_bounds__(256) void mc_chunk_emit_kernel(const McParams p, const McTiling tl,
+                                                            const McGroupRec* __restrict__ rec,
+                                                            const int32_t* __restrict__ group_off,
+                                                            float* __restrict__ verts,
+                                                            int32_t* __restrict__ tris,
+                                                            long long capacity) {
+    __shared__ McEmitLds lds;
+    mc_emit_lds_init(lds);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int ngroups = group_off[tl.tiles];
+    const int waves = (int)gridDim.x * 4;
+    for (int gi = (int)blockIdx.x * 4 + wave; gi < ngroups; gi += waves)
+        mc_emit_group_rec(p, tl, rec[gi], lds, wave, lane, verts, tris, capacity);
+}
+
+// The same walk for the allocation-free entry (no room for records): each
+// wavefront finds its group's tile by a search over group_off and its
+// chunks by a walk over the tile's chunk counts (mc_emit_group_at).
+__global__ __launch_bounds__(256) void mc_chunk_emit_search_kernel(const McParams p, const McTiling tl,
                                                             const uint16_t* __restrict__ cc,
                                                             const long long* __restrict__ tile_off,
                                                             const int32_t* __restrict__ tile_nne,
@@ -315,12 +354,19 @@ static int check_grid(const float* field, const int32_t grid[3], const float vs[
 }
 
 // Tiling of a grid (host side) and the scratch layout: chunk counts
-// uint16[rows * nch] | tile sums int32[tiles] | tile offsets int64[tiles+1],
-// each part 256-byte aligned.
+// uint16[rows * nch] | tile sums int32[tiles] | tile chunks with triangles
+// int32[tiles] | tile offsets int64[tiles+1] | tile first groups
+// int32[tiles+1] (need bytes, the search emit's) | group records
+// McGroupRec[4 * nch per tile] (need_rec bytes in all, the record emit's),
+// each part 256-byte aligned.  Records fit their fields when the tile index
+// stays below 2^24, the triangle count below 2^40 (at most 5 per cube) and a
+// tile's chunks below kNoChunk.
 struct McScratchLayout {
     McTiling t{0, 0, 0, 0};
     int gpl = 0, groups = 0, tiles = 0, nch = 0;
     size_t off_sum = 0, off_nne = 0, off_off = 0, off_goff = 0, need = 0;
+    size_t off_rec = 0, need_rec = 0;
+    bool rec_ok = false;
     explicit McScratchLayout(const int32_t grid[3]) {
         if (grid[0] < 2 || grid[1] < 2 || grid[2] < 2) return;
         t = mc_tiling(grid[0], grid[1], grid[2]);
@@ -335,29 +381,36 @@ struct McScratchLayout {
         off_off = off_nne + a256(4 * (size_t)tiles);
         off_goff = off_off + a256(8 * ((size_t)tiles + 1));
         need = off_goff + 4 * ((size_t)tiles + 1);
+        off_rec = a256(need);
+        need_rec = off_rec + sizeof(McGroupRec) * (size_t)tiles * 4 * (size_t)nch;
+        const double cubes = (double)(grid[0] - 1) * (double)rows;
+        rec_ok = tiles < (1 << 24) && 5.0 * cubes < 0x1p40 && 16 * (size_t)nch < kNoChunk;
     }
 };
 
-// Workgroups of the emit pass: its resident slots on every CU of the current
-// device (looked up once per device).
-static int emit_grid() {
+// Workgroups of an emit pass: its resident slots on every CU of the current
+// device (looked up once per device and kernel; which = 0 the record emit,
+// 1 the search emit).
+static int emit_grid(int which) {
     static std::mutex mu;
-    static int grid[64] = {0};
+    static int grid[2][64] = {{0}};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
     std::lock_guard<std::mutex> lk(mu);
-    if (!grid[dev]) {
+    if (!grid[which][dev]) {
         int cus = 0, occ = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus < 1)
             cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_chunk_emit_kernel, 256, 0) !=
-                hipSuccess ||
-            occ < 1)
-            occ = 1;
-        grid[dev] = cus * occ;
+        const hipError_t e =
+            which == 0
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_chunk_emit_kernel, 256, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, mc_chunk_emit_search_kernel,
+                                                               256, 0);
+        if (e != hipSuccess || occ < 1) occ = 1;
+        grid[which][dev] = cus * occ;
     }
-    return grid[dev];
+    return grid[which][dev];
 }
 
 size_t mc_scratch_need(const int32_t grid[3]) { return McScratchLayout(grid).need; }
@@ -374,12 +427,16 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     if (*scratch_bytes < L.need && !may_allocate)
         return fail(RECON_ERR_RANGE, "marching cubes needs %zu bytes of scratch, %zu available",
                     L.need, *scratch_bytes);
-    if (*scratch_bytes < L.need) {
+    // the record emit when allocating; the allocation-free entry keeps to the
+    // search emit's smaller scratch
+    const bool use_rec = L.rec_ok && may_allocate;
+    const size_t need = use_rec ? L.need_rec : L.need;
+    if (*scratch_bytes < need) {
         if (*scratch) (void)hipFree(*scratch);
         *scratch = nullptr;
         *scratch_bytes = 0;
-        RECON_HIP_TRY(hipMalloc(scratch, L.need), "marching cubes scratch");
-        *scratch_bytes = L.need;
+        RECON_HIP_TRY(hipMalloc(scratch, need), "marching cubes scratch");
+        *scratch_bytes = need;
     }
     char* sb = static_cast<char*>(*scratch);
     uint16_t* cc = reinterpret_cast<uint16_t*>(sb);
@@ -387,6 +444,7 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     int32_t* tile_nne = reinterpret_cast<int32_t*>(sb + L.off_nne);
     long long* tile_off = reinterpret_cast<long long*>(sb + L.off_off);
     int32_t* group_off = reinterpret_cast<int32_t*>(sb + L.off_goff);
+    McGroupRec* rec = reinterpret_cast<McGroupRec*>(sb + L.off_rec);
     hipLaunchKernelGGL(mc_chunk_count_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc, tile_sum,
                        tile_nne);
     RECON_HIP_TRY(hipGetLastError(), "mc_chunk_count_kernel launch");
@@ -397,9 +455,19 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
                                  hipMemcpyDeviceToHost, s),
                   "marching cubes total");
     if (!count_only && capacity > 0 && verts) {
-        hipLaunchKernelGGL(mc_chunk_emit_kernel, dim3(emit_grid()), dim3(256), 0, s, p, L.t, cc,
-                           tile_off, tile_nne, group_off, verts, tris, (long long)capacity);
-        RECON_HIP_TRY(hipGetLastError(), "mc_chunk_emit_kernel launch");
+        if (use_rec) {
+            hipLaunchKernelGGL(mc_group_prep_kernel, dim3((L.tiles + 3) / 4), dim3(256), 0, s, p,
+                               L.t, cc, tile_off, tile_nne, group_off, rec);
+            RECON_HIP_TRY(hipGetLastError(), "mc_group_prep_kernel launch");
+            hipLaunchKernelGGL(mc_chunk_emit_kernel, dim3(emit_grid(0)), dim3(256), 0, s, p, L.t,
+                               rec, group_off, verts, tris, (long long)capacity);
+            RECON_HIP_TRY(hipGetLastError(), "mc_chunk_emit_kernel launch");
+        } else {
+            hipLaunchKernelGGL(mc_chunk_emit_search_kernel, dim3(emit_grid(1)), dim3(256), 0, s, p,
+                               L.t, cc, tile_off, tile_nne, group_off, verts, tris,
+                               (long long)capacity);
+            RECON_HIP_TRY(hipGetLastError(), "mc_chunk_emit_search_kernel launch");
+        }
     }
     RECON_HIP_TRY(hipStreamSynchronize(s), "marching cubes synchronize");
     *total = host_total;

@@ -746,4 +746,101 @@ __device__ __forceinline__ void mc_emit_group_at(const McParams& p, const McTili
     __builtin_amdgcn_wave_barrier();
 }
 
+// Record of one group of four chunks with triangles (the emit's unit of
+// work), written by mc_group_prep for the record-driven emit: bits 0-39 of
+// tb its first triangle, bits 40-63 its tile; c[q] its q-th chunk counted
+// from the tile's first chunk, kNoChunk past the last chunk of the tile's
+// last group.  One 16-byte load gives a wavefront all of it.
+struct alignas(16) McGroupRec {
+    unsigned long long tb;
+    uint16_t c[4];
+};
+constexpr int kRecTriBits = 40;
+constexpr uint16_t kNoChunk = 0xFFFF;
+
+// The first chunk of tile b and the end of its chunks (tiles are runs of
+// kTileGroups consecutive groups, so their cube rows and chunks are
+// consecutive).
+__device__ __forceinline__ void mc_tile_chunks(const McParams& p, const McTiling& tl, int b,
+                                               int& c_begin, int& c_end) {
+    const int qa = b * kTileGroups, qb = min(qa + kTileGroups, tl.groups) - 1;
+    int ya, za, ka, yb, zb, kb;
+    mc_group(p, tl.gpl, qa, ya, za, ka);
+    mc_group(p, tl.gpl, qb, yb, zb, kb);
+    c_begin = (ya + p.cyr * za) * tl.nch;
+    c_end = (yb + p.cyr * zb + kb) * tl.nch;
+}
+
+// The records of tile b's groups, for one wavefront: a walk over the tile's
+// chunk counts, 256 at a time (four per lane; the lanes' counts of chunks
+// with triangles and of triangles prefix-summed in one packed scan) gives
+// each chunk with triangles its rank r and first triangle; it is slot r % 4
+// of group group_off[b] + r / 4.  Every field of every record is written by
+// exactly one lane.
+__device__ __forceinline__ void mc_group_prep(const McParams& p, const McTiling& tl, int b,
+                                              const uint16_t* __restrict__ cc,
+                                              const long long* __restrict__ tile_off,
+                                              const int32_t* __restrict__ tile_nne,
+                                              const int32_t* __restrict__ group_off,
+                                              McGroupRec* __restrict__ rec, int lane) {
+    const int nne = tile_nne[b];
+    if (nne == 0) return;
+    const int g0 = group_off[b];
+    const long long t0 = tile_off[b];
+    int c_begin, c_end;
+    mc_tile_chunks(p, tl, b, c_begin, c_end);
+    const unsigned long long tile_bits = (unsigned long long)b << kRecTriBits;
+    int ne_before = 0, tri_before = 0;
+    for (int w0 = c_begin; w0 < c_end; w0 += 256) {
+        const int ci0 = w0 + 4 * lane;
+        int c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) c[q] = ci0 + q < c_end ? (int)cc[ci0 + q] : 0;
+        int lane_ne = 0, lane_tri = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lane_ne += c[q] > 0 ? 1 : 0;
+            lane_tri += c[q];
+        }
+        // chunks with triangles (<= 256) in bits 0-8, triangles (<= 81920) above
+        const int pk = lane_ne | (lane_tri << 9);
+        const int incl = wave_inclusive_scan(pk, lane);
+        int r = ne_before + ((incl - pk) & 0x1FF);
+        int t = tri_before + ((incl - pk) >> 9);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c[q] > 0) {
+                McGroupRec& R = rec[g0 + (r >> 2)];
+                R.c[r & 3] = (uint16_t)(ci0 + q - c_begin);
+                if ((r & 3) == 0) R.tb = tile_bits | (unsigned long long)(t0 + t);
+                if (r == nne - 1)
+                    for (int e = (r & 3) + 1; e < 4; ++e) R.c[e] = kNoChunk;
+                ++r;
+            }
+            t += c[q];
+        }
+        const int all = __builtin_amdgcn_readlane(incl, 63);
+        ne_before += all & 0x1FF;
+        tri_before += all >> 9;
+    }
+}
+
+// Group gi from its record (mc_group_prep), for one wavefront; then
+// mc_emit_group.
+__device__ __forceinline__ void mc_emit_group_rec(const McParams& p, const McTiling& tl,
+                                                  const McGroupRec& R, McEmitLds& l, int wave,
+                                                  int lane, float* __restrict__ verts,
+                                                  int32_t* __restrict__ tris, long long capacity) {
+    const int b = (int)(R.tb >> kRecTriBits);
+    const long long t0 = (long long)(R.tb & ((1ull << kRecTriBits) - 1ull));
+    int c_begin, c_end;
+    mc_tile_chunks(p, tl, b, c_begin, c_end);
+    const int n = 1 + (R.c[1] != kNoChunk) + (R.c[2] != kNoChunk) + (R.c[3] != kNoChunk);
+    if (lane < 4) l.gchunk[wave][lane] = c_begin + (lane == 0 ? R.c[0] : lane == 1 ? R.c[1]
+                                                   : lane == 2 ? R.c[2] : R.c[3]);
+    __builtin_amdgcn_wave_barrier();
+    mc_emit_group(p, tl.nch, l.gchunk[wave], n, t0, lane, l, l.stage[wave], verts, tris, capacity);
+    __builtin_amdgcn_wave_barrier();
+}
+
 }  // namespace recon
