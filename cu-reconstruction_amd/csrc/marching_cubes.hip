@@ -6,7 +6,9 @@
 // managed memory.  Here the whole volume is done on the device in four
 // launches with a fixed, deterministic output order:
 //   1. chunk count - one wavefront per 4 cube rows (y..y+3, z), streaming
-//                    their 10 voxel rows once, 64 cubes a step: every
+//                    their 10 voxel rows once, 256 cubes a step (16-byte
+//                    loads; 64 with 4-byte loads where rows are not 16-byte
+//                    aligned): every
 //                    cube's class (marching_cubes.cu:76-84,118-130) from
 //                    the corners' sign bits, its triangle count from a
 //                    table, and per 64-cube chunk of each row the chunk's
@@ -183,6 +185,19 @@ __global__ __launch_bounds__(256) void mc_chunk_count_kernel(const McParams p, c
     ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
     __syncthreads();
     mc_count_tile(p, tl, blockIdx.x, cc, ntab, wtot, tile_sum + blockIdx.x, tile_nne + blockIdx.x);
+}
+
+// The same with 16-byte loads (mc_count_tile4): gx % 4 == 0 and a 16-byte
+// aligned field.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void mc_chunk_count4_kernel(const McParams p, const McTiling tl,
+                                                              uint16_t* __restrict__ cc,
+                                                              int32_t* __restrict__ tile_sum,
+                                                              int32_t* __restrict__ tile_nne) {
+    __shared__ uint8_t ntab[256];
+    __shared__ int wtot[8];
+    ntab[threadIdx.x] = mc_ntab_qbyte((int)threadIdx.x);
+    __syncthreads();
+    mc_count_tile4(p, tl, blockIdx.x, cc, ntab, wtot, tile_sum + blockIdx.x, tile_nne + blockIdx.x);
 }
 
 // One wavefront per tile: the records of the tile's groups (mc_group_prep).
@@ -445,8 +460,12 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     long long* tile_off = reinterpret_cast<long long*>(sb + L.off_off);
     int32_t* group_off = reinterpret_cast<int32_t*>(sb + L.off_goff);
     McGroupRec* rec = reinterpret_cast<McGroupRec*>(sb + L.off_rec);
-    hipLaunchKernelGGL(mc_chunk_count_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc, tile_sum,
-                       tile_nne);
+    if (grid[0] % 4 == 0 && reinterpret_cast<uintptr_t>(field) % 16 == 0)
+        hipLaunchKernelGGL(mc_chunk_count4_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc,
+                           tile_sum, tile_nne);
+    else
+        hipLaunchKernelGGL(mc_chunk_count_kernel, dim3(L.tiles), dim3(256), 0, s, p, L.t, cc,
+                           tile_sum, tile_nne);
     RECON_HIP_TRY(hipGetLastError(), "mc_chunk_count_kernel launch");
     int rc = launch_tile_scan(tile_sum, L.tiles, tile_off, s, tile_nne, group_off);
     if (rc) return rc;

@@ -25,6 +25,20 @@ constexpr uint64_t pack_edge_corner(int w) {
 }
 constexpr uint64_t kEdgeA = pack_edge_corner(0), kEdgeB = pack_edge_corner(1);
 
+// The triangle count of every class (the packed table's low 4 bits, at most
+// 5) packed 3 bits per class, 21 classes per word, so that filling a
+// workgroup's count table takes no memory access.
+constexpr uint64_t kTriPacked[256] = RECON_MC_TRI_PACKED_INIT;
+struct NtriWords {
+    uint64_t w[13];
+};
+constexpr NtriWords pack_ntri() {
+    NtriWords r{};
+    for (int c = 1; c < 255; ++c) r.w[c / 21] |= (kTriPacked[c] & 0xFull) << (3 * (c % 21));
+    return r;
+}
+constexpr NtriWords kNtri = pack_ntri();
+
 struct McParams {
     const float* f;
     int32_t gx, gy, gz;
@@ -190,6 +204,19 @@ __device__ __forceinline__ int wave_sum(int v) {
 // The lane owning element k of per-lane counts whose inclusive wave scan is
 // `incl` (k < incl of lane 63): the number of lanes with incl <= k.  Every
 // lane must call.
+// Inclusive max over the 64 lanes of a wavefront of non-negative values, by
+// DPP moves (row shifts, then row broadcasts of lanes 15 and 31).
+__device__ __forceinline__ int wave_inclusive_max(int v) {
+    int t = v;
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x111, 0xF, 0xF, true));  // row_shr:1
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x112, 0xF, 0xF, true));  // row_shr:2
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x114, 0xF, 0xF, true));  // row_shr:4
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x118, 0xF, 0xF, true));  // row_shr:8
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    t = max(t, __builtin_amdgcn_update_dpp(0, t, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return t;
+}
+
 __device__ __forceinline__ int wave_owner(int incl, int k) {
     int pos = 0;
 #pragma unroll
@@ -398,8 +425,19 @@ __device__ __forceinline__ int qbyte_class(int b) {
            ((b & 0x80) >> 1);
 }
 
+// Triangles of class c (class_ntri) from kNtri's immediates.
+__device__ __forceinline__ int class_ntri_imm(int c) {
+    const int wi = c / 21;
+    uint64_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 13; ++i) w = wi == i ? kNtri.w[i] : w;
+    return (int)((w >> (3 * (c - 21 * wi))) & 7ull);
+}
+
 // Entry i of the count pass's LDS table: triangles of the class in byte i.
-__device__ __forceinline__ uint8_t mc_ntab_qbyte(int i) { return (uint8_t)class_ntri(qbyte_class(i)); }
+__device__ __forceinline__ uint8_t mc_ntab_qbyte(int i) {
+    return (uint8_t)class_ntri_imm(qbyte_class(i));
+}
 
 // Sign bits (v < 0, so -0 and NaN give 0: define_voxel_type,
 // marching_cubes.cu:76-84) of the kVoxelRows values, row j at bit 2j.
@@ -512,6 +550,141 @@ __device__ __forceinline__ void mc_count_tile(const McParams& p, const McTiling&
     }
 }
 
+
+// Inclusive sums within each row of 16 lanes (DPP row shifts): lane 15 of a
+// row ends with the row's total.
+__device__ __forceinline__ int row16_inclusive_sum(int v) {
+    int t = v;
+    t += __builtin_amdgcn_update_dpp(0, t, 0x111, 0xF, 0xF, true);  // row_shr:1
+    t += __builtin_amdgcn_update_dpp(0, t, 0x112, 0xF, 0xF, true);  // row_shr:2
+    t += __builtin_amdgcn_update_dpp(0, t, 0x114, 0xF, 0xF, true);  // row_shr:4
+    t += __builtin_amdgcn_update_dpp(0, t, 0x118, 0xF, 0xF, true);  // row_shr:8
+    return t;
+}
+
+// Sign words (mc_sign_bits) of the four voxels x..x+3 a lane holds per row.
+__device__ __forceinline__ void mc_sign_bits4(const float4 v[kVoxelRows], uint32_t m[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+#pragma unroll
+    for (int j = 0; j < kVoxelRows; ++j) {
+        m[0] |= (v[j].x < 0.0f ? 1u : 0u) << (2 * j);
+        m[1] |= (v[j].y < 0.0f ? 1u : 0u) << (2 * j);
+        m[2] |= (v[j].z < 0.0f ? 1u : 0u) << (2 * j);
+        m[3] |= (v[j].w < 0.0f ? 1u : 0u) << (2 * j);
+    }
+}
+
+// mc_count_tile with 16-byte loads, for rows whose every 4-voxel run is
+// 16-byte aligned (gx % 4 == 0 and a 16-byte-aligned field): the same
+// outputs.  Lane l takes cubes x = 256 s + 4 l + q (q < 4) of span s, so
+// chunk 4 s + m is the 16 lanes of DPP row m and each load instruction
+// moves 1 KB of one voxel row.  Spans go in pairs, both loaded at once (one
+// memory round trip per pair).  A cube's x+1 corners are the lane's next
+// word, or for q = 3 the next lane's first (a DPP wave shift); lane 63's
+// come from the pair's second span, or for the second span from voxels at
+// x = 256 (s + 1) that lane 63 loads itself when that span exists.  In a
+// row's last span lane 63's last cube (x = 256 s + 255) is past the row's
+// last cube (64 nch >= gx - 1, and gx % 4 == 0 rules out gx - 1 = 256 nsp),
+// so it needs no next span.  Loads past the row end read its last 16 bytes
+// (in bounds; their cubes are not counted).
+__device__ __forceinline__ void mc_count_tile4(const McParams& p, const McTiling& tl, const int b,
+                                               uint16_t* __restrict__ cc,
+                                               const uint8_t* __restrict__ ntab, int* wtot,
+                                               int32_t* __restrict__ tile_sum_out,
+                                               int32_t* __restrict__ tile_nne_out) {
+    const int gpl = tl.gpl, groups = tl.groups, nch = tl.nch;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int q = b * kTileGroups + wave;
+    int total = 0, nne = 0;
+    if (q < groups) {
+        int y, z, k;
+        mc_group(p, gpl, q, y, z, k);
+        const int r0 = y + p.cyr * z;
+        const char* rowp[kVoxelRows];
+#pragma unroll
+        for (int i = 0; i <= kGroupRows; ++i) {
+            const int yy = min(y + i, p.gy - 1);
+            rowp[2 * i] = reinterpret_cast<const char*>(
+                p.f + ((size_t)p.gx * ((size_t)yy + (size_t)p.gy * z)));
+            rowp[2 * i + 1] = rowp[2 * i] + 4 * (size_t)p.gx * p.gy;
+        }
+        const uint32_t xlast4 = (uint32_t)(p.gx - 4), xlast = (uint32_t)(p.gx - 1);
+        const int nsp = (nch + 3) / 4;
+        uint16_t* ccr = cc + (size_t)r0 * nch;
+        auto load = [&](int sp, float4 v[kVoxelRows]) {
+            const uint32_t off = min((uint32_t)(256 * sp + 4 * lane), xlast4) << 2;
+#pragma unroll
+            for (int j = 0; j < kVoxelRows; ++j)
+                v[j] = *reinterpret_cast<const float4*>(rowp[j] + off);
+        };
+        // span sp from its words pc and lane 63's next word h63
+        auto count = [&](int sp, const uint32_t pc[4], uint32_t h63) {
+            const uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pc[0], 0x130, 0xF, 0xF,
+                                                                      false);  // lane i <- i+1
+            const uint32_t h3 = lane == 63 ? h63 : sh;
+            const int xb = 256 * sp + 4 * lane;
+            int n01 = 0, n23 = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t hb = e < 3 ? pc[e < 3 ? e + 1 : 0] : h3;
+                uint32_t word = pc[e] | (hb << 1);
+                if (xb + e >= p.cx) word = 0;
+                int n[kGroupRows];
+#pragma unroll
+                for (int i = 0; i < kGroupRows; ++i)
+                    n[i] = i < k ? (int)ntab[(word >> (4 * i)) & 0xFFu] : 0;
+                n01 += n[0] | (n[1] << 16);
+                n23 += n[2] | (n[3] << 16);
+            }
+            total += (n01 & 0xFFFF) + (n01 >> 16) + (n23 & 0xFFFF) + (n23 >> 16);
+            // per chunk and row at most 64 cubes x 5 triangles: 16-bit fields
+            n01 = row16_inclusive_sum(n01);
+            n23 = row16_inclusive_sum(n23);
+            const int c = 4 * sp + (lane >> 4);
+            if ((lane & 15) == 15 && c < nch) {
+#pragma unroll
+                for (int i = 0; i < kGroupRows; ++i) {
+                    if (i >= k) break;
+                    const int w = i < 2 ? n01 : n23;
+                    const int rowc = (i & 1) ? (w >> 16) : (w & 0xFFFF);
+                    ccr[(size_t)i * nch + c] = (uint16_t)rowc;
+                    nne += rowc > 0 ? 1 : 0;
+                }
+            }
+        };
+        for (int sp = 0; sp < nsp; sp += 2) {
+            // a lone last span loads its own voxels twice (clamped, in bounds)
+            float4 va[kVoxelRows], vb[kVoxelRows];
+            load(sp, va);
+            load(sp + 1 < nsp ? sp + 1 : sp, vb);
+            uint32_t hn = 0;
+            if (lane == 63 && sp + 2 < nsp) {
+                const uint32_t offn = min((uint32_t)(256 * sp + 512), xlast) << 2;
+#pragma unroll
+                for (int j = 0; j < kVoxelRows; ++j)
+                    hn |= (*reinterpret_cast<const float*>(rowp[j] + offn) < 0.0f ? 1u : 0u)
+                          << (2 * j);
+            }
+            uint32_t pa[4], pb[4];
+            mc_sign_bits4(va, pa);
+            mc_sign_bits4(vb, pb);
+            count(sp, pa, (uint32_t)__builtin_amdgcn_readlane((int)pb[0], 0));
+            if (sp + 1 < nsp) count(sp + 1, pb, hn);
+        }
+    }
+    total = wave_sum(total);
+    nne = wave_sum(nne);
+    if (lane == 0) {
+        wtot[threadIdx.x >> 6] = total;
+        wtot[4 + (threadIdx.x >> 6)] = nne;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *tile_sum_out = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        if (tile_nne_out) *tile_nne_out = wtot[4] + wtot[5] + wtot[6] + wtot[7];
+    }
+}
 // ---- emit -----------------------------------------------------------------------
 // Per-workgroup LDS of the emit pass.
 struct McEmitLds {
@@ -521,6 +694,7 @@ struct McEmitLds {
     uint8_t ntab[256];              // triangles by class
     float stage[4][64 * 9];         // per wavefront: 64 triangles' vertices
     int gchunk[4][4], goff[4][4];   // per wavefront: its group's chunks, first triangles
+    int mark[4][64];                // per wavefront: the triangle owners' marks
 };
 
 // Fills the tables of an emit workgroup (256 threads); the first barrier of
@@ -541,7 +715,7 @@ __device__ __forceinline__ void mc_emit_lds_init(McEmitLds& l) {
 // triangles' vertices and indices then leave in consecutive dwords.
 __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const int* lchunk,
                                               int ne, long long t0, int lane,
-                                              const McEmitLds& l, float* lstage,
+                                              const McEmitLds& l, float* lstage, int* mark,
                                               float* __restrict__ verts,
                                               int32_t* __restrict__ tris, long long capacity) {
     const int slot4 = lane >> 4;
@@ -577,9 +751,20 @@ __device__ __forceinline__ void mc_emit_group(const McParams& p, int nch, const 
     const int cw = cls[0] | (cls[1] << 8) | (cls[2] << 16) | (cls[3] << 24);
     const int cp = c1 | (c2 << 8) | (c3 << 16);  // each <= 15
     __builtin_amdgcn_wave_barrier();
+    // triangle k's owner: the last lane whose first triangle is <= k.  Each
+    // round marks, at position (first - k0), the lanes whose first triangle
+    // falls in [k0, k0 + 64) (lane + 1, 0 elsewhere); a max-scan of the marks
+    // gives each position its owner, positions before the round's first mark
+    // the previous round's last owner.
+    int carry = 0;
     for (int k0 = 0; k0 < gtotal; k0 += 64) {
         const int k = k0 + lane;
-        const int owner = wave_owner(incl, k);
+        mark[lane] = 0;
+        if (lt > 0 && excl >= k0 && excl < k0 + 64) mark[excl - k0] = lane + 1;
+        __builtin_amdgcn_wave_barrier();
+        const int mx = wave_inclusive_max(mark[lane]);
+        const int owner = mx > 0 ? mx - 1 : carry;
+        carry = __builtin_amdgcn_readlane(owner, 63);
         const int ocw = __shfl(cw, owner, 64);
         const int ocp = __shfl(cp, owner, 64);
         const int oex = __shfl(excl, owner, 64);
@@ -677,7 +862,7 @@ __device__ __forceinline__ void mc_emit_tile(const McParams& p, const McTiling& 
         const int nne = all >> 20;
         for (int g = 4 * wave; g < nne; g += 16)
             mc_emit_group(p, nch, l.lchunk + g, min(4, nne - g), base + l.loff[g], lane, l,
-                          l.stage[wave], verts, tris, capacity);
+                          l.stage[wave], l.mark[wave], verts, tris, capacity);
         woff += all & 0xFFFFF;
         __syncthreads();
     }
@@ -742,7 +927,7 @@ __device__ __forceinline__ void mc_emit_group_at(const McParams& p, const McTili
     }
     __builtin_amdgcn_wave_barrier();
     mc_emit_group(p, tl.nch, l.gchunk[wave], want, tile_off[b] + l.goff[wave][0], lane, l,
-                  l.stage[wave], verts, tris, capacity);
+                  l.stage[wave], l.mark[wave], verts, tris, capacity);
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -839,7 +1024,8 @@ __device__ __forceinline__ void mc_emit_group_rec(const McParams& p, const McTil
     if (lane < 4) l.gchunk[wave][lane] = c_begin + (lane == 0 ? R.c[0] : lane == 1 ? R.c[1]
                                                    : lane == 2 ? R.c[2] : R.c[3]);
     __builtin_amdgcn_wave_barrier();
-    mc_emit_group(p, tl.nch, l.gchunk[wave], n, t0, lane, l, l.stage[wave], verts, tris, capacity);
+    mc_emit_group(p, tl.nch, l.gchunk[wave], n, t0, lane, l, l.stage[wave], l.mark[wave], verts,
+                  tris, capacity);
     __builtin_amdgcn_wave_barrier();
 }
 

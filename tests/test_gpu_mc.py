@@ -47,7 +47,11 @@ def test_integrated_volume(cuda):
     assert n > 50000
 
 
-@pytest.mark.parametrize("grid", [(3, 2, 2), (70, 3, 5), (65, 66, 7), (129, 2, 3)])
+# rows of 3, 70, 65 and 129 voxels take the count pass's 4-byte loads; rows
+# of a multiple of 4 its 16-byte loads, 256 cubes a span: one span cut short
+# (4, 64, 68), two (260) and three (516)
+@pytest.mark.parametrize("grid", [(3, 2, 2), (70, 3, 5), (65, 66, 7), (129, 2, 3), (4, 3, 3),
+                                  (64, 5, 9), (68, 7, 6), (260, 9, 4), (516, 6, 5)])
 def test_random_fields(cuda, grid):
     rng = np.random.default_rng(sum(grid))
     gx, gy, gz = grid
@@ -56,6 +60,24 @@ def test_random_fields(cuda, grid):
     f[rng.random(f.shape) < 0.05] = -0.0         # negative zero: not < 0
     f[rng.random(f.shape) < 0.05] = 1e-41        # denormals
     _mc_compare(cuda, f, grid, (0.01, 0.02, 0.03), (0.1, -0.2, 0.3))
+
+
+def test_unaligned_field_takes_4_byte_loads(cuda):
+    # a field one float past a 16-byte boundary: rows of 68 voxels, but the
+    # 4-byte loads; the same results as the aligned copy
+    grid = (68, 7, 6)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(grid[::-1]).astype(np.float32)
+    buf = cuda.zeros(f.size + 1, dtype=cuda.float32, device="cuda")
+    buf[1:] = cuda.from_numpy(f.ravel()).cuda()
+    dev = buf[1:]
+    assert dev.data_ptr() % 16 == 4
+    vs, off = (0.01, 0.02, 0.03), (0.1, -0.2, 0.3)
+    ov, ot, _, _ = O.mc_extract(f, grid, vs, off)
+    verts, tris = recon.mc_extract(dev, grid, vs, off)
+    assert verts.shape[0] == len(ot) > 0
+    assert_bitexact(verts.cpu().numpy(), ov, "vertices")
+    assert np.array_equal(tris.cpu().numpy(), ot)
 
 
 def test_sparse_chunks_of_wide_rows(cuda):
