@@ -11,6 +11,7 @@
 namespace recon {
 
 constexpr int kThreads = 256;
+constexpr int kRcpTable = 256;  // entries of an integrate workgroup's 1/(w+1) table
 
 // Wave-tile of 4*LX x (64/LX) x NR voxels.  Lane l owns voxels x..x+3
 // (x = x0 + 4*(l % LX)) of row y = y0 + l / LX for z = z0 .. z0+NR-1.  With
@@ -343,7 +344,8 @@ template <class S, class Frames, bool VEC, bool COUNT, bool COLOR = false, bool 
 __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int nframes,
                                                const Frames& frames, const uint64_t live,
                                                const int x0, const int y0, const int z0,
-                                               unsigned& cnt, unsigned& touched) {
+                                               unsigned& cnt, unsigned& touched,
+                                               const float* __restrict__ rtab = nullptr) {
     constexpr int NR = S::kRows, NV = 4 * S::kRows;
     float* __restrict__ D = vp.D;
     float* __restrict__ W = vp.W;
@@ -394,8 +396,9 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
             pz[j] = ((F.R[6] * cxw[i] + rzy) + F.R[8] * czw[r]) + F.t[2];
             int u, v;
             const bool in_z = pz[j] > 0.0f;
-            const bool in_u = pixel_index(px, vp.fx, pz[j], vp.cx, vp.fcols, u);
-            const bool in_v = pixel_index(py, vp.fy, pz[j], vp.cy, vp.frows, v);
+            const float rz = 1.0f / pz[j];  // one correctly rounded division for both axes
+            const bool in_u = pixel_index_rcp(px, vp.fx, pz[j], rz, vp.cx, vp.fcols, u);
+            const bool in_v = pixel_index_rcp(py, vp.fy, pz[j], rz, vp.cy, vp.frows, v);
             ok[j] = in_z && in_u && in_v && row_ok[r] && (VEC || i < nvox);
             addr[j] = ok[j] ? v * F.pitch + u : 0;
             if (COLOR) {
@@ -445,7 +448,14 @@ __device__ __forceinline__ void integrate_tile(const VolumeParams& vp, const int
                 w[j] = nw < vp.maxw ? nw : vp.maxw;
             } else {
                 const float nw = w[j] + 1.0f;
-                d[j] = ((d[j] * w[j]) + tsdf[j]) / nw;
+                const float num = (d[j] * w[j]) + tsdf[j];
+                // weights are whole counts below the cap: RN(1/nw) from rtab
+                // (rtab[i] = 1.0f / (i + 1)) and div_by_rcp, else the division
+                const int wi = (int)w[j];
+                if (rtab && (float)wi == w[j] && wi < kRcpTable)
+                    d[j] = div_by_rcp(num, nw, rtab[wi]);
+                else
+                    d[j] = num / nw;
                 w[j] = nw < vp.maxw ? nw : vp.maxw;
             }
             dirty[j >> 2] = true;

@@ -211,5 +211,28 @@ __device__ __forceinline__ bool pixel_index(float pa, float f, float pz, float c
     return true;
 }
 
+// The correctly rounded quotient a/b from y = RN(1/b) (Markstein's
+// correction, P. Markstein, IBM J. Res. Dev. 34(1), 1990): q = RN(a*y),
+// r = a - b*q (exact, one FMA), RN(q + r*y) = RN(a/b) whenever a/b is in
+// the normal range (or zero: then the result is +0 where a = -0).  The
+// integer divisors 1..256 are checked against IEEE division for every
+// float32 mantissa of a, and random a, b over the normal range.
+__device__ __forceinline__ float div_by_rcp(float a, float b, float y) {
+    const float q = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
+}
+
+// pixel_index with rz = RN(1/pz) (1.0f / pz) shared by both axes: the same
+// index and validity.  Outside the normal range (pz near 0, or an overflow)
+// both quotients are infinite, NaN or below ulp(c) / 2, and both indices
+// invalid or equal after + c and the truncation.
+__device__ __forceinline__ bool pixel_index_rcp(float pa, float f, float pz, float rz, float c,
+                                                float size, int& out) {
+    const float s = div_by_rcp(pa * f, pz, rz) + c;
+    if (!(s > -1.0f && s < size)) return false;
+    out = (int)s;
+    return true;
+}
+
 
 }  // namespace recon

@@ -88,6 +88,8 @@ __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
     const unsigned seg_len, unsigned long long* counter, unsigned long long* touch_counter) {
     __shared__ unsigned s_pre[kMaxSegments + 1];
     __shared__ unsigned s_wave[kThreads / 64];
+    __shared__ float rtab[kRcpTable];
+    rtab[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);  // ordered by segment_offsets' barrier
     const unsigned n = __builtin_amdgcn_readfirstlane(segment_offsets(counts, nseg, s_pre, s_wave));
     const unsigned waves = gridDim.x * (kThreads / 64);
     const unsigned lane = threadIdx.x & 63;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void integrate_list_kernel(
             tile_origin<Shape>(vp, tile, x0, y0, z0);
             integrate_tile<Shape, FrameBlock<MAXF>, VEC, COUNT, COLOR, VARW>(vp, nframes, fb, frames,
                                                                              x0, y0, z0, cnt,
-                                                                             touched);
+                                                                             touched, rtab);
         }
     }
     if (COUNT && counter && cnt) atomicAdd(counter, (unsigned long long)cnt);
