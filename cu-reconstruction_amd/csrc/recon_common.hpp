@@ -214,9 +214,10 @@ __device__ __forceinline__ bool pixel_index(float pa, float f, float pz, float c
 // The correctly rounded quotient a/b from y = RN(1/b) (Markstein's
 // correction, P. Markstein, IBM J. Res. Dev. 34(1), 1990): q = RN(a*y),
 // r = a - b*q (exact, one FMA), RN(q + r*y) = RN(a/b) whenever a/b is in
-// the normal range (or zero: then the result is +0 where a = -0).  The
-// integer divisors 1..256 are checked against IEEE division for every
-// float32 mantissa of a, and random a, b over the normal range.
+// the normal range (or zero: then the result is +0 where a = -0).
+// tests/div_correction_check.c checks it against IEEE division for every
+// float32 mantissa of a with the integer divisors 1..256, and for random
+// a, b over the normal range.
 __device__ __forceinline__ float div_by_rcp(float a, float b, float y) {
     const float q = a * y;
     return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
