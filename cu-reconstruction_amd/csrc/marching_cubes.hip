@@ -130,32 +130,35 @@ __global__ __launch_bounds__(1024) void mc_tile_scan_kernel(const int32_t* __res
         }
         long long ex = before + incl - sum;
         int gex = gbefore + ginc - gsum;
-        long long o[kTileScanItems];
-        int go[kTileScanItems];
-#pragma unroll
-        for (int j = 0; j < kTileScanItems; ++j) {
-            o[j] = ex;
-            ex += v[j];
-            go[j] = gex;
-            gex += g[j];
-        }
+        // offsets written as they are formed (no array of them stays live)
         if (whole) {
             longlong2* d2 = reinterpret_cast<longlong2*>(tile_off + i0);
 #pragma unroll
-            for (int q = 0; q < kTileScanItems / 2; ++q) d2[q] = make_longlong2(o[2 * q], o[2 * q + 1]);
+            for (int q = 0; q < kTileScanItems / 2; ++q) {
+                const long long a = ex;
+                ex += v[2 * q];
+                d2[q] = make_longlong2(a, ex);
+                ex += v[2 * q + 1];
+            }
             if (group_off) {
                 int4* g4 = reinterpret_cast<int4*>(group_off + i0);
 #pragma unroll
-                for (int q = 0; q < kTileScanItems / 4; ++q)
-                    g4[q] = make_int4(go[4 * q], go[4 * q + 1], go[4 * q + 2], go[4 * q + 3]);
+                for (int q = 0; q < kTileScanItems / 4; ++q) {
+                    const int a = gex, b = a + g[4 * q], c = b + g[4 * q + 1], d = c + g[4 * q + 2];
+                    g4[q] = make_int4(a, b, c, d);
+                    gex = d + g[4 * q + 3];
+                }
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < kTileScanItems; ++j)
+            for (int j = 0; j < kTileScanItems; ++j) {
                 if (i0 + j < n) {
-                    tile_off[i0 + j] = o[j];
-                    if (group_off) group_off[i0 + j] = go[j];
+                    tile_off[i0 + j] = ex;
+                    if (group_off) group_off[i0 + j] = gex;
                 }
+                ex += v[j];
+                gex += g[j];
+            }
         }
         carry += all;
         gcarry += gall;
