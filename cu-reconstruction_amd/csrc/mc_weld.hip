@@ -268,45 +268,80 @@ __device__ __forceinline__ void weld_chunk_vertices(const WeldParams& w, int c, 
     }
 }
 
-// Vertex index of local edge e of cube (x, y, z).
-__device__ __forceinline__ int32_t weld_index(const WeldParams& w,
+
+// The chunk's vertices, then its cubes' triangles in the soup's order (one
+// wavefront; every lane calls).
+// Masks and first vertices of the 8 chunks a chunk's cubes reach (rows y,
+// y+1 by layers z, z+1, by x-chunks j, j+1), per wavefront in LDS, so that a
+// triangle's vertex indices need no global loads: neighbour n = 4 dy + 2 dz
+// + dj; chunks past the grid hold zeros (no cube reaches them).
+struct WeldNbr {
+    unsigned long long m[8][3];
+    long long base[8];
+};
+
+__device__ __forceinline__ void weld_load_nbr(const WeldParams& w, int c, int lane,
                                               const unsigned long long* __restrict__ masks,
-                                              const long long* __restrict__ base, int x, int y,
-                                              int z, int e) {
+                                              const long long* __restrict__ vbase, WeldNbr& nb) {
+    if (lane < 8) {
+        const int row = c / w.cpr, j = c - row * w.cpr;
+        const int y = row % w.mc.gy, z = row / w.mc.gy;
+        const int Y = y + ((lane >> 2) & 1), Z = z + ((lane >> 1) & 1), J = j + (lane & 1);
+        unsigned long long m0 = 0, m1 = 0, m2 = 0;
+        long long b = 0;
+        if (Y < w.mc.gy && Z < w.mc.gz && J < w.cpr) {
+            const int64_t cn = (int64_t)(Y + w.mc.gy * Z) * w.cpr + J;
+            m0 = masks[3 * cn];
+            m1 = masks[3 * cn + 1];
+            m2 = masks[3 * cn + 2];
+            b = vbase[cn];
+        }
+        nb.m[lane][0] = m0;
+        nb.m[lane][1] = m1;
+        nb.m[lane][2] = m2;
+        nb.base[lane] = b;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Vertex index of local edge e of cube (x, y, z) of the chunk whose
+// neighbours nb holds (x0 the chunk's first voxel): the base of the edge's
+// chunk, plus the set bits of the three masks below the edge's lower end,
+// plus that voxel's own bits of the axes before the edge's.
+__device__ __forceinline__ int32_t weld_index_nbr(const WeldNbr& nb, int x, int x0, int e) {
     // corner offsets packed as bits: dx = bit0, dy = bit1, dz = bit2
     // (0:000 1:001 2:101 3:100 4:010 5:011 6:111 7:110), as in edge_vertex
     constexpr unsigned k_off = (0u << 0) | (1u << 3) | (5u << 6) | (4u << 9) | (2u << 12) |
                                (3u << 15) | (7u << 18) | (6u << 21);
-    const unsigned oa = (k_off >> (3 * c_edge[e][0])) & 7u, ob = (k_off >> (3 * c_edge[e][1])) & 7u;
+    const int ca = (int)((kEdgeA >> (4 * e)) & 0xFull), cb = (int)((kEdgeB >> (4 * e)) & 0xFull);
+    const unsigned oa = (k_off >> (3 * ca)) & 7u, ob = (k_off >> (3 * cb)) & 7u;
     const int a = __builtin_ctz(oa ^ ob);  // the axis the edge runs along
     const unsigned lo = oa & ob;           // its lower end
-    const int X = x + (int)(lo & 1u), Y = y + (int)((lo >> 1) & 1u), Z = z + (int)(lo >> 2);
-    const int c = (Y + w.mc.gy * Z) * w.cpr + (X >> 6);
+    const int X = x + (int)(lo & 1u);
+    const int n = (int)(((lo >> 1) & 1u) << 2) | (int)((lo >> 2) << 1) | ((X >> 6) - (x0 >> 6));
     const int l = X & 63;
-    const unsigned long long m0 = masks[3 * (int64_t)c], m1 = masks[3 * (int64_t)c + 1],
-                             m2 = masks[3 * (int64_t)c + 2];
+    const unsigned long long m0 = nb.m[n][0], m1 = nb.m[n][1], m2 = nb.m[n][2];
     const unsigned long long below = (1ull << l) - 1ull;
-    long long idx = base[c] + __popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below);
+    long long idx = nb.base[n] + __popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below);
     if (a > 0) idx += (long long)((m0 >> l) & 1ull);
     if (a > 1) idx += (long long)((m1 >> l) & 1ull);
     return (int32_t)idx;
 }
 
-// The chunk's vertices, then its cubes' triangles in the soup's order (one
-// wavefront; every lane calls).
 __device__ __forceinline__ void weld_write_chunk(const WeldParams& w, int c, int lane,
                                                  const unsigned long long* __restrict__ masks,
                                                  const long long* __restrict__ vbase,
                                                  const long long* __restrict__ tbase, int nt_chunk,
                                                  float* __restrict__ verts,
                                                  float* __restrict__ normals,
-                                                 int32_t* __restrict__ tris) {
+                                                 int32_t* __restrict__ tris, WeldNbr& nb) {
     if (verts || normals) {
         const unsigned long long m[3] = {masks[3 * (int64_t)c], masks[3 * (int64_t)c + 1],
                                          masks[3 * (int64_t)c + 2]};
         if (m[0] | m[1] | m[2]) weld_chunk_vertices(w, c, lane, m, vbase[c], verts, normals);
     }
     if (!tris || nt_chunk == 0) return;
+    weld_load_nbr(w, c, lane, masks, vbase, nb);
     const ChunkCorners k = chunk_corners(w, c, lane);
     int cls = 0, nt = 0;
     if (k.cube) {
@@ -330,7 +365,7 @@ __device__ __forceinline__ void weld_write_chunk(const WeldParams& w, int c, int
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
-                tris[3 * t + q] = weld_index(w, masks, vbase, x0 + owner, k.y, k.z, e);
+                tris[3 * t + q] = weld_index_nbr(nb, x0 + owner, x0, e);
             }
         }
     }
@@ -349,9 +384,11 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
                                                          float* __restrict__ verts,
                                                          float* __restrict__ normals,
                                                          int32_t* __restrict__ tris) {
+    __shared__ WeldNbr nbr[kWeldWaves];
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * kWeldWaves + (threadIdx.x >> 6);
     if (row >= w.mc.gy * w.mc.gz) return;
+    WeldNbr& nb = nbr[threadIdx.x >> 6];
     const bool want_v = verts || normals;
     const int c0 = row * w.cpr, c1 = c0 + w.cpr;
     for (int cw = c0; cw < c1; cw += 64) {
@@ -363,7 +400,7 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
             const int l = __builtin_ctzll(busy);
             busy &= busy - 1;
             weld_write_chunk(w, cw + l, lane, masks, vbase, tbase, __shfl(nt, l, 64), verts,
-                             normals, tris);
+                             normals, tris, nb);
         }
     }
 }
