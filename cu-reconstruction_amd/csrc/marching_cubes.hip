@@ -48,123 +48,124 @@ namespace recon {
 constexpr int kRowsPerBlock = 4;  // one wavefront per row, 256 threads
 
 // ---- tile scan ------------------------------------------------------------------
-constexpr int kTileScanItems = 16;
+// The scan takes kScanRows rows of kScanRow tiles a pass, thread t holding
+// tiles 4t .. 4t+3 of each row, so every load and store instruction of a
+// wavefront covers contiguous bytes (one workgroup's loads and stores all go
+// through one CU).
+constexpr int kScanRow = 4096, kScanRows = 2;
 
-// Inclusive prefix sum of a 64-bit value over the lanes of a wavefront.
-__device__ __forceinline__ long long wave_inclusive_scan64(long long v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
+// A 64-bit DPP move: the same move applied to both halves.
+template <int Ctrl, int RowMask, int BankMask, bool BoundCtrl>
+__device__ __forceinline__ long long dpp64(long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, Ctrl, RowMask, BankMask, BoundCtrl);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), Ctrl, RowMask, BankMask, BoundCtrl);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
 }
 
-// One workgroup of 1024 threads, 16 consecutive tiles per thread (16-byte
-// loads and stores): tile_off[i] = sum of tile_sum[0..i) and tile_off[n] =
-// the total; when tile_nne is not null also group_off[i] = sum over tiles
-// before i of ceil(tile_nne / 4) (the emit's four-chunk groups) and
-// group_off[n] = their total.  All four arrays are 16-byte aligned.
+// Inclusive prefix sum of a 64-bit value over the 64 lanes of a wavefront:
+// the moves of wave_inclusive_scan on both halves, 64-bit adds.
+__device__ __forceinline__ long long wave_inclusive_scan64(long long v) {
+    long long t = v;
+    t += dpp64<0x111, 0xF, 0xF, true>(v);   // row_shr:1
+    t += dpp64<0x112, 0xF, 0xF, true>(v);   // row_shr:2
+    t += dpp64<0x113, 0xF, 0xF, true>(v);   // row_shr:3
+    t += dpp64<0x114, 0xF, 0xE, true>(t);   // row_shr:4, banks 1-3
+    t += dpp64<0x118, 0xF, 0xC, true>(t);   // row_shr:8, banks 2-3
+    t += dpp64<0x142, 0xA, 0xF, false>(t);  // row_bcast:15, rows 1, 3
+    t += dpp64<0x143, 0xC, 0xF, false>(t);  // row_bcast:31, rows 2, 3
+    return t;
+}
+
+// One workgroup of 1024 threads: tile_off[i] = sum of tile_sum[0..i) and
+// tile_off[n] = the total; when tile_nne is not null also group_off[i] = sum
+// over tiles before i of ceil(tile_nne / 4) (the emit's four-chunk groups)
+// and group_off[n] = their total.  All four arrays are 16-byte aligned.
 __global__ __launch_bounds__(1024) void mc_tile_scan_kernel(const int32_t* __restrict__ tile_sum,
                                                             const int32_t* __restrict__ tile_nne,
                                                             int n, long long* __restrict__ tile_off,
                                                             int32_t* __restrict__ group_off) {
-    __shared__ long long wtot[16];
-    __shared__ int gtot[16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ long long wtot[kScanRows][16];
+    __shared__ int gtot[kScanRows][16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     long long carry = 0;
     int gcarry = 0;
-    for (int base = 0; base < n; base += 1024 * kTileScanItems) {
-        const int i0 = base + (int)threadIdx.x * kTileScanItems;
-        const bool whole = i0 + kTileScanItems <= n;
-        int v[kTileScanItems], g[kTileScanItems];
-        if (whole) {
-            const int4* s4 = reinterpret_cast<const int4*>(tile_sum + i0);
+    for (int base = 0; base < n; base += kScanRow * kScanRows) {
+        int v[kScanRows][4], g[kScanRows][4];
+        long long sum[kScanRows], incl[kScanRows];
+        int gsum[kScanRows], ginc[kScanRows];
 #pragma unroll
-            for (int q = 0; q < kTileScanItems / 4; ++q) {
-                const int4 a = s4[q];
-                v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
-            }
-            if (tile_nne) {
-                const int4* n4 = reinterpret_cast<const int4*>(tile_nne + i0);
+        for (int q = 0; q < kScanRows; ++q) {
+            const int i0 = base + kScanRow * q + 4 * t;
+            if (i0 + 4 <= n) {
+                const int4 a = *reinterpret_cast<const int4*>(tile_sum + i0);
+                v[q][0] = a.x; v[q][1] = a.y; v[q][2] = a.z; v[q][3] = a.w;
+                if (tile_nne) {
+                    const int4 b = *reinterpret_cast<const int4*>(tile_nne + i0);
+                    g[q][0] = (b.x + 3) >> 2; g[q][1] = (b.y + 3) >> 2;
+                    g[q][2] = (b.z + 3) >> 2; g[q][3] = (b.w + 3) >> 2;
+                } else {
+                    g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0;
+                }
+            } else {
 #pragma unroll
-                for (int q = 0; q < kTileScanItems / 4; ++q) {
-                    const int4 a = n4[q];
-                    g[4 * q] = (a.x + 3) >> 2; g[4 * q + 1] = (a.y + 3) >> 2;
-                    g[4 * q + 2] = (a.z + 3) >> 2; g[4 * q + 3] = (a.w + 3) >> 2;
+                for (int k = 0; k < 4; ++k) {
+                    v[q][k] = i0 + k < n ? tile_sum[i0 + k] : 0;
+                    g[q][k] = (tile_nne && i0 + k < n) ? (tile_nne[i0 + k] + 3) >> 2 : 0;
                 }
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < kTileScanItems; ++j) {
-                v[j] = i0 + j < n ? tile_sum[i0 + j] : 0;
-                g[j] = (tile_nne && i0 + j < n) ? (tile_nne[i0 + j] + 3) >> 2 : 0;
-            }
+            sum[q] = (long long)v[q][0] + v[q][1] + v[q][2] + v[q][3];
+            gsum[q] = g[q][0] + g[q][1] + g[q][2] + g[q][3];
         }
-        if (!tile_nne) {
 #pragma unroll
-            for (int j = 0; j < kTileScanItems; ++j) g[j] = 0;
+        for (int q = 0; q < kScanRows; ++q) {
+            incl[q] = wave_inclusive_scan64(sum[q]);
+            ginc[q] = wave_inclusive_scan(gsum[q], lane);
         }
-        long long sum = 0;
-        int gsum = 0;
-#pragma unroll
-        for (int j = 0; j < kTileScanItems; ++j) {
-            sum += v[j];
-            gsum += g[j];
-        }
-        const long long incl = wave_inclusive_scan64(sum, lane);
-        const int ginc = wave_inclusive_scan(gsum, lane);
         if (lane == 63) {
-            wtot[wave] = incl;
-            gtot[wave] = ginc;
+#pragma unroll
+            for (int q = 0; q < kScanRows; ++q) {
+                wtot[q][wave] = incl[q];
+                gtot[q][wave] = ginc[q];
+            }
         }
         __syncthreads();
-        long long before = carry, all = 0;
-        int gbefore = gcarry, gall = 0;
 #pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            before += w < wave ? wtot[w] : 0;
-            all += wtot[w];
-            gbefore += w < wave ? gtot[w] : 0;
-            gall += gtot[w];
+        for (int q = 0; q < kScanRows; ++q) {
+            long long before = 0, all = 0;
+            int gbefore = 0, gall = 0;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) {
+                before += w < wave ? wtot[q][w] : 0;
+                all += wtot[q][w];
+                gbefore += w < wave ? gtot[q][w] : 0;
+                gall += gtot[q][w];
+            }
+            const long long e0 = carry + before + incl[q] - sum[q];
+            const long long e1 = e0 + v[q][0], e2 = e1 + v[q][1], e3 = e2 + v[q][2];
+            const int f0 = gcarry + gbefore + ginc[q] - gsum[q];
+            const int f1 = f0 + g[q][0], f2 = f1 + g[q][1], f3 = f2 + g[q][2];
+            const int i0 = base + kScanRow * q + 4 * t;
+            if (i0 + 4 <= n) {
+                longlong2* d2 = reinterpret_cast<longlong2*>(tile_off + i0);
+                d2[0] = make_longlong2(e0, e1);
+                d2[1] = make_longlong2(e2, e3);
+                if (group_off) *reinterpret_cast<int4*>(group_off + i0) = make_int4(f0, f1, f2, f3);
+            } else {
+                const long long e[4] = {e0, e1, e2, e3};
+                const int f[4] = {f0, f1, f2, f3};
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i0 + k < n) {
+                        tile_off[i0 + k] = e[k];
+                        if (group_off) group_off[i0 + k] = f[k];
+                    }
+            }
+            carry += all;
+            gcarry += gall;
         }
-        long long ex = before + incl - sum;
-        int gex = gbefore + ginc - gsum;
-        // offsets written as they are formed (no array of them stays live)
-        if (whole) {
-            longlong2* d2 = reinterpret_cast<longlong2*>(tile_off + i0);
-#pragma unroll
-            for (int q = 0; q < kTileScanItems / 2; ++q) {
-                const long long a = ex;
-                ex += v[2 * q];
-                d2[q] = make_longlong2(a, ex);
-                ex += v[2 * q + 1];
-            }
-            if (group_off) {
-                int4* g4 = reinterpret_cast<int4*>(group_off + i0);
-#pragma unroll
-                for (int q = 0; q < kTileScanItems / 4; ++q) {
-                    const int a = gex, b = a + g[4 * q], c = b + g[4 * q + 1], d = c + g[4 * q + 2];
-                    g4[q] = make_int4(a, b, c, d);
-                    gex = d + g[4 * q + 3];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < kTileScanItems; ++j) {
-                if (i0 + j < n) {
-                    tile_off[i0 + j] = ex;
-                    if (group_off) group_off[i0 + j] = gex;
-                }
-                ex += v[j];
-                gex += g[j];
-            }
-        }
-        carry += all;
-        gcarry += gall;
-        __syncthreads();
+        __syncthreads();  // wtot and gtot are rewritten by the next pass
     }
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         tile_off[n] = carry;
         if (group_off) group_off[n] = gcarry;
     }
