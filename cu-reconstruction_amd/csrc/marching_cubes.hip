@@ -54,28 +54,6 @@ constexpr int kRowsPerBlock = 4;  // one wavefront per row, 256 threads
 // through one CU).
 constexpr int kScanRow = 4096, kScanRows = 2;
 
-// A 64-bit DPP move: the same move applied to both halves.
-template <int Ctrl, int RowMask, int BankMask, bool BoundCtrl>
-__device__ __forceinline__ long long dpp64(long long v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, Ctrl, RowMask, BankMask, BoundCtrl);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), Ctrl, RowMask, BankMask, BoundCtrl);
-    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
-}
-
-// Inclusive prefix sum of a 64-bit value over the 64 lanes of a wavefront:
-// the moves of wave_inclusive_scan on both halves, 64-bit adds.
-__device__ __forceinline__ long long wave_inclusive_scan64(long long v) {
-    long long t = v;
-    t += dpp64<0x111, 0xF, 0xF, true>(v);   // row_shr:1
-    t += dpp64<0x112, 0xF, 0xF, true>(v);   // row_shr:2
-    t += dpp64<0x113, 0xF, 0xF, true>(v);   // row_shr:3
-    t += dpp64<0x114, 0xF, 0xE, true>(t);   // row_shr:4, banks 1-3
-    t += dpp64<0x118, 0xF, 0xC, true>(t);   // row_shr:8, banks 2-3
-    t += dpp64<0x142, 0xA, 0xF, false>(t);  // row_bcast:15, rows 1, 3
-    t += dpp64<0x143, 0xC, 0xF, false>(t);  // row_bcast:31, rows 2, 3
-    return t;
-}
-
 // One workgroup of 1024 threads: tile_off[i] = sum of tile_sum[0..i) and
 // tile_off[n] = the total; when tile_nne is not null also group_off[i] = sum
 // over tiles before i of ceil(tile_nne / 4) (the emit's four-chunk groups)
@@ -285,9 +263,14 @@ __global__ __launch_bounds__(256) void scan_block_sums_kernel(const int32_t* __r
     __shared__ long long sh[256];
     const int b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
     long long s = 0;
+    if (b0 + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {
+        const int4 a = *reinterpret_cast<const int4*>(in + b0);
+        s = ((long long)a.x + a.y) + ((long long)a.z + a.w);
+    } else {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (b0 + k < n) s += in[b0 + k];
+        for (int k = 0; k < kScanItems; ++k)
+            if (b0 + k < n) s += in[b0 + k];
+    }
     long long total;
     block_exclusive_scan(s, sh, &total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -313,19 +296,29 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t* __restri
                                                          long long* __restrict__ out) {
     __shared__ long long sh[256];
     const int b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    const bool vec = b0 + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(out) & 15) == 0;
     long long vals[kScanItems];
-    long long s = 0;
+    if (vec) {
+        const int4 a = *reinterpret_cast<const int4*>(in + b0);
+        vals[0] = a.x; vals[1] = a.y; vals[2] = a.z; vals[3] = a.w;
+    } else {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        vals[k] = (b0 + k < n) ? in[b0 + k] : 0;
-        s += vals[k];
+        for (int k = 0; k < kScanItems; ++k) vals[k] = (b0 + k < n) ? in[b0 + k] : 0;
     }
+    const long long s = (vals[0] + vals[1]) + (vals[2] + vals[3]);
     long long total;
-    long long ex = block_exclusive_scan(s, sh, &total) + sums[blockIdx.x];
+    const long long e0 = block_exclusive_scan(s, sh, &total) + sums[blockIdx.x];
+    const long long e1 = e0 + vals[0], e2 = e1 + vals[1], e3 = e2 + vals[2];
+    if (vec) {
+        longlong2* o2 = reinterpret_cast<longlong2*>(out + b0);
+        o2[0] = make_longlong2(e0, e1);
+        o2[1] = make_longlong2(e2, e3);
+    } else {
+        const long long e[kScanItems] = {e0, e1, e2, e3};
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (b0 + k < n) out[b0 + k] = ex;
-        ex += vals[k];
+        for (int k = 0; k < kScanItems; ++k)
+            if (b0 + k < n) out[b0 + k] = e[k];
     }
 }
 
