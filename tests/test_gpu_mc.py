@@ -80,6 +80,22 @@ def test_unaligned_field_takes_4_byte_loads(cuda):
     assert np.array_equal(tris.cpu().numpy(), ot)
 
 
+def test_many_tiles(cuda):
+    # 1024 x 1024 cube rows of 7 cubes: 65536 tiles of 16 rows, so the tile
+    # scan makes eight passes and group records carry tile indices past
+    # 2^16; a cylinder along x crosses a quarter of the rows, and noise in
+    # the last 64 layers puts triangles in most chunks there
+    gx, gy, gz = 8, 1025, 1025
+    y = np.arange(gy, dtype=np.float32)[None, :, None]
+    z = np.arange(gz, dtype=np.float32)[:, None, None]
+    f = np.sqrt((y - 500.0) ** 2 + (z - 480.0) ** 2) - np.float32(300.0) + np.zeros(gx, np.float32)
+    f = np.ascontiguousarray(f.astype(np.float32))
+    rng = np.random.default_rng(11)
+    f[-64:] = rng.standard_normal((64, gy, gx)).astype(np.float32)
+    n = _mc_compare(cuda, f, (gx, gy, gz), (0.01, 0.01, 0.01), (0.0, 0.0, 0.0))
+    assert n > 100000
+
+
 def test_sparse_chunks_of_wide_rows(cuda):
     # rows of 2199 cubes: 35 chunks of 64, so chunks 31-34 share the last bit
     # of the emit pass's chunk mask; 5 cube rows per layer, so each layer has
