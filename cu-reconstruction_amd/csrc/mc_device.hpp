@@ -1,8 +1,8 @@
 // mc_device.hpp -- device code of marching cubes shared by the single-volume
 // path (marching_cubes.hip) and the batched path (tsdf_batch.hip):
 // classification (marching_cubes.cu:76-84,118-130), edge intersection
-// (marching_cubes.cu:23-51), the per-row triangle count and emission in the
-// reference's output order, and a deterministic exclusive scan.
+// (marching_cubes.cu:23-51), the chunk-count pass, the group records and the
+// emission in the reference's output order.
 #pragma once
 
 #include "recon_common.hpp"
@@ -13,7 +13,6 @@
 namespace recon {
 
 static __constant__ uint64_t c_tri[256] = RECON_MC_TRI_PACKED_INIT;
-static __constant__ int c_edge[12][2] = RECON_MC_EDGE_CORNERS_INIT;
 
 // The edge table's corners packed 4 bits per edge into two immediates, so
 // that a lane's edge lookup is a shift, not a load.
@@ -241,37 +240,6 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
     return t;
 }
 
-// Bit of the 64-cube chunk starting at x0 in a row's chunk mask: chunk c for
-// c < 31, and bit 31 for every chunk from 31 on.
-__device__ __forceinline__ uint32_t chunk_bit(int x0) { return 1u << min(x0 >> 6, 31); }
-
-// Bitwise OR over the 64 lanes of a wavefront.
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v |= (uint32_t)__shfl_xor((int)v, d, 64);
-    return v;
-}
-
-// Triangles of cube row r = y + (gy-1) * z (one wavefront; every lane returns
-// the row total).  With `mask`, also the row's chunk mask (chunk_bit of every
-// chunk with a triangle).
-__device__ __forceinline__ int mc_row_triangles(const McParams& p, int r, int lane,
-                                                uint32_t* mask = nullptr) {
-    const int y = r % p.cyr, z = r / p.cyr;
-    int n = 0;
-    uint32_t m = 0;
-    CornerStream cs(p, y, z, lane);
-    for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        float v[8];
-        cs.step(x0, lane, v);
-        const int t = x0 + lane < p.cx ? class_ntri(cube_class(v)) : 0;
-        n += t;
-        m |= t > 0 ? chunk_bit(x0) : 0u;
-    }
-    if (mask) *mask = wave_or(m);
-    return wave_sum(n);
-}
-
 // compute_intersection (marching_cubes.cu:23-51): orient the edge so that it
 // runs from the negative corner when start > 0 and end < 0, then
 // p = s + ((-ws) / (we - ws)) * (e - s), component by component.
@@ -298,87 +266,6 @@ __device__ __forceinline__ void edge_vertex(const McParams& p, int x, int y, int
         const float edge = en - s;
         const float o = ratio * edge;
         out[k] = s + o;
-    }
-}
-
-// Writes the triangles of the 64 cubes (x0 + lane, y, z) with corners v
-// starting at triangle `base` (one wavefront) and returns their number: a
-// wave-level prefix sum of the per-cube counts places each cube's triangles
-// in x order, slots in table order, and the triangles are spread over the 64
-// lanes (not one lane per cube), so a cube with 5 triangles does not hold up
-// the wave.  Triangles at or past `capacity` are not written.  Vertex indices
-// (tris, may be null) are 3t+k + tri_index_base.
-__device__ __forceinline__ int mc_emit_chunk(const McParams& p, int x0, int y, int z, int lane,
-                                             const float v[8], long long base,
-                                             float* __restrict__ verts,
-                                             int32_t* __restrict__ tris, long long capacity,
-                                             long long tri_index_base) {
-    const int x = x0 + lane;
-    int c = 0, nt = 0;
-    if (x < p.cx) {
-        c = cube_class(v);
-        nt = class_ntri(c);
-    }
-    const int incl = wave_inclusive_scan(nt, lane);
-    const int excl = incl - nt;
-    const int chunk_total = __shfl(incl, 63, 64);
-    // triangle k of the chunk (k = 0 .. chunk_total-1, in the soup's
-    // order) is written by lane k % 64: its cube is the lane whose
-    // triangles include k, its slot k minus that lane's first triangle
-    for (int k0 = 0; k0 < chunk_total; k0 += 64) {
-        const int k = k0 + lane;
-        const int owner = wave_owner(incl, k);
-        const int oc = __shfl(c, owner, 64);
-        const int slot = k - __shfl(excl, owner, 64);
-        float ov[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) ov[i] = __shfl(v[i], owner, 64);
-        const long long t = base + k;
-        if (k < chunk_total && t < capacity) {
-            const uint64_t word = c_tri[oc];
-            const int ox = x0 + owner;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int e = (int)((word >> (4 + 4 * (3 * slot + q))) & 0xFull);
-                float pt[3];
-                edge_vertex(p, ox, y, z, ov, e, pt);
-                float* dst = verts + 9 * t + 3 * q;
-                dst[0] = pt[0];
-                dst[1] = pt[1];
-                dst[2] = pt[2];
-                if (tris) tris[3 * t + q] = (int32_t)(3 * (t - tri_index_base) + q);
-            }
-        }
-    }
-    return chunk_total;
-}
-
-// Writes the triangles of cube row r starting at triangle `base` (one
-// wavefront), its 64-cube chunks in x order (mc_emit_chunk).  Only chunks
-// that `mask` marks (chunk_bit; a superset of those holding a triangle) are
-// read, their corners loaded directly (x and x+1 of each of the four voxel
-// rows); the others hold no triangle and move nothing.
-__device__ __forceinline__ void mc_emit_row_masked(const McParams& p, int r, int lane,
-                                                   long long base, uint32_t mask,
-                                                   float* __restrict__ verts,
-                                                   int32_t* __restrict__ tris,
-                                                   long long capacity) {
-    const int y = r % p.cyr, z = r / p.cyr;
-    const int sy = p.gx, sz = p.gx * p.gy;
-    const int r0 = sy * y + sz * z;
-    // voxel rows (y,z), (y,z+1), (y+1,z), (y+1,z+1), as CornerStream
-    const int rows[4] = {r0, r0 + sz, r0 + sy, r0 + sy + sz};
-    for (int x0 = 0; x0 < p.cx; x0 += 64) {
-        if (!(mask & chunk_bit(x0))) continue;
-        const int x = x0 + lane;
-        float lo[4], hi[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            lo[k] = x < p.gx ? p.f[rows[k] + x] : 0.0f;
-            hi[k] = x + 1 < p.gx ? p.f[rows[k] + x + 1] : 0.0f;
-        }
-        const float v[8] = {lo[0], hi[0], hi[1], lo[1], lo[2], hi[2], hi[3], lo[3]};
-        base += mc_emit_chunk(p, x0, y, z, lane, v, base, verts, tris, capacity, 0);
     }
 }
 

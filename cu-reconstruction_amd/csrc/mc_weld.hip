@@ -353,7 +353,8 @@ __device__ __forceinline__ void weld_write_chunk(const WeldParams& w, int c, int
     const int total = __shfl(incl, 63, 64);
     const long long t0 = tbase[c];
     const int x0 = k.x - lane;
-    // triangle kk of the chunk is written by lane kk % 64 (as in mc_emit_chunk)
+    // triangle kk of the chunk is written by lane kk % 64 (its cube found by
+    // wave_owner over the lanes' prefix counts)
     for (int k0 = 0; k0 < total; k0 += 64) {
         const int kk = k0 + lane;
         const int owner = wave_owner(incl, kk);
