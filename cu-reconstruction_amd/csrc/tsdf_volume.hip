@@ -243,23 +243,22 @@ static void launch_list(bool varw, int num_cus, hipStream_t s, const VolumeParam
                            la.seg_len, ctr, tctr);
 }
 
-// The three launches of one integrate sweep over `nframes` frames (frame f at
-// depth0 + f * stride_floats).
+// Depth bounds and cull of one integrate sweep over `nframes` frames (frame f
+// at depth0 + f * stride_floats) on stream s, into `bounds` and the cull
+// list `list` / `counts`; fills the frames' bound pointers and `la`.
 template <int MAXF>
-static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* depth0,
-                            size_t stride_floats, int pitch, int nframes, FrameBlock<MAXF>& fb,
-                            hipStream_t s) {
-    int rc = ensure_scratch(h, vp, s);
-    if (rc) return rc;
+static int launch_prestage(const VolumeParams& vp, const float* depth0, size_t stride_floats,
+                           int pitch, int nframes, FrameBlock<MAXF>& fb, float* bounds,
+                           ListEntry* list, unsigned* counts, hipStream_t s, ListArgs* la) {
     const size_t per = (size_t)vp.b0x * vp.b0y + (size_t)vp.b1x * vp.b1y;
     for (int f = 0; f < nframes; ++f) {
-        fb.f[f].bound0 = h->bounds + per * f;
+        fb.f[f].bound0 = bounds + per * f;
         fb.f[f].bound1 = fb.f[f].bound0 + (size_t)vp.b0x * vp.b0y;
     }
     BoundsJob job;
     job.depth = depth0;
-    job.bound0 = h->bounds;
-    job.bound1 = h->bounds + (size_t)vp.b0x * vp.b0y;
+    job.bound0 = bounds;
+    job.bound1 = bounds + (size_t)vp.b0x * vp.b0y;
     job.pitch = pitch;
     job.cols = vp.cols;
     job.rows = vp.rows;
@@ -274,13 +273,19 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
                        stride_floats, per);
     RECON_HIP_TRY(hipGetLastError(), "depth_bounds_kernel launch");
     const int ntiles = vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    ListArgs la;
-    la.list = static_cast<ListEntry*>(h->list);
-    la.counts = h->list_count;
-    cull_segments(ntiles, &la.seg_len, &la.nseg);
-    hipLaunchKernelGGL((cull_kernel<MAXF>), dim3(la.nseg), dim3(256), 0, s, vp, nframes, fb,
-                       static_cast<ListEntry*>(h->list), h->list_count, la.seg_len);
+    la->list = list;
+    la->counts = counts;
+    cull_segments(ntiles, &la->seg_len, &la->nseg);
+    hipLaunchKernelGGL((cull_kernel<MAXF>), dim3(la->nseg), dim3(256), 0, s, vp, nframes, fb, list,
+                       counts, la->seg_len);
     RECON_HIP_TRY(hipGetLastError(), "cull_kernel launch");
+    return RECON_OK;
+}
+
+// The list kernel of one sweep (after launch_prestage's cull) on stream s.
+template <int MAXF>
+static int launch_liststage(recon_tsdf* h, const VolumeParams& vp, int nframes,
+                            const FrameBlock<MAXF>& fb, const ListArgs& la, hipStream_t s) {
     const bool vec = (vp.gx % 4) == 0;
     const bool varw = h->info.weight_type == RECON_WEIGHT_BY_VARIANCE;
     unsigned long long* ctr = h->update_counter;
@@ -307,6 +312,21 @@ static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* 
     }
     RECON_HIP_TRY(hipGetLastError(), "integrate_list_kernel launch");
     return RECON_OK;
+}
+
+// The three launches of one integrate sweep over `nframes` frames (frame f at
+// depth0 + f * stride_floats), all on stream s.
+template <int MAXF>
+static int launch_integrate(recon_tsdf* h, const VolumeParams& vp, const float* depth0,
+                            size_t stride_floats, int pitch, int nframes, FrameBlock<MAXF>& fb,
+                            hipStream_t s) {
+    int rc = ensure_scratch(h, vp, s);
+    if (rc) return rc;
+    ListArgs la;
+    rc = launch_prestage<MAXF>(vp, depth0, stride_floats, pitch, nframes, fb, h->bounds,
+                               static_cast<ListEntry*>(h->list), h->list_count, s, &la);
+    if (rc) return rc;
+    return launch_liststage<MAXF>(h, vp, nframes, fb, la, s);
 }
 
 static VolumeParams volume_params(const recon_tsdf* h, int cols, int rows) {
