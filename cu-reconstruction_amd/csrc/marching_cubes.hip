@@ -466,10 +466,9 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
     RECON_HIP_TRY(hipGetLastError(), "mc_chunk_count_kernel launch");
     int rc = launch_tile_scan(tile_sum, L.tiles, tile_off, s, tile_nne, group_off);
     if (rc) return rc;
-    long long host_total = 0;
-    RECON_HIP_TRY(hipMemcpyAsync(&host_total, tile_off + L.tiles, sizeof(long long),
-                                 hipMemcpyDeviceToHost, s),
-                  "marching cubes total");
+    // The emit checks the capacity on the device, so every launch is queued
+    // before the host waits: the copy of the total to pageable memory comes
+    // last, since it returns only when the stream has reached it.
     if (!count_only && capacity > 0 && verts) {
         if (use_rec) {
             hipLaunchKernelGGL(mc_group_prep_kernel, dim3((L.tiles + 3) / 4), dim3(256), 0, s, p,
@@ -485,6 +484,10 @@ int mc_run(const float* field, const int32_t grid[3], const float vs[3], const f
             RECON_HIP_TRY(hipGetLastError(), "mc_chunk_emit_search_kernel launch");
         }
     }
+    long long host_total = 0;
+    RECON_HIP_TRY(hipMemcpyAsync(&host_total, tile_off + L.tiles, sizeof(long long),
+                                 hipMemcpyDeviceToHost, s),
+                  "marching cubes total");
     RECON_HIP_TRY(hipStreamSynchronize(s), "marching cubes synchronize");
     *total = host_total;
     return RECON_OK;

@@ -375,13 +375,18 @@ __device__ __forceinline__ void weld_write_chunk(const WeldParams& w, int c, int
 // One wavefront per voxel row: one load of the vertex and triangle counts
 // of 64 of the row's chunks at a time, a ballot of the chunks with either,
 // and weld_write_chunk for each of those (most of a volume's chunks hold
-// neither, and an empty row costs one load).
+// neither, and an empty row costs one load).  Nothing is written when the
+// totals (*vtotal, *ttotal, from the scans) exceed int32 indices or a
+// requested output's capacity: the host reports that after the call.
 __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
                                                          const unsigned long long* __restrict__ masks,
                                                          const int32_t* __restrict__ vcount,
                                                          const long long* __restrict__ vbase,
                                                          const int32_t* __restrict__ tcount,
                                                          const long long* __restrict__ tbase,
+                                                         const long long* __restrict__ vtotal,
+                                                         const long long* __restrict__ ttotal,
+                                                         long long vcap, long long tcap,
                                                          float* __restrict__ verts,
                                                          float* __restrict__ normals,
                                                          int32_t* __restrict__ tris) {
@@ -391,6 +396,8 @@ __global__ __launch_bounds__(256) void weld_write_kernel(const WeldParams w,
     if (row >= w.mc.gy * w.mc.gz) return;
     WeldNbr& nb = nbr[threadIdx.x >> 6];
     const bool want_v = verts || normals;
+    const long long nvt = *vtotal, ntt = *ttotal;
+    if (nvt > INT32_MAX || (want_v && nvt > vcap) || (tris && ntt > tcap)) return;
     const int c0 = row * w.cpr, c1 = c0 + w.cpr;
     for (int cw = c0; cw < c1; cw += 64) {
         const int c = cw + lane;
@@ -465,13 +472,24 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     if (rc) return rc;
     rc = launch_exclusive_scan(tcount, w.nchunks, tbase, tsums, s);
     if (rc) return rc;
+    // The write pass checks the totals on the device, so it is queued before
+    // the host waits; the copies of the totals to pageable memory come last,
+    // since each returns only when the stream has reached it.
+    const bool write = verts || normals || tris;
+    if (write) {
+        const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
+        hipLaunchKernelGGL(weld_write_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vcount,
+                           vbase, tcount, tbase, vsums + nb, tsums + nb, (long long)vcap,
+                           (long long)tcap, verts, normals, tris);
+        RECON_HIP_TRY(hipGetLastError(), "weld_write_kernel launch");
+    }
     long long totals[2] = {0, 0};
     RECON_HIP_TRY(hipMemcpyAsync(&totals[0], vsums + nb, 8, hipMemcpyDeviceToHost, s), "vertex total");
     RECON_HIP_TRY(hipMemcpyAsync(&totals[1], tsums + nb, 8, hipMemcpyDeviceToHost, s), "triangle total");
-    RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes counts");
+    RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes");
     *nv_out = totals[0];
     *nt_out = totals[1];
-    if (!verts && !normals && !tris) return RECON_OK;
+    if (!write) return RECON_OK;
     if (totals[0] > INT32_MAX)
         return fail(RECON_ERR_RANGE, "%s: %lld vertices exceed int32 indices", fn, totals[0]);
     if ((verts || normals) && totals[0] > vcap)
@@ -480,13 +498,6 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     if (tris && totals[1] > tcap)
         return fail(RECON_ERR_CAPACITY, "%s: %lld triangles do not fit capacity %lld", fn, totals[1],
                     (long long)tcap);
-    if (totals[0] > 0 || totals[1] > 0) {
-        const int blocks = (grid[1] * grid[2] + kWeldWaves - 1) / kWeldWaves;
-        hipLaunchKernelGGL(weld_write_kernel, dim3(blocks), dim3(256), 0, s, w, masks, vcount,
-                           vbase, tcount, tbase, verts, normals, tris);
-        RECON_HIP_TRY(hipGetLastError(), "weld_write_kernel launch");
-    }
-    RECON_HIP_TRY(hipStreamSynchronize(s), "welded marching cubes");
     return RECON_OK;
 }
 
