@@ -258,9 +258,9 @@ __global__ __launch_bounds__(256) void mc_class_hist_kernel(const McParams p,
     if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void scan_block_sums_kernel(const int32_t* __restrict__ in, int n,
-                                                              long long* __restrict__ sums) {
+__global__ __launch_bounds__(256) void scan_block_sums_kernel(const ScanJobs j, int n) {
     __shared__ long long sh[256];
+    const int32_t* __restrict__ in = blockIdx.y ? j.in[1] : j.in[0];
     const int b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
     long long s = 0;
     if (b0 + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {
@@ -273,28 +273,41 @@ __global__ __launch_bounds__(256) void scan_block_sums_kernel(const int32_t* __r
     }
     long long total;
     block_exclusive_scan(s, sh, &total);
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) (blockIdx.y ? j.sums[1] : j.sums[0])[blockIdx.x] = total;
 }
 
-// One workgroup: exclusive scan of the block sums in place, total at [nb].
-__global__ __launch_bounds__(256) void scan_top_kernel(long long* __restrict__ sums, int nb) {
+// One workgroup per array: exclusive scan of its block sums in place, total
+// at [nb]; each thread takes kTopItems consecutive sums per pass, so up to
+// 256 * kTopItems sums need one workgroup scan.
+constexpr int kTopItems = 8;
+__global__ __launch_bounds__(256) void scan_top_kernel(const ScanJobs j, int nb) {
     __shared__ long long sh[256];
+    long long* __restrict__ sums = blockIdx.y ? j.sums[1] : j.sums[0];
     long long carry = 0;
-    for (int base = 0; base < nb; base += 256) {
-        const int i = base + threadIdx.x;
-        const long long v = i < nb ? sums[i] : 0;
+    for (int base = 0; base < nb; base += 256 * kTopItems) {
+        const int i0 = base + threadIdx.x * kTopItems;
+        long long v[kTopItems], s = 0;
+#pragma unroll
+        for (int k = 0; k < kTopItems; ++k) {
+            v[k] = i0 + k < nb ? sums[i0 + k] : 0;
+            s += v[k];
+        }
         long long total;
-        const long long ex = block_exclusive_scan(v, sh, &total);
-        if (i < nb) sums[i] = carry + ex;
+        long long e = carry + block_exclusive_scan(s, sh, &total);
+#pragma unroll
+        for (int k = 0; k < kTopItems; ++k) {
+            if (i0 + k < nb) sums[i0 + k] = e;
+            e += v[k];
+        }
         carry += total;
     }
     if (threadIdx.x == 0) sums[nb] = carry;
 }
 
-__global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t* __restrict__ in, int n,
-                                                         const long long* __restrict__ sums,
-                                                         long long* __restrict__ out) {
+__global__ __launch_bounds__(256) void scan_apply_kernel(const ScanJobs j, int n) {
     __shared__ long long sh[256];
+    const int32_t* __restrict__ in = blockIdx.y ? j.in[1] : j.in[0];
+    long long* __restrict__ out = blockIdx.y ? j.out[1] : j.out[0];
     const int b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
     const bool vec = b0 + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
                      (reinterpret_cast<uintptr_t>(out) & 15) == 0;
@@ -308,7 +321,8 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t* __restri
     }
     const long long s = (vals[0] + vals[1]) + (vals[2] + vals[3]);
     long long total;
-    const long long e0 = block_exclusive_scan(s, sh, &total) + sums[blockIdx.x];
+    const long long e0 =
+        block_exclusive_scan(s, sh, &total) + (blockIdx.y ? j.sums[1] : j.sums[0])[blockIdx.x];
     const long long e1 = e0 + vals[0], e2 = e1 + vals[1], e3 = e2 + vals[2];
     if (vec) {
         longlong2* o2 = reinterpret_cast<longlong2*>(out + b0);
@@ -322,18 +336,29 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t* __restri
     }
 }
 
-int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
-                          hipStream_t s) {
+int launch_exclusive_scans(const ScanJobs& j, int count, int n, hipStream_t s) {
+    if (count < 1 || count > 2) return fail(RECON_ERR_SIZE, "launch_exclusive_scans: %d arrays", count);
     const int nb = (n + kScanBlock - 1) / kScanBlock;
-    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb), dim3(256), 0, s, counts, n, sums);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nb, count), dim3(256), 0, s, j, n);
     RECON_HIP_TRY(hipGetLastError(), "scan_block_sums_kernel launch");
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, s, sums, nb);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1, count), dim3(256), 0, s, j, nb);
     RECON_HIP_TRY(hipGetLastError(), "scan_top_kernel launch");
-    if (offsets) {
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, s, counts, n, sums, offsets);
+    bool apply = true;
+    for (int i = 0; i < count; ++i) apply = apply && j.out[i];
+    if (apply) {
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb, count), dim3(256), 0, s, j, n);
         RECON_HIP_TRY(hipGetLastError(), "scan_apply_kernel launch");
     }
     return RECON_OK;
+}
+
+int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
+                          hipStream_t s) {
+    ScanJobs j;
+    j.in[0] = j.in[1] = counts;
+    j.sums[0] = j.sums[1] = sums;
+    j.out[0] = j.out[1] = offsets;
+    return launch_exclusive_scans(j, 1, n, s);
 }
 
 static McParams make_params(const float* field, const int32_t grid[3], const float vs[3],

@@ -468,9 +468,14 @@ static int weld_run(const float* field, const int32_t grid[3], const float vs[3]
     hipLaunchKernelGGL(weld_count_kernel, dim3(count_blocks), dim3(256), 0, s, w, masks, vcount,
                        tcount);
     RECON_HIP_TRY(hipGetLastError(), "weld_count_kernel launch");
-    int rc = launch_exclusive_scan(vcount, w.nchunks, vbase, vsums, s);
-    if (rc) return rc;
-    rc = launch_exclusive_scan(tcount, w.nchunks, tbase, tsums, s);
+    ScanJobs jobs;
+    jobs.in[0] = vcount;
+    jobs.in[1] = tcount;
+    jobs.sums[0] = vsums;
+    jobs.sums[1] = tsums;
+    jobs.out[0] = vbase;
+    jobs.out[1] = tbase;
+    const int rc = launch_exclusive_scans(jobs, 2, w.nchunks, s);
     if (rc) return rc;
     // The write pass checks the totals on the device, so it is queued before
     // the host waits; the copies of the totals to pageable memory come last,

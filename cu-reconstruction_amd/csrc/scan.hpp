@@ -60,6 +60,17 @@ __device__ __forceinline__ long long block_exclusive_scan(long long v, long long
 int launch_exclusive_scan(const int32_t* counts, int n, long long* offsets, long long* sums,
                           hipStream_t s);
 
+// Arrays of equal length scanned by one set of the three launches (the
+// launches' blockIdx.y picks the array): in[i] into out[i], with sums[i]
+// as above (nb + 1 long longs each); the apply launch runs when every out[i]
+// is set.
+struct ScanJobs {
+    const int32_t* in[2];
+    long long* sums[2];
+    long long* out[2];
+};
+int launch_exclusive_scans(const ScanJobs& jobs, int count, int n, hipStream_t s);
+
 // One workgroup: tile_off[i] = sum of tile_sum[0..i) for i <= n (int64;
 // tile_off[n] is the total); with tile_nne, also group_off[i] = sum of
 // ceil(tile_nne / 4) over the tiles before i (group_off[n] the total).  The
