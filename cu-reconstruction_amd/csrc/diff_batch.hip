@@ -116,25 +116,25 @@ __global__ __launch_bounds__(256) void dbatch_fill_kernel(const DVol* __restrict
     fill4(vp.D, vp.W, (long long)(b - block_start[v]) * kFill + 4 * threadIdx.x, n, vp.trunc);
 }
 
-__global__ __launch_bounds__(256) void dbatch_cull_kernel(const DVol* __restrict__ vols,
-                                                          const FrameParams* __restrict__ table,
-                                                          const uint2* __restrict__ wg,
-                                                          ListEntry* __restrict__ list,
-                                                          unsigned* __restrict__ list_count) {
-    __shared__ uint64_t s_live[4][64];
-    __shared__ unsigned s_wave[kThreads / 64];
-    __shared__ unsigned s_base;
+__global__ __launch_bounds__(64 * kCullWaves) void dbatch_cull_kernel(
+    const DVol* __restrict__ vols, const FrameParams* __restrict__ table,
+    const uint2* __restrict__ wg, ListEntry* __restrict__ list, unsigned* __restrict__ list_count) {
+    __shared__ uint64_t s_live[kCullWaves][64];
     const uint2 e = wg[blockIdx.x];  // (volume, first tile)
     const DVol& dv = vols[e.x];
     const uint32_t ntiles = (uint32_t)dv.vp.tiles_x * dv.vp.tiles_y * dv.vp.tiles_z;
     const int i = threadIdx.x & 63, g = threadIdx.x >> 6;
     const uint32_t id = e.y + i;
     s_live[g][i] = id < ntiles ? tile_frames<Shape>(dv.vp, dv.nframes,
-                                                    FrameTable{table + dv.first_frame}, id, g, 4)
+                                                    FrameTable{table + dv.first_frame}, id, g,
+                                                    kCullWaves)
                                : 0ull;
     __syncthreads();
-    const uint64_t live = g == 0 ? (s_live[0][i] | s_live[1][i]) | (s_live[2][i] | s_live[3][i]) : 0ull;
-    append_tile(live, id, e.x, list, list_count, s_wave, &s_base);
+    if (g != 0) return;
+    uint64_t live = 0;
+#pragma unroll
+    for (int k = 0; k < kCullWaves; ++k) live |= s_live[k][i];
+    append_wave(live, id, e.x, list, list_count);
 }
 
 __global__ __launch_bounds__(kThreads) void dbatch_integrate_kernel(
@@ -721,8 +721,8 @@ int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t 
                        d_frames, poses, total_frames, table);
     hipLaunchKernelGGL(dbatch_fill_kernel, dim3(fill_blocks), dim3(256), 0, s, d_vols, d_fill_start,
                        n, list_count);
-    hipLaunchKernelGGL(dbatch_cull_kernel, dim3(cull_blocks), dim3(256), 0, s, d_vols, table,
-                       d_cull_wg, list, list_count);
+    hipLaunchKernelGGL(dbatch_cull_kernel, dim3(cull_blocks), dim3(64 * kCullWaves), 0, s, d_vols,
+                       table, d_cull_wg, list, list_count);
     hipLaunchKernelGGL(dbatch_integrate_kernel, dim3(integrate_grid), dim3(kThreads), 0, s, d_vols,
                        table, list, list_count);
     RECON_HIP_TRY(hipGetLastError(), "diff batch forward launch");

@@ -241,6 +241,25 @@ __device__ __forceinline__ void append_tile(uint64_t live, uint32_t tile, uint32
     if (n) list[*s_base + wave_base + incl - n] = ListEntry{tile, volume, live};
 }
 
+// Appends the tiles of one wavefront (lane l: `tile`, frame mask `live`)
+// that some frame reaches to `list`, with one atomic for the wavefront.
+// Every lane of the calling wavefront must call.
+__device__ __forceinline__ void append_wave(uint64_t live, uint32_t tile, uint32_t volume,
+                                            ListEntry* __restrict__ list,
+                                            unsigned* __restrict__ list_count) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t ball = __ballot(live != 0);
+    if (!ball) return;
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(list_count, (unsigned)__popcll(ball));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if (live) list[base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull))] = ListEntry{tile, volume, live};
+}
+
+// Wavefronts per workgroup of the frame-parallel culls: wave g of a
+// workgroup tests frames g, g + kCullWaves, ... of the workgroup's 64 tiles.
+constexpr int kCullWaves = 16;
+
 // ---- segmented cull list (no global counter) -------------------------------------
 // A cull workgroup owns a segment of the list (the entries of its own tiles,
 // at most one per tile) and writes only its count: no returning atomic, whose
