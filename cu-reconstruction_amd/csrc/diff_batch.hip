@@ -123,17 +123,15 @@ __global__ __launch_bounds__(64 * kCullWaves) void dbatch_cull_kernel(
     const uint2 e = wg[blockIdx.x];  // (volume, first tile)
     const DVol& dv = vols[e.x];
     const uint32_t ntiles = (uint32_t)dv.vp.tiles_x * dv.vp.tiles_y * dv.vp.tiles_z;
-    const int i = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int i = threadIdx.x & 63, g = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     const uint32_t id = e.y + i;
     s_live[g][i] = id < ntiles ? tile_frames<Shape>(dv.vp, dv.nframes,
-                                                    FrameTable{table + dv.first_frame}, id, g,
-                                                    kCullWaves)
+                                                    FrameTable{table + dv.first_frame}, id, g, nw)
                                : 0ull;
     __syncthreads();
     if (g != 0) return;
     uint64_t live = 0;
-#pragma unroll
-    for (int k = 0; k < kCullWaves; ++k) live |= s_live[k][i];
+    for (int k = 0; k < nw; ++k) live |= s_live[k][i];
     append_wave(live, id, e.x, list, list_count);
 }
 
@@ -667,6 +665,7 @@ struct recon_diff_batch {
     int fill_blocks = 0;
     uint2* d_cull_wg = nullptr;
     int cull_blocks = 0;
+    int cull_nw = 1;  // wavefronts per cull workgroup (recon::cull_waves)
     ListEntry* list = nullptr;
     unsigned* list_count = nullptr;
     int integrate_grid = 0;
@@ -721,7 +720,7 @@ int recon_diff_batch::forward(const float* poses, double* loss_out, hipStream_t 
                        d_frames, poses, total_frames, table);
     hipLaunchKernelGGL(dbatch_fill_kernel, dim3(fill_blocks), dim3(256), 0, s, d_vols, d_fill_start,
                        n, list_count);
-    hipLaunchKernelGGL(dbatch_cull_kernel, dim3(cull_blocks), dim3(64 * kCullWaves), 0, s, d_vols,
+    hipLaunchKernelGGL(dbatch_cull_kernel, dim3(cull_blocks), dim3(64 * cull_nw), 0, s, d_vols,
                        table, d_cull_wg, list, list_count);
     hipLaunchKernelGGL(dbatch_integrate_kernel, dim3(integrate_grid), dim3(kThreads), 0, s, d_vols,
                        table, list, list_count);
@@ -1025,6 +1024,9 @@ static int plan(const recon_diff_volume* vols, int n, recon_diff_batch* b, hipSt
             hipSuccess || occ < 1)
         occ = 2;
     b->integrate_grid = cus * occ;
+    int most_frames = 1;
+    for (const DVol& dv : b->h_vols) most_frames = std::max(most_frames, dv.nframes);
+    b->cull_nw = cull_waves(b->cull_blocks, most_frames, cus);
     // the host tables must outlive the copies
     RECON_HIP_TRY(hipStreamSynchronize(s), "diff batch create synchronize");
     return RECON_OK;

@@ -256,9 +256,21 @@ __device__ __forceinline__ void append_wave(uint64_t live, uint32_t tile, uint32
     if (live) list[base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull))] = ListEntry{tile, volume, live};
 }
 
-// Wavefronts per workgroup of the frame-parallel culls: wave g of a
-// workgroup tests frames g, g + kCullWaves, ... of the workgroup's 64 tiles.
+// Most wavefronts per workgroup of the frame-parallel culls: with W =
+// blockDim.x / 64 of them (W = min(kCullWaves, frames)), wave g of a
+// workgroup tests frames g, g + W, ... of the workgroup's 64 tiles.
 constexpr int kCullWaves = 16;
+
+// W for `groups` workgroups of 64 tiles and at most `frames` frames a tile:
+// about 32 wavefronts per CU in all (8 per SIMD), at most one per frame and
+// at most kCullWaves (more wavefronts per tile only lengthen the workgroups
+// when the groups already fill the device).
+inline int cull_waves(long long groups, int frames, int cus) {
+    long long w = groups > 0 ? (32LL * cus + groups - 1) / groups : 1;
+    if (w > frames) w = frames;
+    if (w > kCullWaves) w = kCullWaves;
+    return w < 1 ? 1 : (int)w;
+}
 
 // ---- segmented cull list (no global counter) -------------------------------------
 // A cull workgroup owns a segment of the list (the entries of its own tiles,

@@ -76,21 +76,21 @@ __global__ __launch_bounds__(256) void diff_bounds_kernel(const BoundsJob job, s
     bounds_block(j, blockIdx.x, blockIdx.y, s0, s1);
 }
 
-// 64 tiles per workgroup; wave g tests frames g, g + kCullWaves, ... of every
-// tile, and wave 0 ORs the tile's masks and appends it.
+// 64 tiles per workgroup of W = blockDim.x / 64 wavefronts (at most
+// kCullWaves); wave g tests frames g, g + W, ... of every tile, and wave 0
+// ORs the tile's masks and appends it.
 __global__ __launch_bounds__(64 * kCullWaves) void diff_cull_kernel(
     const VolumeParams vp, const int nframes, const FrameTable ft, ListEntry* __restrict__ list,
     unsigned* __restrict__ list_count) {
     __shared__ uint64_t s_live[kCullWaves][64];
     const uint32_t ntiles = (uint32_t)vp.tiles_x * vp.tiles_y * vp.tiles_z;
-    const int i = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int i = threadIdx.x & 63, g = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     const uint32_t id = blockIdx.x * 64u + i;
-    s_live[g][i] = id < ntiles ? tile_frames<Shape>(vp, nframes, ft, id, g, kCullWaves) : 0ull;
+    s_live[g][i] = id < ntiles ? tile_frames<Shape>(vp, nframes, ft, id, g, nw) : 0ull;
     __syncthreads();
     if (g != 0) return;
     uint64_t live = 0;
-#pragma unroll
-    for (int k = 0; k < kCullWaves; ++k) live |= s_live[k][i];
+    for (int k = 0; k < nw; ++k) live |= s_live[k][i];
     append_wave(live, id, 0u, list, list_count);
 }
 
@@ -701,8 +701,10 @@ extern "C" int recon_tsdf_diff_integrate(recon_tsdf* h, const recon_depth_view* 
                            frame_stride_bytes / 4, per, h->list_count);
         RECON_HIP_TRY(hipGetLastError(), "diff_bounds_kernel launch");
         const FrameTable ft{table + f0};
-        hipLaunchKernelGGL(diff_cull_kernel, dim3((ntiles + 63) / 64), dim3(64 * kCullWaves), 0, s,
-                           vp, n, ft, list, h->list_count);
+        const unsigned groups = (unsigned)((ntiles + 63) / 64);
+        hipLaunchKernelGGL(diff_cull_kernel, dim3(groups),
+                           dim3(64 * cull_waves(groups, n, h->num_cus)), 0, s, vp, n, ft, list,
+                           h->list_count);
         RECON_HIP_TRY(hipGetLastError(), "diff_cull_kernel launch");
         if (vp.gx % 4 == 0)
             hipLaunchKernelGGL(diff_integrate_kernel<true>, grid, dim3(kThreads), 0, s, vp, n, ft,
