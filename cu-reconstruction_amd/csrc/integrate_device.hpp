@@ -142,19 +142,24 @@ __device__ __forceinline__ bool bound_rect_reaches(const float* __restrict__ lvl
     return false;
 }
 
-// Depth-bound test of a voxel-centre box for one frame (one thread).
-// Returns false only when no voxel of the box can be updated by the frame:
-// the box's eight corners bound the camera z of its voxels and, when all lie
-// in front of the camera, the hull of their projections bounds the voxels'
-// pixel coordinates.  Two pixels of slack cover float32 rounding, the
-// hardware reciprocal used here (<= 1 ulp) and the truncation to an index.
-// A footprint of more than kMaxFine 8x8 blocks keeps the box after the
-// 32x32 test: the fine scan is a chain of dependent loads, and on a cull
-// launch that fits the device in one pass its longest chain is the kernel's
-// duration.
+// Cull test of a voxel-centre box for one frame (one thread).  Returns
+// false only when no voxel of the box can be updated by the frame.  The
+// box's eight corners bound the camera z of its voxels.  When all lie in
+// front of the camera (z > 1e-3), the hull of their projections bounds the
+// voxels' pixel coordinates (two pixels of slack cover float32 rounding, the
+// hardware reciprocal used here (<= 1 ulp) and the truncation to an index),
+// and the box survives when that footprint meets the image and the depth
+// bound of its blocks reaches the box's nearest z minus the truncation; this
+// also settles the frustum (a footprint outside the image fails, and a box
+// beyond max depth + truncation finds no bound that far).  Otherwise the
+// footprint is unbounded and the box survives when it meets the six frustum
+// half-spaces of F.plane.  A footprint of more than kMaxFine 8x8 blocks keeps
+// the box after the 32x32 test: the fine scan is a chain of dependent loads,
+// and on a cull launch that fits the device in one pass its longest chain is
+// the kernel's duration.
 template <int kMaxFine = 96>
-__device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const FrameParams& F,
-                                              const float lo[3], const float hi[3]) {
+__device__ __forceinline__ bool box_reaches(const VolumeParams& vp, const FrameParams& F,
+                                            const float lo[3], const float hi[3]) {
     float zmin = 3.0e38f, umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -170,8 +175,13 @@ __device__ __forceinline__ bool depth_reaches(const VolumeParams& vp, const Fram
         umin = fminf(umin, u); umax = fmaxf(umax, u);
         vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
     }
-    // corners at or behind the camera: the footprint is unbounded, keep
-    if (!(zmin > 1e-3f)) return true;
+    // corners at or behind the camera: only the frustum planes bound the box
+    if (!(zmin > 1e-3f)) {
+        bool in = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) in = in && (plane_max(F.plane[k], lo, hi) >= 0.0f);
+        return in;
+    }
     umin = fmaxf(umin - 2.0f, 0.0f); umax = fminf(umax + 2.0f, vp.fcols - 1.0f);
     vmin = fmaxf(vmin - 2.0f, 0.0f); vmax = fminf(vmax + 2.0f, vp.frows - 1.0f);
     if (!(umin <= umax && vmin <= vmax)) return false;  // footprint leaves the image
@@ -204,13 +214,8 @@ __device__ __forceinline__ uint64_t tile_frames(const VolumeParams& vp, const in
     hi[1] = voxel_centre(min(y0 + S::Y - 1, vp.gy - 1), vp.vs[1], vp.off[1]);
     hi[2] = voxel_centre(min(z0 + S::Z - 1, vp.gz - 1), vp.vs[2], vp.off[2]);
     uint64_t live = 0;
-    for (int f = first; f < nframes; f += step) {
-        const FrameParams& F = frames.f[f];
-        bool in = true;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) in = in && (plane_max(F.plane[k], lo, hi) >= 0.0f);
-        if (in && depth_reaches<kMaxFine>(vp, F, lo, hi)) live |= 1ull << f;
-    }
+    for (int f = first; f < nframes; f += step)
+        if (box_reaches<kMaxFine>(vp, frames.f[f], lo, hi)) live |= 1ull << f;
     return live;
 }
 
