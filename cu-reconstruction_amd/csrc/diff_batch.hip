@@ -138,6 +138,9 @@ __global__ __launch_bounds__(64 * kCullWaves) void dbatch_cull_kernel(
 __global__ __launch_bounds__(kThreads) void dbatch_integrate_kernel(
     const DVol* __restrict__ vols, const FrameParams* __restrict__ table,
     const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count) {
+    __shared__ float rtab[kRcpTable];
+    rtab[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
+    __syncthreads();
     const unsigned n = __builtin_amdgcn_readfirstlane(*list_count);
     const unsigned waves = gridDim.x * (kThreads / 64);
     for (unsigned i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < n; i += waves) {
@@ -151,9 +154,11 @@ __global__ __launch_bounds__(kThreads) void dbatch_integrate_kernel(
         tile_origin<Shape>(dv.vp, tile, x0, y0, z0);
         const FrameTable ft{table + dv.first_frame};
         if (dv.vp.gx % 4 == 0)
-            diff_integrate_tile<Shape, FrameTable, true>(dv.vp, dv.nframes, ft, mask, x0, y0, z0);
+            diff_integrate_tile<Shape, FrameTable, true>(dv.vp, dv.nframes, ft, mask, x0, y0, z0,
+                                                         rtab);
         else
-            diff_integrate_tile<Shape, FrameTable, false>(dv.vp, dv.nframes, ft, mask, x0, y0, z0);
+            diff_integrate_tile<Shape, FrameTable, false>(dv.vp, dv.nframes, ft, mask, x0, y0, z0,
+                                                          rtab);
     }
 }
 

@@ -36,8 +36,12 @@ __device__ __forceinline__ BiProj bi_project(const VolumeParams& vp, const Frame
     s.px = ((F.R[0] * cx + F.R[1] * cy) + F.R[2] * cz) + F.t[0];
     s.py = ((F.R[3] * cx + F.R[4] * cy) + F.R[5] * cz) + F.t[1];
     s.pz = ((F.R[6] * cx + F.R[7] * cy) + F.R[8] * cz) + F.t[2];
-    const float su = ((s.px * vp.fx) / s.pz) + vp.cx;
-    const float sv = ((s.py * vp.fy) / s.pz) + vp.cy;
+    // one correctly rounded 1/z serves both quotients (div_by_rcp equals the
+    // division for normal-range quotients; outside that range both differ
+    // from cx, cy by less than half their ulp or are invalid alike)
+    const float rz = 1.0f / s.pz;
+    const float su = div_by_rcp(s.px * vp.fx, s.pz, rz) + vp.cx;
+    const float sv = div_by_rcp(s.py * vp.fy, s.pz, rz) + vp.cy;
     s.in = s.pz > 0.0f && su >= 0.0f && su < vp.fcols - 1.0f && sv >= 0.0f && sv < vp.frows - 1.0f;
     const int i0 = s.in ? (int)su : 0, j0 = s.in ? (int)sv : 0;
     s.a = su - (float)i0;
@@ -62,7 +66,8 @@ __device__ __forceinline__ float bi_depth(float a, float b, float d00, float d10
 template <class S, class Frames, bool VEC>
 __device__ __forceinline__ void diff_integrate_tile(const VolumeParams& vp, const int nframes,
                                                     const Frames& frames, const uint64_t live,
-                                                    const int x0, const int y0, const int z0) {
+                                                    const int x0, const int y0, const int z0,
+                                                    const float* __restrict__ rtab = nullptr) {
     constexpr int NR = S::kRows, NV = 4 * S::kRows;
     float* __restrict__ D = vp.D;
     float* __restrict__ W = vp.W;
@@ -107,7 +112,9 @@ __device__ __forceinline__ void diff_integrate_tile(const VolumeParams& vp, cons
         float t00[NV], t10[NV], t01[NV], t11[NV];
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            const float* q = F.depth + pr[j].addr;
+            // the depth maps are device memory: global loads, not flat ones
+            const __attribute__((address_space(1))) float* q =
+                (const __attribute__((address_space(1))) float*)(F.depth + pr[j].addr);
             t00[j] = q[0];
             t10[j] = q[1];
             t01[j] = q[F.pitch];
@@ -147,7 +154,13 @@ __device__ __forceinline__ void diff_integrate_tile(const VolumeParams& vp, cons
         for (int j = 0; j < NV; ++j) {
             if (!pass[j]) continue;
             const float nw = w[j] + 1.0f;
-            d[j] = ((d[j] * w[j]) + tsdf[j]) / nw;
+            const float num = (d[j] * w[j]) + tsdf[j];
+            // whole weights below kRcpTable: RN(1/nw) from rtab and div_by_rcp
+            const int wi = (int)w[j];
+            if (rtab && (float)wi == w[j] && wi < kRcpTable)
+                d[j] = div_by_rcp(num, nw, rtab[wi]);
+            else
+                d[j] = num / nw;
             w[j] = nw < vp.maxw ? nw : vp.maxw;
             dirty[j >> 2] = true;
         }

@@ -98,6 +98,9 @@ template <bool VEC>
 __global__ __launch_bounds__(kThreads) void diff_integrate_kernel(
     const VolumeParams vp, const int nframes, const FrameTable ft,
     const ListEntry* __restrict__ list, const unsigned* __restrict__ list_count) {
+    __shared__ float rtab[kRcpTable];
+    rtab[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);
+    __syncthreads();
     const unsigned n = __builtin_amdgcn_readfirstlane(*list_count);
     const unsigned waves = gridDim.x * (kThreads / 64);
     for (unsigned i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); i < n; i += waves) {
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void diff_integrate_kernel(
             __builtin_amdgcn_readfirstlane((uint32_t)list[i].frames);
         int x0, y0, z0;
         tile_origin<Shape>(vp, tile, x0, y0, z0);
-        diff_integrate_tile<Shape, FrameTable, VEC>(vp, nframes, ft, frames, x0, y0, z0);
+        diff_integrate_tile<Shape, FrameTable, VEC>(vp, nframes, ft, frames, x0, y0, z0, rtab);
     }
 }
 
