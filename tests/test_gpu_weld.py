@@ -78,14 +78,21 @@ def test_counts_capacity_and_degenerate(cuda):
                                        C.byref(nv), C.byref(nt), None) == 0
     ov, _, ot = O.mc_weld(f, (8, 7, 6), (1, 1, 1), (0, 0, 0))
     assert (nv.value, nt.value) == (ov.shape[0], ot.shape[0])
-    buf = cuda.empty((nv.value, 3), dtype=cuda.float32, device="cuda")
-    tri = cuda.empty((nt.value, 3), dtype=cuda.int32, device="cuda")
+    buf = cuda.full((nv.value, 3), -7.0, dtype=cuda.float32, device="cuda")
+    tri = cuda.full((nt.value, 3), -7, dtype=cuda.int32, device="cuda")
     st = lib.recon_mc_extract_welded(dev.data_ptr(), g, vs, off, buf.data_ptr(), None,
                                      nv.value - 1, None, 0, C.byref(nv), C.byref(nt), None)
     assert st == ffi.ERR_CAPACITY
     st = lib.recon_mc_extract_welded(dev.data_ptr(), g, vs, off, None, None, 0, tri.data_ptr(),
                                      nt.value - 1, C.byref(nv), C.byref(nt), None)
     assert st == ffi.ERR_CAPACITY
+    # one output too small: the other, large enough, is not written either
+    st = lib.recon_mc_extract_welded(dev.data_ptr(), g, vs, off, buf.data_ptr(), None, nv.value,
+                                     tri.data_ptr(), nt.value - 1, C.byref(nv), C.byref(nt), None)
+    assert st == ffi.ERR_CAPACITY
+    assert (nv.value, nt.value) == (ov.shape[0], ot.shape[0])
+    cuda.cuda.synchronize()
+    assert (buf.cpu().numpy() == -7.0).all() and (tri.cpu().numpy() == -7).all()
     # triangles only
     assert lib.recon_mc_extract_welded(dev.data_ptr(), g, vs, off, None, None, 0, tri.data_ptr(),
                                        nt.value, C.byref(nv), C.byref(nt), None) == 0
