@@ -262,8 +262,8 @@ __device__ __forceinline__ void append_wave(uint64_t live, uint32_t tile, uint32
 }
 
 // Most wavefronts per workgroup of the frame-parallel culls: with W =
-// blockDim.x / 64 of them (W = min(kCullWaves, frames)), wave g of a
-// workgroup tests frames g, g + W, ... of the workgroup's 64 tiles.
+// blockDim.x / 64 of them (cull_waves), wave g of a workgroup tests frames
+// g, g + W, ... of the workgroup's 64 tiles.
 constexpr int kCullWaves = 16;
 
 // W for `groups` workgroups of 64 tiles and at most `frames` frames a tile:
